@@ -1,0 +1,11 @@
+"""hmglib_amd -- MI355X-native H-matrix engine (drop-in for hmglib's C API).
+
+The product is the shared library libhmglib.so built from csrc/ (HIP kernels for
+gfx950 behind the C ABI of include/hmglib.h).  This package is its Python
+front-end: a ctypes view of the ABI, device-buffer helpers and a thin wrapper
+that drives the ABI like the reference's drivers do.  Importing it fails loudly
+when the library is missing; there is no CPU fallback.
+"""
+from ._abi import (HMG_OK, KERNEL_EXPONENTIAL, KERNEL_GAUSSIAN, KERNEL_GAUSSIAN_REF,  # noqa: F401
+                   KERNEL_MATERN_REF, WT_ACA, WT_DENSE, HMGError, HMatrixData, WorkItem, lib)
+from .hmatrix import HMatrix, Kernel, rng_normal, rng_uniform  # noqa: F401

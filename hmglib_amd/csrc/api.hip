@@ -1,0 +1,852 @@
+// api.hip -- the C ABI: hmglib's entry points (hmglib.h:104-120,
+// kernel_system_assembler.h:66-102) plus hmg_* status twins and extensions.
+// Host orchestration only; every computation runs in the HIP kernels of
+// prims/morton/tree/aca/mvp.hip on the engine's stream.  All calls are
+// synchronous, as in the reference.
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <cstddef>
+
+#include "hmg_internal.h"
+
+namespace hmg {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+Engine* engine_of(const h_matrix_data* d) { return d ? (Engine*)d->engine : nullptr; }
+
+void Profiler::begin(int cls, hipStream_t s)
+{
+    if (!on) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, s);
+    ev[cls].push_back(e);
+}
+
+void Profiler::end(int cls, hipStream_t s)
+{
+    if (!on || ev[cls].size() % 2 == 0) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, s);
+    ev[cls].push_back(e);
+}
+
+void Profiler::clear()
+{
+    for (auto& v : ev) {
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
+        v.clear();
+    }
+}
+
+KernelParams make_kernel_params(const system_assembler& a, int dim)
+{
+    KernelParams kp{};
+    kp.kind = a.kind;
+    kp.dim = dim;
+    kp.reg = a.regularization;
+    if (a.kind == HMG_KERNEL_GAUSSIAN) kp.c0 = 1.0 / (2.0 * a.length_scale * a.length_scale);
+    if (a.kind == HMG_KERNEL_EXPONENTIAL) kp.c0 = 1.0 / a.length_scale;
+    kp.c1 = std::pow(2.0, (double)dim / 2.0) * std::tgamma(1.0 + (double)dim / 2.0);
+    return kp;
+}
+
+namespace {
+
+Coords coords_of(const h_matrix_data* d, int set)
+{
+    Coords c{};
+    for (int k = 0; k < d->dim && k < MAX_POINT_DIMENSION; k++) c.x[k] = d->coords_d[set][k];
+    c.n = d->point_count[set];
+    return c;
+}
+
+hmg_status read_kernel(const h_matrix_data* d, KernelParams* kp)
+{
+    if (!d->assem) {
+        set_error("h_matrix_data.assem is not set (create a kernel object first)");
+        return HMG_ERR_NOT_READY;
+    }
+    system_assembler a{};
+    HMG_HIP(hipMemcpy(&a, d->assem, sizeof(a), hipMemcpyDeviceToHost));
+    if (a.kind < HMG_KERNEL_GAUSSIAN_REF || a.kind > HMG_KERNEL_EXPONENTIAL) {
+        set_error("unknown kernel kind " + std::to_string(a.kind));
+        return HMG_ERR_INVALID;
+    }
+    if ((a.kind == HMG_KERNEL_GAUSSIAN || a.kind == HMG_KERNEL_EXPONENTIAL) && !(a.length_scale > 0)) {
+        set_error("kernel length scale must be positive");
+        return HMG_ERR_INVALID;
+    }
+    *kp = make_kernel_params(a, d->dim);
+    return HMG_OK;
+}
+
+hmg_status check_ready(const h_matrix_data* d, Engine** out)
+{
+    Engine* e = engine_of(d);
+    if (!e) {
+        set_error("h_matrix_data not initialised (call init_h_matrix_data)");
+        return HMG_ERR_NOT_READY;
+    }
+    if (!e->built) {
+        set_error("H-matrix not set up (call setup_h_matrix)");
+        return HMG_ERR_NOT_READY;
+    }
+    *out = e;
+    return HMG_OK;
+}
+
+// compute_current_dense_work_size_magma (linear_algebra.cu:4059-4108), int64 inside
+int dense_work_size_at(const std::vector<int4>& bl, int ndense, int64_t max_size, int start)
+{
+    int cnt = 0;
+    int64_t size = 0;
+    while (size <= max_size && start + cnt < ndense) {
+        const int4 b = bl[start + cnt];
+        size += (int64_t)(b.y - b.x + 1) * (b.w - b.z + 1);
+        cnt++;
+    }
+    if (size > max_size) cnt--;
+    return cnt;
+}
+
+// compute_current_aca_work_size (linear_algebra.cu:4158-4174)
+int aca_work_size_at(const std::vector<int4>& bl, int nblocks, int64_t max_size, int start)
+{
+    int cnt = 0;
+    int64_t size = 0;
+    while (size <= max_size && start + cnt < nblocks) {
+        const int4 b = bl[start + cnt];
+        size += (int64_t)(b.y - b.x + 1);
+        cnt++;
+    }
+    if (size > max_size) cnt--;
+    return cnt;
+}
+
+void release_factors(Engine& e, h_matrix_data* d)
+{
+    e.U.release();
+    e.V.release();
+    e.aca_k = -1;
+    d->U = nullptr;
+    d->V = nullptr;
+}
+
+void release_build(Engine& e, h_matrix_data* d)
+{
+    release_factors(e, d);
+    e.A.release();
+    e.aoff.release();
+    e.dense_stored = false;
+    d->dA = nullptr;
+    e.blocks.release();
+    e.blk.release();
+    e.h_blk.clear();
+    e.kb.release();
+    e.keff.release();
+    e.uoff.release();
+    e.voff.release();
+    e.tile_ptr.release();
+    e.tile_blk.release();
+    e.toff.release();
+    e.h_toff.clear();
+    e.xs.release();
+    e.ys.release();
+    e.T.release();
+    e.built = false;
+    if (d->mat_vec_data) {
+        free(d->mat_vec_data);
+        d->mat_vec_data = nullptr;
+    }
+    free(d->dense_work_size);
+    free(d->aca_work_size);
+    d->dense_work_size = nullptr;
+    d->aca_work_size = nullptr;
+}
+
+// ACA factors into (U, V, keff) for the current block list and rank k
+hmg_status build_factors(Engine& e, h_matrix_data* d, const KernelParams& kp, int k, int tie_last,
+                         DevBuf<double>& U, DevBuf<double>& V)
+{
+    HMG_TRY(aca_layout(e, k));
+    const int na = e.naca;
+    HMG_TRY(U.alloc(std::max<int64_t>(e.h_uoff[na], 1)));
+    HMG_TRY(V.alloc(std::max<int64_t>(e.h_voff[na], 1)));
+    HMG_HIP(hipMemsetAsync(U.p, 0, sizeof(double) * U.n, e.stream));
+    HMG_HIP(hipMemsetAsync(V.p, 0, sizeof(double) * V.n, e.stream));
+    HMG_TRY(aca_build(e, coords_of(d, 0), coords_of(d, 1), kp, k, tie_last, U.p, V.p, e.keff.p));
+    HMG_HIP(hipStreamSynchronize(e.stream));
+    return HMG_OK;
+}
+
+}  // namespace
+}  // namespace hmg
+
+using namespace hmg;
+
+extern "C" {
+
+const char* hmg_last_error(void) { return g_error.c_str(); }
+
+// ------------------------------------------------------------------ init
+hmg_status hmg_init_h_matrix_data(struct h_matrix_data* data, int point_count[2], int dim, int bits)
+{
+    if (!data || !point_count) {
+        set_error("null argument");
+        return HMG_ERR_INVALID;
+    }
+    if (dim < 1 || dim > MAX_POINT_DIMENSION) {
+        set_error("dim must be in 1..5 (MAX_POINT_DIMENSION)");
+        return HMG_ERR_INVALID;
+    }
+    if (point_count[0] < 1 || point_count[1] < 1) {
+        set_error("point counts must be positive");
+        return HMG_ERR_INVALID;
+    }
+    if (bits < 1 || bits > 64) {
+        set_error("bits must be in 1..64");
+        return HMG_ERR_INVALID;
+    }
+    Engine* e = new Engine();
+    HMG_HIP(hipStreamCreate(&e->stream));
+    e->dim = dim;
+    e->bits = bits;
+    data->engine = e;
+    data->dim = dim;
+    data->bits = bits;
+    data->magma_queue = (void*)e->stream;
+    data->U = data->V = data->dA = nullptr;
+    data->mat_vec_data = nullptr;
+    data->mat_vec_data_count = 0;
+    data->mat_vec_data_array_size = 0;
+    data->dense_work_size = data->aca_work_size = nullptr;
+    data->dense_batch_count = data->aca_batch_count = 0;
+    data->root_level_set_1 = 0;
+    data->root_level_set_2 = 0;
+    data->assem = nullptr;
+    data->mat_vec_info = mat_vec_data_info{0, 0, 0};
+    for (int i = 0; i < 2; i++) {
+        const int n = point_count[i];
+        e->n[i] = n;
+        data->point_count[i] = n;
+        data->coords_d[i] = (double**)calloc(dim, sizeof(double*));
+        for (int d = 0; d < dim; d++)
+            HMG_HIP(hipMalloc((void**)&data->coords_d[i][d], sizeof(double) * n));
+        HMG_HIP(hipMalloc((void**)&data->point_ids_d[i], sizeof(unsigned) * n));
+        HMG_HIP(hipMalloc((void**)&data->max_per_dim_d[i], sizeof(double) * dim));
+        HMG_HIP(hipMalloc((void**)&data->min_per_dim_d[i], sizeof(double) * dim));
+        HMG_HIP(hipMalloc((void**)&data->coords_device[i], sizeof(double*) * dim));
+        HMG_HIP(hipMemcpy(data->coords_device[i], data->coords_d[i], sizeof(double*) * dim,
+                          hipMemcpyHostToDevice));
+        HMG_HIP(hipMalloc((void**)&data->code_d[i], sizeof(uint64_t) * n));
+        HMG_HIP(hipMalloc((void**)&data->order[i], sizeof(uint64_t) * n));
+        point_set ps{data->coords_device[i], dim, n, data->max_per_dim_d[i], data->min_per_dim_d[i],
+                     data->point_ids_d[i]};
+        HMG_HIP(hipMalloc((void**)&data->points_d[i], sizeof(point_set)));
+        HMG_HIP(hipMemcpy(data->points_d[i], &ps, sizeof(ps), hipMemcpyHostToDevice));
+        morton_code mc{data->code_d[i], n, dim, bits, nullptr, nullptr};
+        HMG_HIP(hipMalloc((void**)&data->morton_d[i], sizeof(morton_code)));
+        HMG_HIP(hipMemcpy(data->morton_d[i], &mc, sizeof(mc), hipMemcpyHostToDevice));
+    }
+    return HMG_OK;
+}
+
+// ------------------------------------------------------------------ setup
+hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
+{
+    Engine* e = engine_of(data);
+    if (!e) {
+        set_error("h_matrix_data not initialised (call init_h_matrix_data)");
+        return HMG_ERR_NOT_READY;
+    }
+    if (data->max_level < 0 || data->c_leaf < 0) {
+        set_error("max_level and c_leaf must be non-negative");
+        return HMG_ERR_INVALID;
+    }
+    release_build(*e, data);
+    hipStream_t s = e->stream;
+    const int dim = data->dim;
+    const bool spread_defined = (dim == 3 && data->bits == 20) || (dim == 2 && data->bits == 32) ||
+                                (dim == 10 && data->bits == 4);
+    const int key_bits = spread_defined ? std::min(64, dim * data->bits) : 8;
+    for (int i = 0; i < 2; i++) {
+        const int n = data->point_count[i];
+        // extremal values (helper.cu:108-138)
+        for (int d = 0; d < dim; d++)
+            HMG_TRY(minmax_f64(data->coords_d[i][d], n, data->min_per_dim_d[i] + d,
+                               data->max_per_dim_d[i] + d, s));
+        double mn[MAX_POINT_DIMENSION], mx[MAX_POINT_DIMENSION];
+        HMG_HIP(hipMemcpyAsync(mn, data->min_per_dim_d[i], sizeof(double) * dim, hipMemcpyDeviceToHost, s));
+        HMG_HIP(hipMemcpyAsync(mx, data->max_per_dim_d[i], sizeof(double) * dim, hipMemcpyDeviceToHost, s));
+        HMG_HIP(hipStreamSynchronize(s));
+        for (int d = 0; d < dim; d++) {
+            if (!(mx[d] > mn[d])) {
+                set_error("point set " + std::to_string(i) + " has no extent in dimension " +
+                          std::to_string(d) + " (Morton quantisation would divide by zero)");
+                return HMG_ERR_INVALID;
+            }
+        }
+        // Morton codes (morton.cu:127-155), stable sort (helper.cu:140-167)
+        Coords c = coords_of(data, i);
+        HMG_TRY(morton_codes(c, dim, data->bits, data->min_per_dim_d[i], data->max_per_dim_d[i],
+                             data->code_d[i], s));
+        HMG_TRY(e->perm32[i].alloc(n));
+        HMG_TRY(iota_u32(e->perm32[i].p, n, s));
+        HMG_TRY(radix_sort_u64(data->code_d[i], e->perm32[i].p, n, key_bits, s));
+        HMG_TRY(perm_to_u64(e->perm32[i].p, data->order[i], data->point_ids_d[i], n, s));
+        // reorder_point_set (helper.cu:293-328): gather into a temporary, copy back
+        DevBuf<double> tmp;
+        HMG_TRY(tmp.alloc(n));
+        for (int d = 0; d < dim; d++) {
+            HMG_TRY(gather_f64(data->coords_d[i][d], e->perm32[i].p, tmp.p, n, s));
+            HMG_HIP(hipMemcpyAsync(data->coords_d[i][d], tmp.p, sizeof(double) * n,
+                                   hipMemcpyDeviceToDevice, s));
+        }
+        HMG_HIP(hipStreamSynchronize(s));
+        tmp.release();
+        e->perm32[i].release();
+    }
+    // block-cluster tree (tree.cu:1429-1557) and DENSE-first partition
+    TreeParams tp{data->eta, data->max_level, data->c_leaf, (int)data->root_level_set_1,
+                  (int)data->root_level_set_2};
+    HMG_TRY(build_tree(*e, coords_of(data, 0), coords_of(data, 1), data->code_d[0], data->code_d[1], tp));
+    data->mat_vec_data = (work_item**)malloc(sizeof(work_item*));
+    *data->mat_vec_data = e->blocks.p;
+    data->mat_vec_data_count = e->nblocks;
+    data->mat_vec_data_array_size = e->nblocks;
+    data->max_elements_in_array = -1;
+    data->max_elements_in_mat_vec_data_array = -1;
+    data->mat_vec_info.dense_count = e->ndense;
+    data->mat_vec_info.aca_count = e->naca;
+    data->mat_vec_info.total_count = e->nblocks;
+
+    // batch sizing with the reference's semantics (linear_algebra.cu:4176-4288,
+    // hmglib.cu:211-219), computed in 64-bit and clamped into the int fields
+    int64_t required = 0;
+    for (int b = 0; b < e->ndense; b++) {
+        const int4 bl = e->h_blk[b];
+        required += (int64_t)(bl.y - bl.x + 1) * (bl.w - bl.z + 1);
+    }
+    const int64_t predicted = (int64_t)((double)required * 1.05);
+    data->max_batched_dense_size = (int)std::min<int64_t>(predicted, INT_MAX);
+    std::vector<int> dws, aws;
+    {
+        int idx = 0;
+        int w = dense_work_size_at(e->h_blk, e->ndense, predicted, idx);
+        dws.push_back(w);
+        while (w > 0) {
+            idx += w;
+            w = dense_work_size_at(e->h_blk, e->ndense, predicted, idx);
+            dws.push_back(w);
+        }
+        idx = e->ndense;
+        const int64_t amax = data->max_batched_aca_size;
+        w = aca_work_size_at(e->h_blk, e->nblocks, amax, idx);
+        aws.push_back(w);
+        while (w > 0) {
+            idx += w;
+            w = aca_work_size_at(e->h_blk, e->nblocks, amax, idx);
+            aws.push_back(w);
+        }
+    }
+    data->dense_batch_count = (int)dws.size();
+    data->aca_batch_count = (int)aws.size();
+    data->dense_work_size = (int*)malloc(sizeof(int) * dws.size());
+    data->aca_work_size = (int*)malloc(sizeof(int) * aws.size());
+    memcpy(data->dense_work_size, dws.data(), sizeof(int) * dws.size());
+    memcpy(data->aca_work_size, aws.data(), sizeof(int) * aws.size());
+
+    // rank layout for k (k_b = min(k,m1,m2)) and the MVP schedule
+    HMG_TRY(aca_layout(*e, std::max(data->k, 0)));
+    HMG_TRY(build_schedule(*e));
+    e->built = true;
+    return HMG_OK;
+}
+
+// ------------------------------------------------------------------ precompute
+hmg_status hmg_precompute_aca(struct h_matrix_data* data)
+{
+    Engine* e;
+    HMG_TRY(check_ready(data, &e));
+    KernelParams kp;
+    HMG_TRY(read_kernel(data, &kp));
+    release_factors(*e, data);
+    // The reference skips the whole far field when the first ACA block exceeds
+    // max_batched_aca_size (hmglib.cu:232, linear_algebra.cu:4163-4171); the
+    // engine has no batch cap and always builds every ACA block.
+    HMG_TRY(build_factors(*e, data, kp, std::max(data->k, 0), 1, e->U, e->V));
+    data->U = e->U.p;
+    data->V = e->V.p;
+    return HMG_OK;
+}
+
+hmg_status hmg_precompute_dense(struct h_matrix_data* data)
+{
+    Engine* e;
+    HMG_TRY(check_ready(data, &e));
+    KernelParams kp;
+    HMG_TRY(read_kernel(data, &kp));
+    int64_t entries = 0;
+    for (int b = 0; b < e->ndense; b++) {
+        const int4 bl = e->h_blk[b];
+        entries += (int64_t)(bl.y - bl.x + 1) * (bl.w - bl.z + 1);
+    }
+    size_t free_b = 0, total_b = 0;
+    HMG_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double need = 8.0 * (double)entries;
+    if (need > (double)free_b * 0.95) {
+        set_error("dense blocks need " + std::to_string(need / 1e9) + " GB but only " +
+                  std::to_string(free_b / 1e9) + " GB of device memory are free; "
+                  "precomputing the dense blocks is impossible (apply evaluates them on the fly)");
+        return HMG_ERR_OOM;
+    }
+    e->A.release();
+    HMG_TRY(dense_fill(*e, coords_of(data, 0), coords_of(data, 1), kp));
+    data->dA = e->A.p;
+    return HMG_OK;
+}
+
+// ------------------------------------------------------------------ apply
+hmg_status hmg_apply_h_matrix_mvp_multi(const double* X, double* Y, int nrhs, struct h_matrix_data* data)
+{
+    Engine* e;
+    HMG_TRY(check_ready(data, &e));
+    if (!X || !Y || nrhs < 1) {
+        set_error("invalid x / y / nrhs");
+        return HMG_ERR_INVALID;
+    }
+    KernelParams kp;
+    HMG_TRY(read_kernel(data, &kp));
+    const bool pre_aca = data->U != nullptr && e->aca_k >= 0;
+    const bool pre_dense = data->dA != nullptr && e->dense_stored;
+    DevBuf<double> tU, tV;
+    const double* U = e->U.p;
+    const double* V = e->V.p;
+    if (!pre_aca) {
+        // no precomputed factors: rebuild the ACA for this product (h_matrix_mvp
+        // without precompute, linear_algebra.cu:4359-4370)
+        HMG_TRY(build_factors(*e, data, kp, std::max(data->k, 0), 1, tU, tV));
+        U = tU.p;
+        V = tV.p;
+    }
+    HMG_TRY(hmvp(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[0], data->order[1], X, Y,
+                 nrhs, U, V, e->keff.p, pre_dense));
+    HMG_HIP(hipStreamSynchronize(e->stream));
+    tU.release();
+    tV.release();
+    if (!pre_aca) e->aca_k = -1;
+    return HMG_OK;
+}
+
+hmg_status hmg_apply_h_matrix_mvp(double* x, double* y, struct h_matrix_data* data)
+{
+    return hmg_apply_h_matrix_mvp_multi(x, y, 1, data);
+}
+
+hmg_status hmg_apply_h_matrix_mvp_without_batching(double* x, double* y, struct h_matrix_data* data)
+{
+    Engine* e;
+    HMG_TRY(check_ready(data, &e));
+    if (!x || !y) {
+        set_error("invalid x / y");
+        return HMG_ERR_INVALID;
+    }
+    KernelParams kp;
+    HMG_TRY(read_kernel(data, &kp));
+    // ignores precomputed data (hmglib.cu:294); unbatched pivot rule = first argmax
+    DevBuf<double> tU, tV;
+    const bool had = data->U != nullptr;
+    std::vector<int> keep_keff;
+    if (had && e->naca > 0) {
+        keep_keff.resize(e->naca);
+        HMG_HIP(hipMemcpy(keep_keff.data(), e->keff.p, sizeof(int) * e->naca, hipMemcpyDeviceToHost));
+    }
+    const int had_k = e->aca_k;
+    HMG_TRY(build_factors(*e, data, kp, std::max(data->k, 0), 0, tU, tV));
+    HMG_TRY(hmvp(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[0], data->order[1], x, y,
+                 1, tU.p, tV.p, e->keff.p, false));
+    HMG_HIP(hipStreamSynchronize(e->stream));
+    tU.release();
+    tV.release();
+    if (had) {
+        // restore the stored factors' layout and ranks
+        HMG_TRY(aca_layout(*e, had_k));
+        if (e->naca > 0)
+            HMG_HIP(hipMemcpy(e->keff.p, keep_keff.data(), sizeof(int) * e->naca, hipMemcpyHostToDevice));
+    } else {
+        e->aca_k = -1;
+    }
+    return HMG_OK;
+}
+
+hmg_status hmg_apply_full_mvp(double* x, double* y, struct h_matrix_data* data)
+{
+    Engine* e;
+    HMG_TRY(check_ready(data, &e));
+    KernelParams kp;
+    HMG_TRY(read_kernel(data, &kp));
+    HMG_TRY(full_mvp(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[0], data->order[1],
+                     x, y, 1));
+    HMG_HIP(hipStreamSynchronize(e->stream));
+    return HMG_OK;
+}
+
+hmg_status hmg_dense_rows(const double* X, int nrhs, const int64_t* rows, int nrows, double* Yrows,
+                          struct h_matrix_data* data)
+{
+    Engine* e;
+    HMG_TRY(check_ready(data, &e));
+    if (nrows <= 0) return HMG_OK;
+    KernelParams kp;
+    HMG_TRY(read_kernel(data, &kp));
+    DevBuf<int64_t> inv;
+    HMG_TRY(inv.alloc(data->point_count[0]));
+    HMG_TRY(inverse_perm(data->order[0], inv.p, data->point_count[0], e->stream));
+    HMG_TRY(dense_rows(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[1], X, nrhs, rows,
+                       nrows, Yrows, inv.p));
+    HMG_HIP(hipStreamSynchronize(e->stream));
+    inv.release();
+    return HMG_OK;
+}
+
+hmg_status hmg_set_gaussian_kernel_rhs(double* b, struct h_matrix_data* data)
+{
+    Engine* e;
+    HMG_TRY(check_ready(data, &e));
+    HMG_TRY(gaussian_rhs(*e, coords_of(data, 0), data->order[0], b));
+    HMG_HIP(hipStreamSynchronize(e->stream));
+    return HMG_OK;
+}
+
+hmg_status hmg_get_work(const struct h_matrix_data* data, int64_t out[9])
+{
+    Engine* e;
+    HMG_TRY(check_ready(data, &e));
+    int64_t en = 0, el1 = 0, el2 = 0, rsum = 0;
+    for (int b = 0; b < e->ndense; b++) {
+        const int4 bl = e->h_blk[b];
+        en += (int64_t)(bl.y - bl.x + 1) * (bl.w - bl.z + 1);
+    }
+    std::vector<int> ranks(e->h_kb);
+    if (data->U && e->aca_k >= 0 && e->naca > 0)
+        HMG_HIP(hipMemcpy(ranks.data(), e->keff.p, sizeof(int) * e->naca, hipMemcpyDeviceToHost));
+    for (int a = 0; a < e->naca; a++) {
+        const int4 bl = e->h_blk[e->ndense + a];
+        el1 += (int64_t)(bl.y - bl.x + 1) * ranks[a];
+        el2 += (int64_t)(bl.w - bl.z + 1) * ranks[a];
+        rsum += ranks[a];
+    }
+    out[0] = en;
+    out[1] = el1 + el2;
+    out[2] = e->ndense;
+    out[3] = e->naca;
+    out[4] = e->levels;
+    out[5] = e->quirk_levels;
+    out[6] = el1;
+    out[7] = el2;
+    out[8] = rsum;
+    return HMG_OK;
+}
+
+hmg_status hmg_set_profiling(struct h_matrix_data* data, int enable)
+{
+    Engine* e = engine_of(data);
+    if (!e) {
+        set_error("h_matrix_data not initialised");
+        return HMG_ERR_NOT_READY;
+    }
+    HMG_HIP(hipStreamSynchronize(e->stream));
+    e->prof.clear();
+    e->prof.on = enable != 0;
+    return HMG_OK;
+}
+
+hmg_status hmg_get_profile(const struct h_matrix_data* data, int kernel_class, double* total_ms,
+                           int64_t* launches)
+{
+    Engine* e = engine_of(data);
+    if (!e) {
+        set_error("h_matrix_data not initialised");
+        return HMG_ERR_NOT_READY;
+    }
+    if (kernel_class < 0 || kernel_class >= PROF_NCLASS) {
+        set_error("unknown kernel class");
+        return HMG_ERR_INVALID;
+    }
+    HMG_HIP(hipStreamSynchronize(e->stream));
+    const auto& v = e->prof.ev[kernel_class];
+    double tot = 0.0;
+    int64_t n = 0;
+    for (size_t i = 0; i + 1 < v.size(); i += 2) {
+        float ms = 0.f;
+        HMG_HIP(hipEventElapsedTime(&ms, v[i], v[i + 1]));
+        tot += ms;
+        n++;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = n;
+    return HMG_OK;
+}
+
+// ------------------------------------------------------------------ destroy
+hmg_status hmg_destroy_h_matrix_data(struct h_matrix_data* data)
+{
+    Engine* e = engine_of(data);
+    if (!e) return HMG_OK;
+    release_build(*e, data);
+    for (int i = 0; i < 2; i++) {
+        for (int d = 0; d < data->dim; d++) (void)hipFree(data->coords_d[i][d]);
+        free(data->coords_d[i]);
+        data->coords_d[i] = nullptr;
+        (void)hipFree(data->point_ids_d[i]);
+        (void)hipFree(data->max_per_dim_d[i]);
+        (void)hipFree(data->min_per_dim_d[i]);
+        (void)hipFree(data->coords_device[i]);
+        (void)hipFree(data->code_d[i]);
+        (void)hipFree(data->order[i]);
+        (void)hipFree(data->points_d[i]);
+        (void)hipFree(data->morton_d[i]);
+        data->point_ids_d[i] = nullptr;
+        data->max_per_dim_d[i] = data->min_per_dim_d[i] = nullptr;
+        data->coords_device[i] = nullptr;
+        data->code_d[i] = nullptr;
+        data->order[i] = nullptr;
+        data->points_d[i] = nullptr;
+        data->morton_d[i] = nullptr;
+        e->perm32[i].release();
+    }
+    e->prof.clear();
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+    data->engine = nullptr;
+    data->magma_queue = nullptr;
+    return HMG_OK;
+}
+
+// ------------------------------------------------------------------ kernels
+int hmg_create_kernel_object(struct system_assembler** assem, int kind, double regularization,
+                             double length_scale)
+{
+    if (!assem) {
+        set_error("null slot");
+        return HMG_ERR_INVALID;
+    }
+    system_assembler* dptr = hmg_kernel_new(kind, regularization, length_scale);
+    if (!dptr) return HMG_ERR_HIP;
+    HMG_HIP(hipMemcpy(assem, &dptr, sizeof(dptr), hipMemcpyHostToDevice));
+    return HMG_OK;
+}
+
+int hmg_destroy_kernel_object(struct system_assembler** assem)
+{
+    if (!assem) return HMG_OK;
+    system_assembler* dptr = nullptr;
+    HMG_HIP(hipMemcpy(&dptr, assem, sizeof(dptr), hipMemcpyDeviceToHost));
+    hmg_kernel_free(dptr);
+    return HMG_OK;
+}
+
+struct system_assembler* hmg_kernel_new(int kind, double regularization, double length_scale)
+{
+    if (kind < HMG_KERNEL_GAUSSIAN_REF || kind > HMG_KERNEL_EXPONENTIAL) {
+        set_error("unknown kernel kind");
+        return nullptr;
+    }
+    system_assembler h{};
+    h.kind = kind;
+    h.regularization = regularization;
+    h.length_scale = length_scale;
+    system_assembler* dptr = nullptr;
+    if (hipMalloc((void**)&dptr, sizeof(h)) != hipSuccess) {
+        set_error("hipMalloc of kernel object failed");
+        return nullptr;
+    }
+    if (hipMemcpy(dptr, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(dptr);
+        set_error("hipMemcpy of kernel object failed");
+        return nullptr;
+    }
+    return dptr;
+}
+
+void hmg_kernel_free(struct system_assembler* assem)
+{
+    if (assem) (void)hipFree(assem);
+}
+
+// ------------------------------------------------------------------ layout
+int64_t hmg_struct_layout(int which, int field)
+{
+#define OFF(T, f) (int64_t)offsetof(T, f)
+    switch (which) {
+    case 0: {
+        const int64_t offs[] = {
+            OFF(h_matrix_data, coords_d), OFF(h_matrix_data, point_ids_d),
+            OFF(h_matrix_data, max_per_dim_d), OFF(h_matrix_data, min_per_dim_d),
+            OFF(h_matrix_data, coords_device), OFF(h_matrix_data, code_d),
+            OFF(h_matrix_data, points_d), OFF(h_matrix_data, morton_d), OFF(h_matrix_data, order),
+            OFF(h_matrix_data, bits), OFF(h_matrix_data, dim), OFF(h_matrix_data, eta),
+            OFF(h_matrix_data, max_level), OFF(h_matrix_data, c_leaf),
+            OFF(h_matrix_data, mat_vec_data_count), OFF(h_matrix_data, max_elements_in_array),
+            OFF(h_matrix_data, max_elements_in_mat_vec_data_array), OFF(h_matrix_data, mat_vec_data),
+            OFF(h_matrix_data, mat_vec_data_array_size), OFF(h_matrix_data, k),
+            OFF(h_matrix_data, epsilon), OFF(h_matrix_data, mat_vec_info),
+            OFF(h_matrix_data, point_count), OFF(h_matrix_data, assem),
+            OFF(h_matrix_data, max_batched_dense_size), OFF(h_matrix_data, max_batched_aca_size),
+            OFF(h_matrix_data, dense_batching_ratio), OFF(h_matrix_data, U), OFF(h_matrix_data, V),
+            OFF(h_matrix_data, dA), OFF(h_matrix_data, magma_queue),
+            OFF(h_matrix_data, dense_work_size), OFF(h_matrix_data, aca_work_size),
+            OFF(h_matrix_data, dense_batch_count), OFF(h_matrix_data, aca_batch_count),
+            OFF(h_matrix_data, root_level_set_1), OFF(h_matrix_data, root_level_set_2),
+            OFF(h_matrix_data, engine)};
+        if (field < 0) return (int64_t)sizeof(h_matrix_data);
+        if (field < (int)(sizeof(offs) / sizeof(offs[0]))) return offs[field];
+        return -1;
+    }
+    case 1: {
+        const int64_t offs[] = {OFF(work_item, set1_l), OFF(work_item, set1_u), OFF(work_item, set2_l),
+                                OFF(work_item, set2_u), OFF(work_item, work_type), OFF(work_item, max1),
+                                OFF(work_item, min1), OFF(work_item, max2), OFF(work_item, min2),
+                                OFF(work_item, dim), OFF(work_item, level_1), OFF(work_item, level_2)};
+        if (field < 0) return (int64_t)sizeof(work_item);
+        if (field < (int)(sizeof(offs) / sizeof(offs[0]))) return offs[field];
+        return -1;
+    }
+    case 2:
+        return field < 0 ? (int64_t)sizeof(point_set) : -1;
+    case 3:
+        return field < 0 ? (int64_t)sizeof(morton_code) : -1;
+    case 4:
+        return field < 0 ? (int64_t)sizeof(system_assembler) : -1;
+    case 5:
+        return field < 0 ? (int64_t)sizeof(mat_vec_data_info) : -1;
+    }
+    return -1;
+#undef OFF
+}
+
+// ------------------------------------------------------------------ generator
+static uint64_t sm64(uint64_t* st)
+{
+    uint64_t z = (*st += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+struct Xoshiro {
+    uint64_t s[4];
+    explicit Xoshiro(uint64_t seed)
+    {
+        uint64_t st = seed;
+        for (auto& w : s) w = sm64(&st);
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next()
+    {
+        const uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+void hmg_rng_uniform(uint64_t seed, int64_t n, double* out)
+{
+    Xoshiro g(seed);
+    for (int64_t i = 0; i < n; i++) out[i] = g.uniform();
+}
+
+void hmg_rng_normal(uint64_t seed, int64_t n, double* out)
+{
+    Xoshiro g(seed);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int64_t i = 0; i < n; i += 2) {
+        const double u1 = 1.0 - g.uniform();
+        const double u2 = g.uniform();
+        const double rad = std::sqrt(-2.0 * std::log(u1));
+        out[i] = rad * std::cos(two_pi * u2);
+        if (i + 1 < n) out[i + 1] = rad * std::sin(two_pi * u2);
+    }
+}
+
+// ------------------------------------------------------------------ void drop-ins
+static void die_on(hmg_status s, const char* where)
+{
+    if (s != HMG_OK) {
+        fprintf(stderr, "hmglib error in %s: %s\n", where, hmg_last_error());
+        exit(EXIT_FAILURE);
+    }
+}
+
+void init_h_matrix_data(struct h_matrix_data* data, int point_count[2], int dim, int bits)
+{
+    die_on(hmg_init_h_matrix_data(data, point_count, dim, bits), "init_h_matrix_data");
+}
+void setup_h_matrix(struct h_matrix_data* data) { die_on(hmg_setup_h_matrix(data), "setup_h_matrix"); }
+void apply_h_matrix_mvp(double* x, double* y, struct h_matrix_data* data)
+{
+    die_on(hmg_apply_h_matrix_mvp(x, y, data), "apply_h_matrix_mvp");
+}
+void apply_h_matrix_mvp_without_batching(double* x, double* y, struct h_matrix_data* data)
+{
+    die_on(hmg_apply_h_matrix_mvp_without_batching(x, y, data), "apply_h_matrix_mvp_without_batching");
+}
+void destroy_h_matrix_data(struct h_matrix_data* data)
+{
+    die_on(hmg_destroy_h_matrix_data(data), "destroy_h_matrix_data");
+}
+void precompute_aca(struct h_matrix_data* data) { die_on(hmg_precompute_aca(data), "precompute_aca"); }
+void precompute_dense(struct h_matrix_data* data)
+{
+    die_on(hmg_precompute_dense(data), "precompute_dense");
+}
+void apply_full_mvp(double* x, double* y, struct h_matrix_data* data)
+{
+    die_on(hmg_apply_full_mvp(x, y, data), "apply_full_mvp");
+}
+void set_gaussian_kernel_rhs(double* b, struct h_matrix_data* data)
+{
+    die_on(hmg_set_gaussian_kernel_rhs(b, data), "set_gaussian_kernel_rhs");
+}
+
+void create_gaussian_kernel_system_assembler_object(struct gaussian_kernel_system_assembler** assem,
+                                                    double regularization)
+{
+    die_on((hmg_status)hmg_create_kernel_object((system_assembler**)assem, HMG_KERNEL_GAUSSIAN_REF,
+                                                regularization, 1.0),
+           "create_gaussian_kernel_system_assembler_object");
+}
+void destroy_gaussian_kernel_system_assembler_object(struct gaussian_kernel_system_assembler** assem)
+{
+    die_on((hmg_status)hmg_destroy_kernel_object((system_assembler**)assem),
+           "destroy_gaussian_kernel_system_assembler_object");
+}
+void create_matern_kernel_system_assembler_object(struct matern_kernel_system_assembler** assem,
+                                                  double regularization)
+{
+    die_on((hmg_status)hmg_create_kernel_object((system_assembler**)assem, HMG_KERNEL_MATERN_REF,
+                                                regularization, 1.0),
+           "create_matern_kernel_system_assembler_object");
+}
+void destroy_matern_kernel_system_assembler_object(struct matern_kernel_system_assembler** assem)
+{
+    die_on((hmg_status)hmg_destroy_kernel_object((system_assembler**)assem),
+           "destroy_matern_kernel_system_assembler_object");
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+"""Python front-end over the C ABI, used by tests and benchmarks.
+
+It drives the engine exactly like the reference's drivers drive hmglib
+(src/hmglib_test.cu:43-140): init_h_matrix_data, write coordinates into
+data.coords_d, set the parameters, setup_h_matrix, attach a kernel object,
+precompute_aca / precompute_dense, apply.  Every compute step is a call into
+libhmglib.so; Python only moves host arrays to and from device buffers.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from .hip import DeviceArray, memcpy_dtoh, memcpy_htod
+
+
+def rng_uniform(seed, n):
+    out = np.empty(n, dtype=np.float64)
+    _abi.lib().hmg_rng_uniform(seed, n, out.ctypes.data_as(_abi.c_double_p))
+    return out
+
+
+def rng_normal(seed, n):
+    out = np.empty(n, dtype=np.float64)
+    _abi.lib().hmg_rng_normal(seed, n, out.ctypes.data_as(_abi.c_double_p))
+    return out
+
+
+class Kernel:
+    """A kernel object in device memory (struct system_assembler)."""
+
+    def __init__(self, kind, regularization=0.0, length_scale=1.0):
+        self.kind = kind
+        self.ptr = _abi.lib().hmg_kernel_new(kind, regularization, length_scale)
+        if not self.ptr:
+            raise _abi.HMGError(_abi.HMG_ERR_HIP, "hmg_kernel_new",
+                                _abi.lib().hmg_last_error().decode())
+
+    def free(self):
+        if self.ptr:
+            _abi.lib().hmg_kernel_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HMatrix:
+    """One H-matrix built from two point sets (SoA arrays of shape (dim, n))."""
+
+    def __init__(self, X1, X2, bits, eta, max_level, c_leaf, k, epsilon=1e-6,
+                 root_levels=(0, 0), max_batched_aca_size=1 << 30):
+        self.L = _abi.lib()
+        X1 = np.ascontiguousarray(X1, dtype=np.float64)
+        X2 = np.ascontiguousarray(X2, dtype=np.float64)
+        self.dim = X1.shape[0]
+        self.n1, self.n2 = X1.shape[1], X2.shape[1]
+        self.data = _abi.HMatrixData()
+        pc = (ctypes.c_int * 2)(self.n1, self.n2)
+        _abi.check(self.L.hmg_init_h_matrix_data(ctypes.byref(self.data), pc, self.dim, bits),
+                   "init_h_matrix_data")
+        self._alive = True
+        d = self.data
+        d.eta = eta
+        d.max_level = max_level
+        d.c_leaf = c_leaf
+        d.k = k
+        d.epsilon = epsilon
+        d.max_batched_dense_size = 8192
+        d.max_batched_aca_size = max_batched_aca_size
+        d.root_level_set_1 = bytes([root_levels[0] & 0xFF])
+        d.root_level_set_2 = bytes([root_levels[1] & 0xFF])
+        for s, X in ((0, X1), (1, X2)):
+            for k_ in range(self.dim):
+                memcpy_htod(d.coords_d[s][k_], X[k_])
+        self.kernel = None
+
+    # ---- lifecycle ---------------------------------------------------------
+    def setup(self):
+        _abi.check(self.L.hmg_setup_h_matrix(ctypes.byref(self.data)), "setup_h_matrix")
+
+    def set_kernel(self, kernel):
+        self.kernel = kernel
+        self.data.assem = kernel.ptr
+
+    def precompute_aca(self):
+        _abi.check(self.L.hmg_precompute_aca(ctypes.byref(self.data)), "precompute_aca")
+
+    def precompute_dense(self):
+        _abi.check(self.L.hmg_precompute_dense(ctypes.byref(self.data)), "precompute_dense")
+
+    def destroy(self):
+        if getattr(self, "_alive", False):
+            self.L.hmg_destroy_h_matrix_data(ctypes.byref(self.data))
+            self._alive = False
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    # ---- products on device buffers ------------------------------------------
+    def apply_dev(self, xd, yd, nrhs=1):
+        _abi.check(self.L.hmg_apply_h_matrix_mvp_multi(xd.ptr, yd.ptr, nrhs, ctypes.byref(self.data)),
+                   "apply_h_matrix_mvp")
+
+    # ---- products on host arrays ---------------------------------------------
+    def apply(self, x):
+        x = np.asarray(x, dtype=np.float64)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        xd = DeviceArray(np.asfortranarray(x.reshape(self.n2, nrhs)))
+        yd = DeviceArray.empty((self.n1, nrhs))
+        self.apply_dev(xd, yd, nrhs)
+        y = yd.download()
+        return y[:, 0] if x.ndim == 1 else y
+
+    def apply_without_batching(self, x):
+        xd = DeviceArray(np.ascontiguousarray(x, dtype=np.float64))
+        yd = DeviceArray.empty(self.n1)
+        _abi.check(self.L.hmg_apply_h_matrix_mvp_without_batching(xd.ptr, yd.ptr, ctypes.byref(self.data)),
+                   "apply_h_matrix_mvp_without_batching")
+        return yd.download()
+
+    def apply_full(self, x):
+        xd = DeviceArray(np.ascontiguousarray(x, dtype=np.float64))
+        yd = DeviceArray.empty(self.n1)
+        _abi.check(self.L.hmg_apply_full_mvp(xd.ptr, yd.ptr, ctypes.byref(self.data)), "apply_full_mvp")
+        return yd.download()
+
+    def dense_rows(self, rows, x):
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        x = np.asarray(x, dtype=np.float64)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        xd = DeviceArray(np.asfortranarray(x.reshape(self.n2, nrhs)))
+        rd = DeviceArray(rows)
+        yd = DeviceArray.empty((rows.size, nrhs))
+        _abi.check(self.L.hmg_dense_rows(xd.ptr, nrhs, rd.ptr, rows.size, yd.ptr, ctypes.byref(self.data)),
+                   "hmg_dense_rows")
+        y = yd.download()
+        return y[:, 0] if x.ndim == 1 else y
+
+    def gaussian_rhs(self):
+        bd = DeviceArray.empty(self.n1)
+        _abi.check(self.L.hmg_set_gaussian_kernel_rhs(bd.ptr, ctypes.byref(self.data)),
+                   "set_gaussian_kernel_rhs")
+        return bd.download()
+
+    # ---- readers ----------------------------------------------------------------
+    def codes(self, s):
+        n = self.n1 if s == 0 else self.n2
+        out = np.empty(n, dtype=np.uint64)
+        return memcpy_dtoh(out, self.data.code_d[s])
+
+    def order(self, s):
+        n = self.n1 if s == 0 else self.n2
+        out = np.empty(n, dtype=np.uint64)
+        return memcpy_dtoh(out, self.data.order[s])
+
+    def point_ids(self, s):
+        n = self.n1 if s == 0 else self.n2
+        out = np.empty(n, dtype=np.uint32)
+        return memcpy_dtoh(out, self.data.point_ids_d[s])
+
+    def sorted_coords(self, s):
+        n = self.n1 if s == 0 else self.n2
+        out = np.empty((self.dim, n))
+        for k in range(self.dim):
+            memcpy_dtoh(out[k], self.data.coords_d[s][k])
+        return out
+
+    def minmax(self, s):
+        mn = np.empty(self.dim)
+        mx = np.empty(self.dim)
+        memcpy_dtoh(mn, self.data.min_per_dim_d[s])
+        memcpy_dtoh(mx, self.data.max_per_dim_d[s])
+        return mn, mx
+
+    def blocks(self, dtype):
+        n = self.data.mat_vec_data_count
+        out = np.zeros(n, dtype=dtype)
+        if n:
+            memcpy_dtoh(out, ctypes.cast(self.data.mat_vec_data[0], ctypes.c_void_p).value)
+        return out
+
+    def set_profiling(self, on=True):
+        _abi.check(self.L.hmg_set_profiling(ctypes.byref(self.data), 1 if on else 0), "hmg_set_profiling")
+
+    PROFILE_CLASSES = {"gather": 0, "lr_vtx": 1, "row": 2, "aca": 3, "dense_fill": 4}
+
+    def profile(self):
+        out = {}
+        for name, cls in self.PROFILE_CLASSES.items():
+            ms = ctypes.c_double()
+            n = ctypes.c_int64()
+            _abi.check(self.L.hmg_get_profile(ctypes.byref(self.data), cls, ctypes.byref(ms),
+                                              ctypes.byref(n)), "hmg_get_profile")
+            out[name] = (ms.value, n.value)
+        return out
+
+    def work(self):
+        out = (ctypes.c_int64 * 9)()
+        _abi.check(self.L.hmg_get_work(ctypes.byref(self.data), out), "hmg_get_work")
+        return {"E_near": out[0], "E_lr": out[1], "dense": out[2], "aca": out[3], "levels": out[4],
+                "quirk_levels": out[5], "lr_m1_rank": out[6], "lr_m2_rank": out[7],
+                "rank_sum": out[8]}

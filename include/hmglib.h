@@ -1,0 +1,161 @@
+/*
+ * hmglib.h -- C API of the MI355X-native H-matrix engine (drop-in for hmglib).
+ *
+ * Keeps hmglib's struct tag, function names and user-visible field names and
+ * types (reference: src/hmglib.h:29-120) so that a caller of hmglib recompiles
+ * unchanged.  Changes at the type level, all library-owned:
+ *   - magma_queue_t magma_queue -> void* (a hipStream_t owned by the library);
+ *   - struct system_assembler is a plain kernel descriptor in device memory,
+ *     not a C++ object with a virtual get_matrix_entry;
+ *   - one trailing field `engine` holds the library's private state.
+ * Device pointers are HIP device pointers.  Everything is synchronous.
+ *
+ * The void entry points behave like the reference: on error they print a
+ * message and exit(EXIT_FAILURE) (src/helper.cu:31-37).  Every one has an
+ * hmg_* twin returning hmg_status instead.
+ */
+#ifndef HMGLIB_H
+#define HMGLIB_H
+
+#include <stdint.h>
+#include "morton.h"
+#include "tree.h"
+#include "linear_algebra.h"
+#include "system_assembler.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reference: src/hmglib.h:29-102 (same order, same types except as noted) */
+struct h_matrix_data
+{
+	/* point coordinates, per set a HOST array of `dim` DEVICE pointers (SoA,
+	 * point_count[i] doubles each).  The caller writes them between
+	 * init_h_matrix_data and setup_h_matrix; setup reorders them in place into
+	 * Morton order (reference: hmglib.cu:157-158). */
+	double** coords_d[2];
+	unsigned int* point_ids_d[2];
+	double* max_per_dim_d[2];
+	double* min_per_dim_d[2];
+	double** coords_device[2];
+	uint64_t* code_d[2];
+	struct point_set* points_d[2];
+	struct morton_code* morton_d[2];
+	uint64_t* order[2];
+
+	int bits;
+	int dim;
+	double eta;
+	int max_level;
+	int c_leaf;
+	int mat_vec_data_count;
+	int max_elements_in_array;
+	int max_elements_in_mat_vec_data_array;
+	struct work_item** mat_vec_data;
+	int mat_vec_data_array_size;
+	int k;
+	double epsilon;
+	struct mat_vec_data_info mat_vec_info;
+	int point_count[2];
+	struct system_assembler* assem;
+	int max_batched_dense_size;
+	int max_batched_aca_size;
+	double dense_batching_ratio;
+
+	double* U;
+	double* V;
+	double* dA;
+
+	void* magma_queue;          /* hipStream_t (reference: magma_queue_t) */
+
+	int* dense_work_size;
+	int* aca_work_size;
+	int dense_batch_count;
+	int aca_batch_count;
+
+	char root_level_set_1;
+	char root_level_set_2;
+
+	void* engine;               /* library-private state (not in the reference) */
+};
+
+typedef enum {
+	HMG_OK = 0,
+	HMG_ERR_INVALID = 1,       /* invalid argument or state */
+	HMG_ERR_OOM = 2,           /* device memory exhausted or budget exceeded */
+	HMG_ERR_HIP = 3,           /* HIP runtime error */
+	HMG_ERR_NOT_READY = 4      /* e.g. apply before setup, no kernel set */
+} hmg_status;
+
+/* ---- drop-in entry points (reference: src/hmglib.h:104-120) ---------------- */
+/* replaces hmglib.h:104 / hmglib.cu:68-123 */
+extern void init_h_matrix_data(struct h_matrix_data* data, int point_count[2], int dim, int bits);
+/* replaces hmglib.h:106 / hmglib.cu:125-225 */
+extern void setup_h_matrix(struct h_matrix_data* data);
+/* replaces hmglib.h:108 / hmglib.cu:271-286: y = H x, x and y device vectors in the
+ * caller's point order; x is restored exactly, y is overwritten. */
+extern void apply_h_matrix_mvp(double* x, double* y, struct h_matrix_data* data);
+/* replaces hmglib.h:110 / hmglib.cu:288-301: per-block path, ACA recomputed with the
+ * unbatched pivot rule (first argmax, linear_algebra.cu:473), nothing precomputed. */
+extern void apply_h_matrix_mvp_without_batching(double* x, double* y, struct h_matrix_data* data);
+/* replaces hmglib.h:112 / hmglib.cu:305-348 */
+extern void destroy_h_matrix_data(struct h_matrix_data* data);
+/* replaces hmglib.h:114 / hmglib.cu:228-236 */
+extern void precompute_aca(struct h_matrix_data* data);
+/* replaces hmglib.h:116 / hmglib.cu:239-268 */
+extern void precompute_dense(struct h_matrix_data* data);
+/* replaces hmglib.h:118 / hmglib.cu:351-401: dense product y = K x (never forms K) */
+extern void apply_full_mvp(double* x, double* y, struct h_matrix_data* data);
+/* replaces hmglib.h:120 / hmglib.cu:420-439 */
+extern void set_gaussian_kernel_rhs(double* b, struct h_matrix_data* data);
+
+/* ---- status-returning twins ------------------------------------------------- */
+hmg_status hmg_init_h_matrix_data(struct h_matrix_data* data, int point_count[2], int dim, int bits);
+hmg_status hmg_setup_h_matrix(struct h_matrix_data* data);
+hmg_status hmg_apply_h_matrix_mvp(double* x, double* y, struct h_matrix_data* data);
+hmg_status hmg_apply_h_matrix_mvp_without_batching(double* x, double* y, struct h_matrix_data* data);
+hmg_status hmg_destroy_h_matrix_data(struct h_matrix_data* data);
+hmg_status hmg_precompute_aca(struct h_matrix_data* data);
+hmg_status hmg_precompute_dense(struct h_matrix_data* data);
+hmg_status hmg_apply_full_mvp(double* x, double* y, struct h_matrix_data* data);
+hmg_status hmg_set_gaussian_kernel_rhs(double* b, struct h_matrix_data* data);
+
+/* ---- extensions (new symbols; no counterpart in the reference) -------------- */
+/* Multi-RHS apply: X is point_count[1] x nrhs, Y is point_count[0] x nrhs, both
+ * column-major device arrays in the caller's point order.  Deterministic: every
+ * row of Y is summed in a fixed order, so a result is bitwise the same whatever
+ * else runs on the device. */
+hmg_status hmg_apply_h_matrix_mvp_multi(const double* X, double* Y, int nrhs, struct h_matrix_data* data);
+/* dense product for nrhs right-hand sides on sampled ORIGINAL rows (checker helper
+ * at scale): Yrows[q + c*nrows] = sum_j K(rows[q], j) X[j + c*n2]. rows is a device
+ * int64 array. */
+hmg_status hmg_dense_rows(const double* X, int nrhs, const int64_t* rows, int nrows, double* Yrows,
+                          struct h_matrix_data* data);
+/* Algorithmic work of the built H-matrix: out[0] = sum_dense m1*m2 (E_near),
+ * out[1] = sum_aca (m1+m2)*rank (E_lr, ranks as built; k_b before precompute_aca),
+ * out[2] = #dense blocks, out[3] = #ACA blocks, out[4] = tree levels,
+ * out[5] = levels on which a size-1 cluster changed the bounding-box table,
+ * out[6] = sum_aca m1*rank, out[7] = sum_aca m2*rank, out[8] = sum_aca rank. */
+hmg_status hmg_get_work(const struct h_matrix_data* data, int64_t out[9]);
+/* Per-kernel timing with HIP events recorded on the engine's stream around each
+ * launch.  enable != 0 starts (and clears) recording.  Kernel classes: 0 = x
+ * permutation, 1 = low-rank V^T x, 2 = row tiles (near field + U t), 3 = ACA,
+ * 4 = near-field fill.  Returns the summed milliseconds and the launch count. */
+hmg_status hmg_set_profiling(struct h_matrix_data* data, int enable);
+hmg_status hmg_get_profile(const struct h_matrix_data* data, int kernel_class, double* total_ms,
+                           int64_t* launches);
+/* last error message of this thread */
+const char* hmg_last_error(void);
+/* sizeof / field offsets of the public structs, for FFI layout checks */
+int64_t hmg_struct_layout(int which, int field);
+/* seeded host generator shared by tests and benchmarks: SplitMix64 seeding
+ * xoshiro256**; uniform = (next>>11)*2^-53; normal = Box-Muller. */
+void hmg_rng_uniform(uint64_t seed, int64_t n, double* out);
+void hmg_rng_normal(uint64_t seed, int64_t n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,96 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every entry
+point that include/*.h declares, and its struct layouts match the ctypes view.
+Nothing here launches GPU work."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import hmglib_amd
+from hmglib_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def _declared_functions():
+    names = set()
+    for fn in os.listdir(INCLUDE):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"^\s*(?:extern\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", text, re.M):
+            name = m.group(1)
+            if name in ("if", "while", "for", "return", "sizeof"):
+                continue
+            names.add(name)
+    return sorted(names)
+
+
+def test_header_declares_the_reference_entry_points():
+    names = set(_declared_functions())
+    for ref in ["init_h_matrix_data", "setup_h_matrix", "apply_h_matrix_mvp",
+                "apply_h_matrix_mvp_without_batching", "destroy_h_matrix_data", "precompute_aca",
+                "precompute_dense", "apply_full_mvp", "set_gaussian_kernel_rhs",
+                "create_gaussian_kernel_system_assembler_object",
+                "destroy_gaussian_kernel_system_assembler_object",
+                "create_matern_kernel_system_assembler_object",
+                "destroy_matern_kernel_system_assembler_object"]:
+        assert ref in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = hmglib_amd.lib()
+    names = _declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the ctypes signature table covers the same set
+    assert set(names) == set(_abi.SIGNATURES)
+
+
+def test_struct_layouts_match():
+    L = hmglib_amd.lib()
+    assert L.hmg_struct_layout(0, -1) == ctypes.sizeof(_abi.HMatrixData)
+    for i, (name, _) in enumerate(_abi.HMatrixData._fields_):
+        assert getattr(_abi.HMatrixData, name).offset == L.hmg_struct_layout(0, i), name
+    assert L.hmg_struct_layout(1, -1) == ctypes.sizeof(_abi.WorkItem) == 192
+    for i, (name, _) in enumerate(_abi.WorkItem._fields_):
+        assert getattr(_abi.WorkItem, name).offset == L.hmg_struct_layout(1, i), name
+    # reference x86-64 layout of work_item (tree.h:38-55): max1 @24, dim @184
+    assert _abi.WorkItem.max1.offset == 24 and _abi.WorkItem.dim.offset == 184
+    assert L.hmg_struct_layout(4, -1) == ctypes.sizeof(_abi.SystemAssembler) == 24
+    assert L.hmg_struct_layout(5, -1) == 12
+
+
+def test_invalid_arguments_are_rejected_without_touching_the_gpu():
+    L = hmglib_amd.lib()
+    d = _abi.HMatrixData()
+    pc = (ctypes.c_int * 2)(100, 100)
+    assert L.hmg_init_h_matrix_data(ctypes.byref(d), pc, 0, 32) == _abi.HMG_ERR_INVALID
+    assert b"dim" in L.hmg_last_error()
+    assert L.hmg_init_h_matrix_data(ctypes.byref(d), pc, 6, 32) == _abi.HMG_ERR_INVALID
+    bad = (ctypes.c_int * 2)(0, 100)
+    assert L.hmg_init_h_matrix_data(ctypes.byref(d), bad, 2, 32) == _abi.HMG_ERR_INVALID
+    assert L.hmg_init_h_matrix_data(None, pc, 2, 32) == _abi.HMG_ERR_INVALID
+    # an uninitialised struct is refused by every stage
+    z = _abi.HMatrixData()
+    assert L.hmg_setup_h_matrix(ctypes.byref(z)) == _abi.HMG_ERR_NOT_READY
+    assert L.hmg_precompute_aca(ctypes.byref(z)) == _abi.HMG_ERR_NOT_READY
+    assert L.hmg_precompute_dense(ctypes.byref(z)) == _abi.HMG_ERR_NOT_READY
+    assert L.hmg_apply_h_matrix_mvp(None, None, ctypes.byref(z)) == _abi.HMG_ERR_NOT_READY
+    assert L.hmg_apply_full_mvp(None, None, ctypes.byref(z)) == _abi.HMG_ERR_NOT_READY
+    out = (ctypes.c_int64 * 6)()
+    assert L.hmg_get_work(ctypes.byref(z), out) == _abi.HMG_ERR_NOT_READY
+    assert L.hmg_destroy_h_matrix_data(ctypes.byref(z)) == _abi.HMG_OK
+    assert not L.hmg_kernel_new(99, 0.0, 1.0)
+
+
+def test_python_front_end_raises_when_library_is_missing(monkeypatch, tmp_path):
+    monkeypatch.setattr(_abi, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(_abi, "_lib", None)
+    with pytest.raises(RuntimeError, match="not built"):
+        _abi.lib()

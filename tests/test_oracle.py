@@ -1,0 +1,354 @@
+"""CPU tests that pin the oracle (oracle/) independently of its own restatement.
+
+The reference ships no golden vectors or fixtures (its drivers only print a
+relative error against a dense cuBLAS product, hmglib_test.cu:124-128), so each
+function of the oracle is pinned by what the mathematics fixes: published
+known-answer vectors, brute force on small inputs, closed forms and invariants.
+"""
+import collections
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+# ---------------------------------------------------------------- generator
+def test_splitmix64_known_answer():
+    # first outputs of Vigna's splitmix64.c for seed 1234567
+    assert O.splitmix64(1234567, 5) == [6457827717110365317, 3203168211198807973,
+                                        9817491932198370423, 4593380528125082431,
+                                        16408922859458223821]
+
+
+def test_uniform_range_and_moments():
+    u = O.uniform(2001, 200000)
+    assert u.min() >= 0.0 and u.max() < 1.0
+    assert abs(u.mean() - 0.5) < 5e-3
+    assert abs(u.var() - 1.0 / 12.0) < 5e-3
+    # 53-bit granularity
+    assert np.all(u * 2.0 ** 53 == np.floor(u * 2.0 ** 53))
+    z = O.normal(2002, 200001)
+    assert abs(z.mean()) < 1e-2 and abs(z.std() - 1.0) < 1e-2
+
+
+def test_product_generator_matches_oracle():
+    from hmglib_amd.hmatrix import rng_normal, rng_uniform
+    np.testing.assert_array_equal(rng_uniform(1001, 10001), O.uniform(1001, 10001))
+    np.testing.assert_array_equal(rng_normal(3001, 999), O.normal(3001, 999))
+
+
+# ---------------------------------------------------------------- Morton
+def _interleave(q, dim, bits):
+    out = np.zeros(q.shape[-1], dtype=np.uint64)
+    for d in range(dim):
+        for b in range(bits):
+            bit = (q[d] >> np.uint64(b)) & np.uint64(1)
+            out |= bit << np.uint64(dim * b + d)
+    return out
+
+
+def test_stretch_exhaustive_3d_and_sampled_2d():
+    rng = np.random.default_rng(0)
+    vals3 = list(range(0, 1 << 20, 4099)) + [0, 1, (1 << 20) - 1, 0xAAAAA, 0x55555]
+    for v in vals3:
+        assert O.stretch(v, 3, 20) == sum(((v >> b) & 1) << (3 * b) for b in range(20))
+    for v in list(rng.integers(0, 1 << 32, 2000)) + [0, (1 << 32) - 1, 0xAAAAAAAA]:
+        v = int(v)
+        assert O.stretch(v, 2, 32) == sum(((v >> b) & 1) << (2 * b) for b in range(32))
+    # undefined (dim,bits) pairs give 0 (morton.cu:114)
+    assert O.stretch(12345, 3, 21) == 0
+    assert O.stretch(7, 10, 4) == sum(((7 >> b) & 1) << (10 * b) for b in range(4))
+
+
+@pytest.mark.parametrize("dim,bits", [(2, 32), (3, 20)])
+def test_morton_quantisation_closed_form(dim, bits):
+    n = 5000
+    X = O.uniform(7 + dim, dim * n).reshape(dim, n)
+    codes = O.morton(X, bits)
+    mn, mx = X.min(axis=1), X.max(axis=1)
+    q = np.empty((dim, n), dtype=np.uint64)
+    for d in range(dim):
+        t = (X[d] - mn[d]) / (mx[d] - mn[d])
+        qq = np.floor(t * float(1 << bits)).astype(np.uint64)
+        q[d] = np.minimum(qq, np.uint64((1 << bits) - 1))
+    np.testing.assert_array_equal(codes, _interleave(q, dim, bits))
+    # the maximum point maps to the last cell, the minimum to cell 0
+    assert np.all(q.max(axis=1) == (1 << bits) - 1) and np.all(q.min(axis=1) == 0)
+
+
+def test_sort_is_stable():
+    rng = np.random.default_rng(1)
+    codes = rng.integers(0, 50, 20000).astype(np.uint64)
+    s, order = O.sort_codes(codes)
+    ref = np.argsort(codes, kind="stable")
+    np.testing.assert_array_equal(order, ref.astype(np.uint64))
+    np.testing.assert_array_equal(s, codes[ref])
+
+
+def _split_brute(codes, first, last):
+    c = [int(v) for v in codes]
+    if c[first] == c[last]:
+        return (first + last) >> 1
+    common = 64 - (c[first] ^ c[last]).bit_length()
+    best = first
+    for i in range(first, last):
+        if 64 - (c[first] ^ c[i]).bit_length() > common:
+            best = i
+    return best
+
+
+def test_find_split_brute_force():
+    rng = np.random.default_rng(2)
+    for trial in range(300):
+        n = int(rng.integers(2, 60))
+        codes = np.sort(rng.integers(0, 1 << int(rng.integers(3, 62)), n).astype(np.uint64))
+        a = int(rng.integers(0, n - 1))
+        b = int(rng.integers(a + 1, n))
+        assert O.find_split(codes, a, b) == _split_brute(codes, a, b)
+
+
+# ---------------------------------------------------------------- tree
+def _admissible(b1, b2, eta):
+    """tree.cu:121-160 with the diameter accumulated as fma(t, t, acc)."""
+    def diam(lo, hi):
+        acc = 0.0
+        for t in hi - lo:
+            acc = _fma(float(t), float(t), acc)
+        return np.sqrt(acc)
+    (lo1, hi1), (lo2, hi2) = b1, b2
+    dist = 0.0
+    for d in range(lo1.size):
+        a = max(0.0, lo1[d] - hi2[d])
+        b = max(0.0, lo2[d] - hi1[d])
+        dist = dist + (a * a + b * b)
+    return max(diam(lo1, hi1), diam(lo2, hi2)) <= eta * np.sqrt(dist)
+
+
+def _fma(a, b, c):
+    from fractions import Fraction
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def _tree_brute(X1s, X2s, codes1, codes2, eta, max_level, c_leaf):
+    """Level-by-level traversal with exact boxes computed directly from the points."""
+    def box(X, l, u):
+        return X[:, l:u + 1].min(axis=1), X[:, l:u + 1].max(axis=1)
+
+    cur = [(0, X1s.shape[1] - 1, 0, X2s.shape[1] - 1, 0, 0)]
+    leaves = []
+    for level in range(max_level):
+        nxt = []
+        for (l1, u1, l2, u2, v1, v2) in cur:
+            adm = _admissible(box(X1s, l1, u1), box(X2s, l2, u2), eta)
+            if adm or (u1 - l1 + 1) <= c_leaf or (u2 - l2 + 1) <= c_leaf or level + 1 >= max_level:
+                leaves.append((l1, u1, l2, u2, O.WT_ACA if adm else O.WT_DENSE, v1, v2))
+            else:
+                s1 = _split_brute(codes1, l1, u1)
+                s2 = _split_brute(codes2, l2, u2)
+                nxt += [(l1, s1, l2, s2, v1 + 1, v2 + 1), (l1, s1, s2 + 1, u2, v1 + 1, v2 + 1),
+                        (s1 + 1, u1, l2, s2, v1 + 1, v2 + 1), (s1 + 1, u1, s2 + 1, u2, v1 + 1, v2 + 1)]
+        cur = nxt
+        if not cur:
+            break
+    dense = [b for b in leaves if b[4] == O.WT_DENSE]
+    return dense + [b for b in leaves if b[4] == O.WT_ACA]
+
+
+@pytest.mark.parametrize("dim,bits,n1,n2,eta,c_leaf,seed", [
+    (2, 32, 300, 300, 1.5, 8, 1), (2, 32, 400, 250, 0.7, 8, 2), (3, 20, 350, 350, 1.0, 16, 3),
+    (3, 20, 200, 500, 2.0, 8, 4)])
+def test_tree_matches_brute_force(dim, bits, n1, n2, eta, c_leaf, seed):
+    X1 = O.uniform(seed, dim * n1).reshape(dim, n1)
+    X2 = X1 if n1 == n2 else O.uniform(seed + 100, dim * n2).reshape(dim, n2)
+    hm = O.HMatrix(X1, X2, bits, eta, 50, c_leaf)
+    diag = hm.diagnostics()
+    assert diag["box_differs"] == 0 and diag["undefined_table"] == 0
+    ref = _tree_brute(hm.sorted_coords(0), hm.sorted_coords(1), hm.codes(0), hm.codes(1), eta, 50,
+                      c_leaf)
+    got = hm.blocks()
+    got_t = [(int(b["set1_l"]), int(b["set1_u"]), int(b["set2_l"]), int(b["set2_u"]),
+              int(b["work_type"]), int(b["level_1"]), int(b["level_2"])) for b in got]
+    assert got_t == ref
+    assert hm.ndense == sum(1 for b in ref if b[4] == O.WT_DENSE)
+
+
+def test_tree_invariants_tiling_and_admissibility():
+    n = 4096
+    X = O.uniform(1001, 2 * n).reshape(2, n)
+    hm = O.HMatrix(X, X, 32, 1.5, 50, 64)
+    b = hm.blocks()
+    # blocks tile [0,N1) x [0,N2) exactly once
+    cover = np.zeros((n, n), dtype=np.int8)
+    for w in b:
+        cover[w["set1_l"]:w["set1_u"] + 1, w["set2_l"]:w["set2_u"] + 1] += 1
+    assert cover.min() == 1 and cover.max() == 1
+    Xs = hm.sorted_coords(0)
+    for w in b:
+        l1, u1, l2, u2 = (int(w[f]) for f in ("set1_l", "set1_u", "set2_l", "set2_u"))
+        b1 = (Xs[:, l1:u1 + 1].min(1), Xs[:, l1:u1 + 1].max(1))
+        b2 = (Xs[:, l2:u2 + 1].min(1), Xs[:, l2:u2 + 1].max(1))
+        np.testing.assert_array_equal(w["min1"][:2], b1[0])
+        np.testing.assert_array_equal(w["max2"][:2], b2[1])
+        if w["work_type"] == O.WT_ACA:
+            assert _admissible(b1, b2, 1.5)
+        else:
+            assert (u1 - l1 + 1 <= 64 or u2 - l2 + 1 <= 64) and not _admissible(b1, b2, 1.5)
+    # DENSE first, then ACA
+    wt = b["work_type"]
+    assert np.all(wt[:hm.ndense] == O.WT_DENSE) and np.all(wt[hm.ndense:] == O.WT_ACA)
+
+
+def test_single_block_when_sets_are_small():
+    X = O.uniform(5, 2 * 40).reshape(2, 40)
+    hm = O.HMatrix(X, X, 32, 1.5, 50, 64)
+    b = hm.blocks()
+    assert len(b) == 1 and b[0]["work_type"] == O.WT_DENSE
+    assert (b[0]["set1_l"], b[0]["set1_u"], b[0]["set2_l"], b[0]["set2_u"]) == (0, 39, 0, 39)
+
+
+def _quirk_points(n):
+    """One point at the origin and n-1 points in [0.6,1]^2: the first split of the
+    Morton order isolates the origin in a size-1 cluster."""
+    rest = 0.6 + 0.4 * O.uniform(77, 2 * (n - 1)).reshape(2, n - 1)
+    X = np.concatenate([np.zeros((2, 1)), rest], axis=1)
+    X[:, 1] = 1.0  # pin the extent to [0,1]^2
+    return X
+
+
+def test_quirk_hand_worked_example():
+    """tree.cu:1000-1024 with a singleton cluster t=0 = [0,0] and t=1 = [1,N-1]:
+    keys[0] = +1 then -1 (same thread); keys[1] = +2; keys[N-1] = -2; inclusive
+    scan gives [-1, 1, ..., 1, -1]; fix-up keys[0] = 1, keys[N-1] = 2; runs are
+    [0, N-2] (key 1) and [N-1] (key 2).  So the singleton's box is the box of
+    points 0..N-2 and the big cluster's box is the single point N-1."""
+    n = 64
+    X = _quirk_points(n)
+    hm = O.HMatrix(X, X, 32, 1.0, 2, 0)
+    Xs = hm.sorted_coords(0)
+    assert np.all(Xs[:, 0] == 0.0)
+    b = hm.blocks()
+    assert len(b) == 4  # level-1 nodes become leaves at max_level 2
+    box_run0 = (Xs[:, :n - 1].min(1), Xs[:, :n - 1].max(1))
+    box_run1 = (Xs[:, n - 1], Xs[:, n - 1])
+    for w in b:
+        for s, (lo_f, hi_f, l_f) in enumerate((("min1", "max1", "set1_l"), ("min2", "max2", "set2_l"))):
+            expect = box_run0 if w[l_f] == 0 else box_run1
+            np.testing.assert_array_equal(w[lo_f][:2], expect[0])
+            np.testing.assert_array_equal(w[hi_f][:2], expect[1])
+    d = hm.diagnostics()
+    assert d["singleton_levels"] >= 2 and d["box_differs"] > 0
+
+
+# ---------------------------------------------------------------- kernels
+def test_kernel_entries_closed_form():
+    n = 50
+    X1 = O.uniform(3, 3 * n).reshape(3, n)
+    X2 = O.uniform(4, 3 * n).reshape(3, n)
+    hm = O.HMatrix(X1, X2, 20, 1.0, 50, 64)
+    A, B = hm.sorted_coords(0), hm.sorted_coords(1)
+    from scipy.special import gamma, yn
+    for kind, ls in [(O.K_GAUSSIAN_REF, 1.0), (O.K_GAUSSIAN, 0.1), (O.K_EXPONENTIAL, 0.1),
+                     (O.K_MATERN_REF, 1.0)]:
+        k = O.kernel(kind, 3, 0.25, ls)
+        for i, j in [(0, 0), (3, 7), (49, 2), (10, 10)]:
+            r = np.sqrt(np.sum((A[:, i] - B[:, j]) ** 2))
+            if kind == O.K_GAUSSIAN_REF:
+                v = np.exp(-r * r)
+            elif kind == O.K_GAUSSIAN:
+                v = np.exp(-r * r / (2 * ls * ls))
+            elif kind == O.K_EXPONENTIAL:
+                v = np.exp(-r / ls)
+            else:
+                v = yn(1, r + 1e-15) * r / (2.0 ** 1.5 * gamma(2.5))
+            if i == j:
+                v += 0.25
+            assert hm.kernel_entry(k, i, j) == pytest.approx(v, rel=1e-13)
+
+
+# ---------------------------------------------------------------- ACA
+def test_aca_rank_one_is_exact():
+    a = np.array([1.0, -2.0, 3.5, 0.25])
+    b = np.array([0.5, 4.0, -1.0])
+    A = np.outer(a, b)
+    U, V, rp, cp, r = O.aca_matrix(A, 3)
+    assert r == 1  # residual rows are zero, so the row search runs out of rows
+    np.testing.assert_allclose(U @ V.T, A, rtol=0, atol=1e-15)
+    assert rp[0] == 0 and cp[0] == 1  # |4.0| is the max of row 0
+
+
+def test_aca_pivot_rules():
+    # row 0 is zero and must be skipped; row 1 has a tie |3| at columns 0 and 2
+    A = np.array([[0.0, 0.0, 0.0], [3.0, 1.0, -3.0], [1.0, 2.0, 0.5]])
+    _, _, rp, cp, r = O.aca_matrix(A, 1, tie_last=True)
+    assert rp[0] == 1 and cp[0] == 2  # batched path: last argmax
+    _, _, rp, cp, r = O.aca_matrix(A, 1, tie_last=False)
+    assert rp[0] == 1 and cp[0] == 0  # unbatched path: first argmax
+    # scaled by the pivot read before scaling: V[cp] == 1
+    U, V, rp, cp, r = O.aca_matrix(A, 1)
+    assert V[cp[0], 0] == 1.0
+    np.testing.assert_array_equal(U[:, 0], A[:, cp[0]])
+
+
+def test_aca_full_rank_reproduces_matrix():
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((9, 6))
+    U, V, rp, cp, r = O.aca_matrix(A, 100)
+    assert r == 6
+    np.testing.assert_allclose(U @ V.T, A, atol=1e-12)
+    assert list(rp) == list(range(6))  # sequential rows
+
+
+def test_aca_rank_deficient_stops_when_rows_run_out():
+    rng = np.random.default_rng(6)
+    A = rng.standard_normal((5, 2)) @ rng.standard_normal((2, 7))
+    U, V, rp, cp, r = O.aca_matrix(A, 5)
+    assert r == 2
+    np.testing.assert_allclose(U @ V.T, A, atol=1e-12)
+
+
+def test_hmvp_full_rank_is_exact_against_numpy_dense():
+    n = 600
+    X = O.uniform(8, 2 * n).reshape(2, n)
+    hm = O.HMatrix(X, X, 32, 1.5, 50, 16)
+    k = O.kernel(O.K_GAUSSIAN, 2, 0.0, 0.2)
+    hm.aca(k, 10000)  # k_b = min(m1, m2): every ACA block is reproduced exactly
+    x = O.normal(9, n)
+    Xs = hm.sorted_coords(0)
+    D = np.exp(-((Xs[:, :, None] - Xs[:, None, :]) ** 2).sum(0) / (2 * 0.2 ** 2))
+    order = hm.order(0).astype(np.int64)
+    y_ref = np.empty(n)
+    y_ref[order] = D @ x[order]
+    np.testing.assert_allclose(hm.dense_mvp(k, x), y_ref, rtol=1e-12, atol=1e-12)
+    y = hm.hmvp(k, x)
+    assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < 1e-10
+
+
+def test_c1_configuration_error():
+    """Config 1 (N=4096, U[0,1]^2, exp(-r^2/0.02), leaf 64, eta 1.5, k 32): the
+    fixed-rank H-matrix error against the dense FP64 product."""
+    n = 4096
+    X = O.uniform(1001, 2 * n).reshape(2, n)
+    hm = O.HMatrix(X, X, 32, 1.5, 50, 64)
+    k = O.kernel(O.K_GAUSSIAN, 2, 0.0, 0.1)
+    hm.aca(k, 32)
+    x = O.uniform(1003, n) * 2 - 1
+    err = np.linalg.norm(hm.hmvp(k, x) - hm.dense_mvp(k, x)) / np.linalg.norm(hm.dense_mvp(k, x))
+    assert err < 1e-4
+    rows = np.array([0, 17, 4095])
+    np.testing.assert_allclose(hm.dense_rows(k, rows, x), hm.dense_mvp(k, x)[rows], rtol=1e-13)
+
+
+def test_gaussian_rhs_is_the_integral():
+    """b(x) = int_[0,1]^2 exp(-|x-y|^2) dy (hmglib.cu:403-418), by a tensor Gauss rule."""
+    n = 30
+    X = O.uniform(10, 2 * n).reshape(2, n)
+    hm = O.HMatrix(X, X, 32, 1.5, 50, 64)
+    b = hm.gaussian_rhs()
+    g, w = np.polynomial.legendre.leggauss(40)
+    g = 0.5 * (g + 1.0)
+    w = 0.5 * w
+    for i in range(n):
+        f1 = (w * np.exp(-(X[0, i] - g) ** 2)).sum()
+        f2 = (w * np.exp(-(X[1, i] - g) ** 2)).sum()
+        assert b[i] == pytest.approx(f1 * f2, rel=1e-13)
