@@ -42,31 +42,60 @@ def kernel_roofline(hm, prof, steps_prof):
     w = hm.work()
     n1 = hm.n1
     e_near = int(w["E_near"])
-    # algorithmic bytes per launch: V_b once, T written once (lr_vtx); U_b once,
-    # T read once per block, y written once (row kernel, low-rank part)
-    bytes_vtx = 8 * int(w["lr_m2_rank"] + w["rank_sum"])
-    bytes_row_lr = 8 * int(w["lr_m1_rank"] + n1 + w["rank_sum"])
-    # FP64 flops of the matrix-free near field: per entry 3 sub + 3 fma (r^2),
-    # sqrt, exp, 1 fma, counted as 2*(3+3+1) + 10 (sqrt) + 20 (exp) = 44
-    flops_near = 44 * e_near
-    t_vtx = prof["lr_vtx"][0] / max(prof["lr_vtx"][1], 1)
-    t_row = prof["row"][0] / max(prof["row"][1], 1)
+    sch = hm.schedule()
+    # algorithmic bytes per launch
+    #   lr_vtx: V_b once (8 B x sum m2*rank) + x~_J once per block (8 B x sum m2)
+    #           + t written (8 B x sum rank)
+    #   lr_u:   U once (8 B x sum m1*rank) + t once per 64-row item + partials
+    bytes_vtx = 8 * int(w["lr_m2_rank"] + sch["lr_m2"] + w["rank_sum"])
+    bytes_lru = 8 * int(w["lr_m1_rank"] + sch["lr_t_reads"] + sch["lr_rows"])
+    # FP64 flops of the matrix-free near field per entry: d subtractions + d FMAs
+    # (2 flops each) + sqrt (10) + exp (20) + scale (1) + accumulate FMA (2)
+    c_eval = 3 * hm.dim + 10 + 20 + 1 + 2
+    flops_near = c_eval * e_near
+
+    def avg(name):
+        ms, cnt = prof[name]
+        return ms / max(cnt, 1)
+
+    t = {k: avg(k) for k in ("gather", "lr_vtx", "lr_u", "near", "combine")}
     res = {
-        "lr_vtx": {"ms": t_vtx, "bytes": bytes_vtx, "GBs": bytes_vtx / (t_vtx * 1e-3) / 1e9},
-        "row": {"ms": t_row, "bytes_lowrank": bytes_row_lr, "near_flops": flops_near,
-                "near_TFLOPs": flops_near / (t_row * 1e-3) / 1e12,
-                "lowrank_GBs": bytes_row_lr / (t_row * 1e-3) / 1e9},
+        "lr_vtx": {"ms": t["lr_vtx"], "bytes": bytes_vtx, "GBs": bytes_vtx / (t["lr_vtx"] * 1e-3) / 1e9},
+        "lr_u": {"ms": t["lr_u"], "bytes": bytes_lru, "GBs": bytes_lru / (t["lr_u"] * 1e-3) / 1e9},
+        "near": {"ms": t["near"], "flops": flops_near, "c_eval": c_eval,
+                 "TFLOPs": flops_near / (t["near"] * 1e-3) / 1e12},
+        "gather_ms": t["gather"], "combine_ms": t["combine"],
     }
-    if t_row >= t_vtx:
-        ach = flops_near / (t_row * 1e-3) / 1e12
-        roof = {"kernel": "row_kernel", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
-                "note": "FP64 matrix-free near field; peak is MI355X FP64 (vector = matrix rate)"}
+    dominant = max(("lr_vtx", "lr_u", "near"), key=lambda k: t[k])
+    kname = {"lr_vtx": "lr_v_kernel", "lr_u": "lr_u_kernel", "near": "near_fly_kernel"}[dominant]
+    traffic = measured_traffic(hm, kname)
+    if dominant == "near":
+        ach = res["near"]["TFLOPs"]
+        roof = {"kernel": kname, "bound": "mfma", "achieved": ach,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                "traffic": traffic,
+                "note": f"FP64 near field evaluated on the fly (VALU, not MFMA); {c_eval} FP64 "
+                        f"flops per entry; peak = MI355X FP64 (vector and matrix rates are equal)"}
     else:
-        ach = bytes_vtx / (t_vtx * 1e-3) / 1e9
-        roof = {"kernel": "lr_vtx_kernel", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+        ach = res[dominant]["GBs"]
+        roof = {"kernel": kname, "bound": "hbm", "achieved": ach,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic}
     return roof, res
+
+
+def measured_traffic(hm, kname):
+    """HBM bytes per launch of `kname` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a 1 GiB copy), when it was
+    measured on this same workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "hmvp_fp64_n2p20_hbm_traffic.json")
+    if hm.n1 != (1 << 20) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d["kernels"].get(kname)
+    if k is None:
+        return None
+    return (k["read_GB"] + k["write_GB"]) * 1e9
 
 
 def cpu_baseline(X, x, cfg, stride):
@@ -102,11 +131,11 @@ def cpu_baseline(X, x, cfg, stride):
 
 
 WORKLOADS = {
-    # the plain method at scale
+    # the plain method at scale (the headline workload)
     "c2": dict(dim=3, bits=20, n=1 << 20, kind=4, ls=0.1, c_leaf=256, eta=1.0, k=24, seed_pts=2001,
                seed_x=2002, seed_rows=2003, desc="N=2^20 U[0,1]^3, exp(-r/0.1), leaf 256, eta 1.0, "
                                                  "ACA rank 24, FP64, 1 RHS"),
-    # the CPU-runnable case
+    # the small plain case (N=4096, 2D)
     "c1": dict(dim=2, bits=32, n=4096, kind=3, ls=0.1, c_leaf=64, eta=1.5, k=32, seed_pts=1001,
                seed_x=1003, seed_rows=2003, desc="N=4096 U[0,1]^2, exp(-r^2/0.02), leaf 64, eta 1.5, "
                                                  "ACA rank 32, FP64, 1 RHS"),
@@ -122,6 +151,8 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override the point count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-stride", type=int, default=16)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -136,6 +167,17 @@ def main():
     if hip.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(3)
+    if args.copy_probe:
+        import ctypes
+        nprobe = 1 << 27  # 1 GiB of doubles read and 1 GiB written per launch
+        src = hip.DeviceArray.empty(nprobe)
+        dst = hip.DeviceArray.empty(nprobe)
+        src.zero()
+        for _ in range(3):
+            hmglib_amd.lib().hmg_copy_probe(ctypes.c_void_p(src.ptr), ctypes.c_void_p(dst.ptr), nprobe)
+        log(f"[bench] copy probe: 3 launches of {nprobe * 8 / 2**30:.0f} GiB read + write")
+        src.free()
+        dst.free()
     cfg = dict(WORKLOADS[args.workload])
     if args.n:
         cfg["n"] = args.n

@@ -124,8 +124,10 @@ SIGNATURES = {
     "hmg_apply_h_matrix_mvp_multi": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _D]),
     "hmg_dense_rows": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _D]),
     "hmg_get_work": (ctypes.c_int, [_D, _P(ctypes.c_int64)]),
+    "hmg_get_schedule": (ctypes.c_int, [_D, _P(ctypes.c_int64)]),
     "hmg_set_profiling": (ctypes.c_int, [_D, ctypes.c_int]),
     "hmg_get_profile": (ctypes.c_int, [_D, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64)]),
+    "hmg_copy_probe": (ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
     "hmg_last_error": (ctypes.c_char_p, []),
     "hmg_struct_layout": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int]),
     "hmg_rng_uniform": (None, [ctypes.c_uint64, ctypes.c_int64, c_double_p]),

@@ -28,25 +28,28 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Factor layout: U_b is column-major m1 x k_b at U + uoff[a] (leading dim m1);
+// V_b is stored transposed, V_b[j, l] at V + voff[a] + j * ldv[a] + l, i.e. row j
+// of V_b is contiguous (ldv = total rank of the column cluster's blocks).
 __global__ void __launch_bounds__(kAcaThreads)
 aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, int naca,
            const int* __restrict__ kb, const int64_t* __restrict__ uoff,
-           const int64_t* __restrict__ voff, double* __restrict__ U, double* __restrict__ V,
-           int* __restrict__ keff, int tie_last)
+           const int64_t* __restrict__ voff, const int* __restrict__ ldv, double* __restrict__ U,
+           double* __restrict__ V, int* __restrict__ keff, int tie_last)
 {
     const int lane = threadIdx.x % kWave;
-    const int a = blockIdx.x * kAcaWaves + threadIdx.x / kWave;
+    const int a = blockIdx.x * kAcaWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     if (a >= naca) return;
     const int4 b = blk[a];
     const int l1 = b.x, m1 = b.y - b.x + 1;
     const int l2 = b.z, m2 = b.w - b.z + 1;
     const int k = kb[a];
+    const int64_t ld = ldv[a];
     double* __restrict__ Ub = U + uoff[a];
     double* __restrict__ Vb = V + voff[a];
     int i_r = -1;
     int ke = 0;
     for (int r = 0; r < k; r++) {
-        double* __restrict__ v = Vb + (int64_t)r * m2;
         bool stop = false;
         // residual-row search (compute_current_batched_v_r, :2068-2132)
         for (;;) {
@@ -58,20 +61,25 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
             double nrm = 0.0;
             for (int j = lane; j < m2; j += kWave) {
                 double val = kernel_entry(kp, c1, l1 + i_r, c2, l2 + j);
-                for (int l = 0; l < r; l++)
-                    val = __fma_rn(-Ub[(int64_t)l * m1 + i_r], Vb[(int64_t)l * m2 + j], val);
-                v[j] = val;
+                const double* __restrict__ vrow = Vb + (int64_t)j * ld;
+                for (int l = 0; l < r; l++) val = __fma_rn(-Ub[(int64_t)l * m1 + i_r], vrow[l], val);
+                Vb[(int64_t)j * ld + r] = val;
                 nrm = __fma_rn(val, val, nrm);
             }
             nrm = wave_sum(nrm);
             if (sqrt(nrm) >= 1.0e-13) break;
         }
-        if (stop) break;
+        if (stop) {
+            // the rank stops here: column r of V (last rejected row) is cleared, column r
+            // of U was never written; both stay zero in the product
+            for (int j = lane; j < m2; j += kWave) Vb[(int64_t)j * ld + r] = 0.0;
+            break;
+        }
         // column pivot (tuple_absolute_maximum, :991-1018)
         double best = -1.0, bval = 0.0;
         int bj = -1;
         for (int j = lane; j < m2; j += kWave) {
-            const double x = v[j];
+            const double x = Vb[(int64_t)j * ld + r];
             const double ax = fabs(x);
             if (tie_last ? (ax >= best) : (ax > best)) {
                 best = ax;
@@ -93,14 +101,14 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
         }
         const int jr = bj;
         const double p = bval;
-        for (int j = lane; j < m2; j += kWave) v[j] = v[j] / p;
+        for (int j = lane; j < m2; j += kWave) Vb[(int64_t)j * ld + r] = Vb[(int64_t)j * ld + r] / p;
         __threadfence_block();
         // residual column (batched_fill_kernel_vector_and_scaled_substraction_for_u_r)
+        const double* __restrict__ vpiv = Vb + (int64_t)jr * ld;
         double* __restrict__ u = Ub + (int64_t)r * m1;
         for (int i = lane; i < m1; i += kWave) {
             double val = kernel_entry(kp, c1, l1 + i, c2, l2 + jr);
-            for (int l = 0; l < r; l++)
-                val = __fma_rn(-Vb[(int64_t)l * m2 + jr], Ub[(int64_t)l * m1 + i], val);
+            for (int l = 0; l < r; l++) val = __fma_rn(-vpiv[l], Ub[(int64_t)l * m1 + i], val);
             u[i] = val;
         }
         __threadfence_block();
@@ -110,45 +118,6 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
 }
 
 }  // namespace
-
-// per-block ranks and factor offsets (host, from the block list)
-hmg_status aca_layout(Engine& eng, int k)
-{
-    const int na = eng.naca;
-    eng.h_kb.assign(na, 0);
-    eng.h_uoff.assign(na + 1, 0);
-    eng.h_voff.assign(na + 1, 0);
-    eng.kmax = 0;
-    for (int a = 0; a < na; a++) {
-        const int4 b = eng.h_blk[eng.ndense + a];
-        const int m1 = b.y - b.x + 1, m2 = b.w - b.z + 1;
-        int kb = k;
-        if (kb > m1) kb = m1;
-        if (kb > m2) kb = m2;  // set_k_per_item (:769-787)
-        if (kb < 0) kb = 0;
-        eng.h_kb[a] = kb;
-        eng.kmax = std::max(eng.kmax, kb);
-        eng.h_uoff[a + 1] = eng.h_uoff[a] + (int64_t)m1 * kb;
-        eng.h_voff[a + 1] = eng.h_voff[a] + (int64_t)m2 * kb;
-    }
-    hipStream_t s = eng.stream;
-    HMG_TRY(eng.kb.alloc(std::max(na, 1)));
-    HMG_TRY(eng.keff.alloc(std::max(na, 1)));
-    HMG_TRY(eng.uoff.alloc(na + 1));
-    HMG_TRY(eng.voff.alloc(na + 1));
-    if (na > 0) {
-        HMG_HIP(hipMemcpyAsync(eng.kb.p, eng.h_kb.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
-        HMG_HIP(hipMemsetAsync(eng.keff.p, 0, sizeof(int) * na, s));
-    }
-    HMG_HIP(hipMemcpyAsync(eng.uoff.p, eng.h_uoff.data(), sizeof(int64_t) * (na + 1),
-                           hipMemcpyHostToDevice, s));
-    HMG_HIP(hipMemcpyAsync(eng.voff.p, eng.h_voff.data(), sizeof(int64_t) * (na + 1),
-                           hipMemcpyHostToDevice, s));
-    HMG_HIP(hipStreamSynchronize(s));
-    eng.aca_k = k;
-    eng.h_toff.clear();  // T offsets follow the ranks
-    return HMG_OK;
-}
 
 hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp, int k,
                      int tie_last, double* U, double* V, int* keff)
@@ -160,7 +129,7 @@ hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const Kern
     const unsigned grid = (unsigned)((na + kAcaWaves - 1) / kAcaWaves);
     eng.prof.begin(PROF_ACA, s);
     aca_kernel<<<grid, kAcaThreads, 0, s>>>(c1, c2, kp, eng.blk.p + eng.ndense, na, eng.kb.p,
-                                            eng.uoff.p, eng.voff.p, U, V, keff, tie_last);
+                                            eng.uoff.p, eng.voff.p, eng.ldv.p, U, V, keff, tie_last);
     HMG_HIP(hipGetLastError());
     eng.prof.end(PROF_ACA, s);
     return HMG_OK;

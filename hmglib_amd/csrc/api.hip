@@ -155,13 +155,33 @@ void release_build(Engine& e, h_matrix_data* d)
     e.keff.release();
     e.uoff.release();
     e.voff.release();
-    e.tile_ptr.release();
-    e.tile_blk.release();
     e.toff.release();
     e.h_toff.clear();
+    e.h_uoff.clear();
+    e.h_voff.clear();
+    e.h_kb.clear();
+    e.h_dgrp_ptr.clear();
+    e.h_dgrp_blk.clear();
+    e.h_agrp_ptr.clear();
+    e.h_agrp_blk.clear();
+    e.h_cgrp_ptr.clear();
+    e.h_cgrp_blk.clear();
+    e.h_ldv.clear();
+    e.ldv.release();
+    e.lv_items.release();
+    e.lv_slices.release();
+    e.tmap.release();
+    e.Tp.release();
+    e.n_lv_items = e.n_lv_slices = 0;
+    e.near_items.release();
+    e.near_blk.release();
+    e.lr_items.release();
+    e.row_ptr.release();
+    e.row_idx.release();
+    e.P.release();
     e.xs.release();
-    e.ys.release();
     e.T.release();
+    e.n_near_items = e.n_lr_items = 0;
     e.built = false;
     if (d->mat_vec_data) {
         free(d->mat_vec_data);
@@ -218,8 +238,13 @@ hmg_status hmg_init_h_matrix_data(struct h_matrix_data* data, int point_count[2]
     }
     Engine* e = new Engine();
     HMG_HIP(hipStreamCreate(&e->stream));
+    HMG_HIP(hipStreamCreate(&e->stream2));
+    HMG_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+    HMG_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
     e->dim = dim;
     e->bits = bits;
+    if (const char* v = getenv("HMG_NEAR_LDS_KB")) e->near_lds_pad = std::max(0, atoi(v)) * 1024;
+    if (const char* v = getenv("HMG_SERIALIZE")) e->serialize = atoi(v) != 0;
     data->engine = e;
     data->dim = dim;
     data->bits = bits;
@@ -366,8 +391,8 @@ hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
     memcpy(data->aca_work_size, aws.data(), sizeof(int) * aws.size());
 
     // rank layout for k (k_b = min(k,m1,m2)) and the MVP schedule
+    HMG_TRY(build_groups(*e));
     HMG_TRY(aca_layout(*e, std::max(data->k, 0)));
-    HMG_TRY(build_schedule(*e));
     e->built = true;
     return HMG_OK;
 }
@@ -557,6 +582,36 @@ hmg_status hmg_get_work(const struct h_matrix_data* data, int64_t out[9])
     return HMG_OK;
 }
 
+hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[7])
+{
+    Engine* e;
+    HMG_TRY(check_ready(data, &e));
+    std::vector<LrItem> items((size_t)e->n_lr_items);
+    if (!items.empty())
+        HMG_HIP(hipMemcpy(items.data(), e->lr_items.p, sizeof(LrItem) * items.size(),
+                          hipMemcpyDeviceToHost));
+    int64_t rows = 0, treads = 0, m1s = 0, m2s = 0;
+    for (const LrItem& it : items) {
+        rows += it.nrows;
+        treads += it.K;
+    }
+    for (int a = 0; a < e->naca; a++) {
+        const int4 b = e->h_blk[e->ndense + a];
+        m1s += b.y - b.x + 1;
+        m2s += b.w - b.z + 1;
+    }
+    int total = 0;
+    if (e->n[0] > 0) HMG_HIP(hipMemcpy(&total, e->row_ptr.p + e->n[0], sizeof(int), hipMemcpyDeviceToHost));
+    out[0] = e->n_near_items;
+    out[1] = e->n_lr_items;
+    out[2] = rows;
+    out[3] = treads;
+    out[4] = m1s;
+    out[5] = m2s;
+    out[6] = total;
+    return HMG_OK;
+}
+
 hmg_status hmg_set_profiling(struct h_matrix_data* data, int enable)
 {
     Engine* e = engine_of(data);
@@ -625,6 +680,9 @@ hmg_status hmg_destroy_h_matrix_data(struct h_matrix_data* data)
         e->perm32[i].release();
     }
     e->prof.clear();
+    (void)hipEventDestroy(e->ev_fork);
+    (void)hipEventDestroy(e->ev_join);
+    (void)hipStreamDestroy(e->stream2);
     (void)hipStreamDestroy(e->stream);
     delete e;
     data->engine = nullptr;

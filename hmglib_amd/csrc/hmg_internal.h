@@ -90,14 +90,43 @@ struct DevBuf {
 };
 
 // ---- per-kernel timing with HIP events on the engine's stream ---------------
-enum ProfClass { PROF_GATHER = 0, PROF_LR_VTX = 1, PROF_ROW = 2, PROF_ACA = 3, PROF_DENSE_FILL = 4,
-                 PROF_NCLASS = 5 };
+enum ProfClass { PROF_GATHER = 0, PROF_LR_VTX = 1, PROF_NEAR = 2, PROF_ACA = 3, PROF_DENSE_FILL = 4,
+                 PROF_LR_U = 5, PROF_COMBINE = 6, PROF_NCLASS = 7 };
 struct Profiler {
     bool on = false;
     std::vector<hipEvent_t> ev[PROF_NCLASS];   // start/stop pairs
     void begin(int cls, hipStream_t s);
     void end(int cls, hipStream_t s);
     void clear();
+};
+
+// low-rank work item: rows [row0, row0+nrows) of the row cluster's U matrix
+// (m1 rows, K columns, column-major, U + uoff points at row0), times t at T + toff
+struct LrItem {
+    int row0;
+    int nrows;
+    int m1;
+    int K;
+    int64_t uoff;
+    int64_t toff;
+};
+
+// V^T x work item: columns [0, ncols) of a 64-column slice of a column cluster's
+// transposed V (leading dim ld), rows (points) j0 .. j0+nrows-1, V + vbase at row j0
+struct LvItem {
+    int j0;
+    int nrows;
+    int ld;
+    int ncols;
+    int64_t vbase;
+};
+// reduction of the row chunks of one 64-column slice: items item0 .. item0+nchunks-1,
+// slice columns map to T through tmap[tcol0 + l]
+struct LvSlice {
+    int item0;
+    int nchunks;
+    int ncols;
+    int tcol0;
 };
 
 // ---- engine state hung off h_matrix_data::engine ---------------------------
@@ -117,32 +146,54 @@ struct Engine {
     int levels = 0;
     int quirk_levels = 0;
 
-    // ACA factors (FP64), per ACA block contiguous
+    // ACA factors (FP64).  U of all ACA blocks that share a row cluster C is one
+    // column-major m1(C) x K(C) matrix (blocks in block order, k_b columns each);
+    // V of all ACA blocks that share a column cluster D is one row-major
+    // m2(D) x K(D) matrix (V_b[j,l] at voff[a] + j*ldv[a] + l).  T (V^T x) is laid
+    // out in U order; tmap sends V columns to T.
     int aca_k = -1;                             // k used for the stored factors
-    std::vector<int> h_kb;
-    std::vector<int64_t> h_uoff, h_voff;        // naca+1
-    DevBuf<int> kb, keff;
-    DevBuf<int64_t> uoff, voff;
+    std::vector<int> h_kb, h_ldv;
+    std::vector<int64_t> h_uoff, h_voff, h_toff; // per ACA block
+    DevBuf<int> kb, keff, ldv;
+    DevBuf<int64_t> uoff, voff, toff;
     DevBuf<double> U, V;
     int kmax = 0;
+    int64_t tsize = 0;
 
-    // dense near field (FP64), per DENSE block contiguous
+    // dense near field (FP64), per DENSE block contiguous, column-major m1 x m2
     std::vector<int64_t> h_aoff;                // ndense+1
     DevBuf<int64_t> aoff;
     DevBuf<double> A;
     bool dense_stored = false;
 
-    // MVP schedule
-    int ntiles = 0;
-    DevBuf<int> tile_ptr, tile_blk;
-    DevBuf<int64_t> toff;                       // per ACA block offset into T (naca+1)
-    std::vector<int64_t> h_toff;
-    DevBuf<double> xs, ys, T;                   // permuted x, permuted y, V^T x
-    std::vector<int> h_lr_order;                // ACA blocks, largest m2 first
-    DevBuf<int> lr_order;
+    // row-cluster groups: DENSE and ACA blocks grouped by their row cluster
+    // (groups in order of first appearance, blocks in block order inside a group)
+    std::vector<int> h_dgrp_ptr, h_dgrp_blk;    // dense groups
+    std::vector<int> h_agrp_ptr, h_agrp_blk;    // ACA groups (indices into the ACA list)
+    std::vector<int> h_cgrp_ptr, h_cgrp_blk;    // ACA blocks grouped by column cluster
 
-    // host-side batch sizing copies (reference semantics) for dense/aca_work_size
-    std::vector<int> dense_ws, aca_ws;
+    // MVP schedule: work items write 64-row partials into P; every row sums its
+    // partials in a fixed order (CSR rowptr/rowidx) and writes y once.
+    int n_near_items = 0, n_lr_items = 0, n_lv_items = 0, n_lv_slices = 0;
+    DevBuf<int4> near_items;                    // {row0, nrows, b0, b1} into near_blk
+    DevBuf<int> near_blk;
+    DevBuf<LrItem> lr_items;
+    DevBuf<LvItem> lv_items;
+    DevBuf<LvSlice> lv_slices;
+    DevBuf<int> tmap;
+    DevBuf<double> Tp;                          // V^T x partials per row chunk
+    DevBuf<int> row_ptr, row_idx;
+    DevBuf<double> P, xs, T;
+
+    hipStream_t stream2 = nullptr;              // near field runs beside the low-rank part
+    // tuning: LDS bytes reserved per near-field workgroup (caps its occupancy so
+    // the low-rank streams keep wave slots on every CU); run the near field on
+    // the main stream instead (no overlap)
+    // 40 KB + the kernel's own 12 KB leave room for 3 near-field workgroups per CU
+    // (measured best on MI355X at N = 2^20: 60.9 ms vs 68.9 ms without the cap)
+    int near_lds_pad = 40 * 1024;
+    bool serialize = false;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 Engine* engine_of(const h_matrix_data* d);
@@ -183,11 +234,15 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
 // tie_last: 1 = last argmax (batched path), 0 = first argmax (unbatched path)
 hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
                      int k, int tie_last, double* U, double* V, int* keff);
+
+// ---- schedule (schedule.hip, host) ---------------------------------------------------
+// row-cluster groups and near-field work items (once per tree)
+hmg_status build_groups(Engine& eng);
+// ranks k_b = min(k,m1,m2), factor offsets, low-rank work items, per-row partial lists
 hmg_status aca_layout(Engine& eng, int k);
 
 // ---- near field / MVP (mvp.hip) ----------------------------------------------------
 hmg_status dense_fill(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp);
-hmg_status build_schedule(Engine& eng);
 hmg_status hmvp(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
                 const uint64_t* order1, const uint64_t* order2, const double* x, double* y,
                 int nrhs, const double* U, const double* V, const int* keff, bool dense_stored);

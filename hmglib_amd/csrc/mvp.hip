@@ -3,14 +3,19 @@
 // y = P1^T ( sum_dense A_b x~_J + sum_aca U_b (V_b^T x~_J) ),  x~ = P2 x
 // (reference: hmglib.cu:271-286, linear_algebra.cu:4292-4379, 1429-1591,
 // 2817-3137).  Instead of batched GEMVs and FP64 CAS atomics into y
-// (linear_algebra.cu:1152-1181) the engine uses row ownership:
-//   phase 1  one wavefront per ACA block: t_b = V_b^T x~_J  (stream V once)
-//   phase 2  one wavefront per tile of 64 Morton-ordered rows: every lane owns a
-//            row and sums, in block order, the dense rows (stored A_b, or
-//            evaluated on the fly) and the low-rank rows U_b[i,:] t_b, then writes
-//            y[order1[i]] once.
-// Every row is summed in a fixed order, so results are deterministic and do not
-// depend on what else runs on the device.
+// (linear_algebra.cu:1152-1181) the product runs as:
+//   lr_v      one wavefront per (column cluster, 64 columns, <= 1024 points):
+//             t = V_D^T x~_D over the column cluster's row-major V, one lane per
+//             column, no cross-lane reductions (HBM stream of V); lr_v_finalize
+//             sums the point chunks and scatters t into the U order
+//   lr_u      one wavefront per 64 rows of a row cluster: y_C = U_C t_C, one long
+//             contiguous GEMV per row cluster (HBM stream of U)
+//   near      one wavefront per (row cluster, 64 rows, column chunk): the dense
+//             near field, either streamed from the stored blocks or evaluated on
+//             the fly with the columns staged through LDS (VALU bound); it runs
+//             on a second stream beside the two low-rank kernels
+//   combine   every row adds its partials in a fixed order, writes y[order1[i]]
+// so results are deterministic and independent of what else runs on the device.
 #include "hmg_internal.h"
 #include "kernel_eval.h"
 
@@ -36,87 +41,242 @@ __global__ void gather_perm_kernel(const double* __restrict__ x, const uint64_t*
     if (p < n) xs[p] = x[order[p]];
 }
 
-// phase 1: T[toff[a] + l] = sum_j V_b[j + l*m2] * xs[l2 + j]
-__global__ void __launch_bounds__(kThreads)
-lr_vtx_kernel(const int4* __restrict__ blk, int naca, const int* __restrict__ keff,
-              const int64_t* __restrict__ voff, const int64_t* __restrict__ toff,
-              const double* __restrict__ V, const double* __restrict__ xs, double* __restrict__ T)
+// rocPRIM-style wavefront barrier: LDS traffic between lanes of one wave
+__device__ __forceinline__ void wave_barrier()
 {
-    const int lane = threadIdx.x % kWave;
-    const int a = blockIdx.x * kWaves + threadIdx.x / kWave;
-    if (a >= naca) return;
-    const int4 b = blk[a];
-    const int l2 = b.z, m2 = b.w - b.z + 1;
-    const int ke = keff[a];
-    const double* __restrict__ Vb = V + voff[a];
-    const double* __restrict__ xb = xs + l2;
-    for (int l = 0; l < ke; l++) {
-        const double* __restrict__ col = Vb + (int64_t)l * m2;
-        double s0 = 0.0, s1 = 0.0;
-        int j = lane;
-        for (; j + kWave < m2; j += 2 * kWave) {
-            s0 = __fma_rn(col[j], xb[j], s0);
-            s1 = __fma_rn(col[j + kWave], xb[j + kWave], s1);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+// Tp[item*64 + l] = sum_{j < nrows} V[vbase + l + j*ld] * xs[j0 + j]
+__global__ void __launch_bounds__(kThreads)
+lr_v_kernel(const LvItem* __restrict__ items, int nitems, const double* __restrict__ V,
+            const double* __restrict__ xs, double* __restrict__ Tp)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = blockIdx.x * kWaves + wave_id();
+    if (q >= nitems) return;
+    const LvItem it = items[q];
+    const bool act = lane < it.ncols;
+    const double* __restrict__ v = V + it.vbase + (act ? lane : 0);
+    const double* __restrict__ xb = xs + it.j0;
+    const int64_t ld = it.ld;
+    const int n = it.nrows;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int j = 0;
+    for (; j + 8 <= n; j += 8) {
+        const double v0 = v[(j + 0) * ld], v1 = v[(j + 1) * ld], v2 = v[(j + 2) * ld],
+                     v3 = v[(j + 3) * ld], v4 = v[(j + 4) * ld], v5 = v[(j + 5) * ld],
+                     v6 = v[(j + 6) * ld], v7 = v[(j + 7) * ld];
+        s0 = __fma_rn(v0, xb[j + 0], s0);
+        s1 = __fma_rn(v1, xb[j + 1], s1);
+        s2 = __fma_rn(v2, xb[j + 2], s2);
+        s3 = __fma_rn(v3, xb[j + 3], s3);
+        s0 = __fma_rn(v4, xb[j + 4], s0);
+        s1 = __fma_rn(v5, xb[j + 5], s1);
+        s2 = __fma_rn(v6, xb[j + 6], s2);
+        s3 = __fma_rn(v7, xb[j + 7], s3);
+    }
+    for (; j < n; j++) s0 = __fma_rn(v[j * ld], xb[j], s0);
+    Tp[(int64_t)q * kTileRows + lane] = (s0 + s1) + (s2 + s3);
+}
+
+// T[tmap[tcol0 + l]] = sum over the slice's point chunks, in chunk order
+__global__ void __launch_bounds__(kThreads)
+lr_v_finalize_kernel(const LvSlice* __restrict__ slices, int nslices, const double* __restrict__ Tp,
+                     const int* __restrict__ tmap, double* __restrict__ T)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = blockIdx.x * kWaves + wave_id();
+    if (q >= nslices) return;
+    const LvSlice sl = slices[q];
+    if (lane >= sl.ncols) return;
+    double s = 0.0;
+    for (int c = 0; c < sl.nchunks; c++) s += Tp[(int64_t)(sl.item0 + c) * kTileRows + lane];
+    T[tmap[sl.tcol0 + lane]] = s;
+}
+
+// y_C rows [row0, row0+nrows) = U_C[rows, :] t_C  ->  P[(pbase + item) * 64 + lane]
+__global__ void __launch_bounds__(kThreads)
+lr_u_kernel(const LrItem* __restrict__ items, int nitems, const double* __restrict__ U,
+            const double* __restrict__ T, double* __restrict__ P, int pbase)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = blockIdx.x * kWaves + wave_id();
+    if (q >= nitems) return;
+    const LrItem it = items[q];
+    const bool act = lane < it.nrows;
+    const double* __restrict__ u = U + it.uoff + (act ? lane : 0);
+    const double* __restrict__ t = T + it.toff;
+    const int64_t m1 = it.m1;
+    const int K = it.K;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int l = 0;
+    for (; l + 8 <= K; l += 8) {
+        const double u0 = u[(l + 0) * m1], u1 = u[(l + 1) * m1], u2 = u[(l + 2) * m1],
+                     u3 = u[(l + 3) * m1], u4 = u[(l + 4) * m1], u5 = u[(l + 5) * m1],
+                     u6 = u[(l + 6) * m1], u7 = u[(l + 7) * m1];
+        s0 = __fma_rn(u0, t[l + 0], s0);
+        s1 = __fma_rn(u1, t[l + 1], s1);
+        s2 = __fma_rn(u2, t[l + 2], s2);
+        s3 = __fma_rn(u3, t[l + 3], s3);
+        s0 = __fma_rn(u4, t[l + 4], s0);
+        s1 = __fma_rn(u5, t[l + 5], s1);
+        s2 = __fma_rn(u6, t[l + 6], s2);
+        s3 = __fma_rn(u7, t[l + 7], s3);
+    }
+    for (; l < K; l++) s0 = __fma_rn(u[l * m1], t[l], s0);
+    if (act) P[(int64_t)(pbase + q) * kTileRows + lane] = (s0 + s1) + (s2 + s3);
+}
+
+template <int KIND>
+__device__ __forceinline__ double kernel_value(const KernelParams& kp, double r2)
+{
+    if (KIND == HMG_KERNEL_GAUSSIAN) return exp(-(r2 * kp.c0));
+    if (KIND == HMG_KERNEL_EXPONENTIAL) return exp(-(sqrt(r2) * kp.c0));
+    if (KIND == HMG_KERNEL_GAUSSIAN_REF) {
+        const double r = sqrt(r2);
+        return exp(-(r * r));
+    }
+    const double r = sqrt(r2);
+    return (yn(1, r + 1.0e-15) * r) / kp.c1;
+}
+
+// The near field of one work item (up to 128 rows of a row cluster, one or two
+// rows per lane) against the columns of its dense blocks, which are staged 64 at a
+// time through this wave's LDS slice.  NR rows per lane give NR independent
+// evaluation chains per staged column.  DIM = 0 means "dim at run time".
+constexpr int kNearRows = 2 * kWave;
+
+template <int KIND, int DIM, bool REG, int NR>
+__device__ __forceinline__ void near_fly_item(const Coords& c1, const Coords& c2, const KernelParams& kp,
+                                              const int4 it, const int* __restrict__ near_blk,
+                                              const int4* __restrict__ blk,
+                                              const double* __restrict__ xs,
+                                              double (*sh)[kWave], int lane, double* __restrict__ out)
+{
+    constexpr int kS = MAX_POINT_DIMENSION + 1;
+    const int dim = DIM > 0 ? DIM : kp.dim;
+    bool act[NR];
+    int row[NR];
+    double xi[NR][MAX_POINT_DIMENSION];
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        act[r] = lane + r * kWave < it.y;
+        row[r] = it.x + (act[r] ? lane + r * kWave : 0);
+        acc[r] = 0.0;
+#pragma unroll
+        for (int d = 0; d < MAX_POINT_DIMENSION; d++) xi[r][d] = d < dim ? c1.x[d][row[r]] : 0.0;
+    }
+    for (int bi = it.z; bi < it.w; bi++) {
+        const int4 b = blk[near_blk[bi]];
+        const int l2 = b.z, m2 = b.w - b.z + 1;
+        for (int j0 = 0; j0 < m2; j0 += kWave) {
+            const int cnt = min(kWave, m2 - j0);
+            wave_barrier();
+            if (lane < cnt) {
+#pragma unroll
+                for (int d = 0; d < MAX_POINT_DIMENSION; d++)
+                    if (d < dim) sh[d][lane] = c2.x[d][l2 + j0 + lane];
+                sh[kS - 1][lane] = xs[l2 + j0 + lane];
+            }
+            wave_barrier();
+            for (int jj = 0; jj < cnt; jj++) {
+                double xj[MAX_POINT_DIMENSION];
+#pragma unroll
+                for (int d = 0; d < MAX_POINT_DIMENSION; d++) xj[d] = d < dim ? sh[d][jj] : 0.0;
+                const double xv = sh[kS - 1][jj];
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    double r2 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < MAX_POINT_DIMENSION; d++) {
+                        if (d < dim) {
+                            const double t = xi[r][d] - xj[d];
+                            r2 = __fma_rn(t, t, r2);
+                        }
+                    }
+                    double kv = kernel_value<KIND>(kp, r2);
+                    if (REG && row[r] == l2 + j0 + jj) kv += kp.reg;
+                    acc[r] = __fma_rn(kv, xv, acc[r]);
+                }
+            }
         }
-        if (j < m2) s0 = __fma_rn(col[j], xb[j], s0);
-        const double s = wave_sum(s0 + s1);
-        if (lane == 0) T[toff[a] + l] = s;
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+        if (act[r]) out[lane + r * kWave] = acc[r];
+}
+
+// near field evaluated on the fly; P[q*128 + row offset] for work item q
+template <int KIND, int DIM, bool REG>
+__global__ void __launch_bounds__(kThreads)
+near_fly_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ items, int nitems,
+                const int* __restrict__ near_blk, const int4* __restrict__ blk,
+                const double* __restrict__ xs, double* __restrict__ P)
+{
+    __shared__ double sh[kWaves][MAX_POINT_DIMENSION + 1][kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_id();
+    const int q = blockIdx.x * kWaves + w;
+    if (q >= nitems) return;
+    const int4 it = items[q];
+    double* out = P + (int64_t)q * kNearRows;
+    if (it.y > kWave)
+        near_fly_item<KIND, DIM, REG, 2>(c1, c2, kp, it, near_blk, blk, xs, sh[w], lane, out);
+    else
+        near_fly_item<KIND, DIM, REG, 1>(c1, c2, kp, it, near_blk, blk, xs, sh[w], lane, out);
+}
+
+// near field from the stored blocks (column-major m1 x m2 per dense block)
+__global__ void __launch_bounds__(kThreads)
+near_stored_kernel(const int4* __restrict__ items, int nitems, const int* __restrict__ near_blk,
+                   const int4* __restrict__ blk, const int64_t* __restrict__ aoff,
+                   const double* __restrict__ A, const double* __restrict__ xs,
+                   double* __restrict__ P)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = blockIdx.x * kWaves + wave_id();
+    if (q >= nitems) return;
+    const int4 it = items[q];
+    for (int half = 0; half * kWave < it.y; half++) {
+        const bool act = lane + half * kWave < it.y;
+        const int row = it.x + (act ? lane + half * kWave : 0);
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        for (int bi = it.z; bi < it.w; bi++) {
+            const int bidx = near_blk[bi];
+            const int4 b = blk[bidx];
+            const int64_t m1 = b.y - b.x + 1;
+            const int l2 = b.z, m2 = b.w - b.z + 1;
+            const double* __restrict__ a = A + aoff[bidx] + (row - b.x);
+            const double* __restrict__ xb = xs + l2;
+            int j = 0;
+            for (; j + 4 <= m2; j += 4) {
+                s0 = __fma_rn(a[(j + 0) * m1], xb[j + 0], s0);
+                s1 = __fma_rn(a[(j + 1) * m1], xb[j + 1], s1);
+                s2 = __fma_rn(a[(j + 2) * m1], xb[j + 2], s2);
+                s3 = __fma_rn(a[(j + 3) * m1], xb[j + 3], s3);
+            }
+            for (; j < m2; j++) s0 = __fma_rn(a[j * m1], xb[j], s0);
+        }
+        if (act) P[(int64_t)q * kNearRows + lane + half * kWave] = (s0 + s1) + (s2 + s3);
     }
 }
 
-// phase 2: one wavefront per 64-row tile
-__global__ void __launch_bounds__(kThreads)
-row_kernel(Coords c1, Coords c2, KernelParams kp, int ntiles, int n1, int ndense,
-           const int* __restrict__ tile_ptr, const int* __restrict__ tile_blk,
-           const int4* __restrict__ blk, const double* __restrict__ xs,
-           const int64_t* __restrict__ aoff, const double* __restrict__ A,
-           const int* __restrict__ keff, const int64_t* __restrict__ uoff,
-           const int64_t* __restrict__ toff, const double* __restrict__ U,
-           const double* __restrict__ T, const uint64_t* __restrict__ order1, double* __restrict__ y)
+// y[order1[i]] = sum of row i's partials, in list order
+__global__ void combine_kernel(const int* __restrict__ ptr, const int* __restrict__ idx,
+                               const double* __restrict__ P, const uint64_t* __restrict__ order1,
+                               double* __restrict__ y, int n1)
 {
-    const int lane = threadIdx.x % kWave;
-    const int tile = blockIdx.x * kWaves + threadIdx.x / kWave;
-    if (tile >= ntiles) return;
-    const int row = tile * kTileRows + lane;
-    double acc = 0.0;
-    const int p0 = tile_ptr[tile], p1 = tile_ptr[tile + 1];
-    for (int p = p0; p < p1; p++) {
-        const int bidx = tile_blk[p];
-        const int4 b = blk[bidx];
-        const int l1 = b.x, u1 = b.y, l2 = b.z;
-        const int m1 = u1 - l1 + 1, m2 = b.w - b.z + 1;
-        const bool active = row >= l1 && row <= u1;
-        if (bidx < ndense) {
-            if (A) {
-                if (active) {
-                    const double* __restrict__ Ab = A + aoff[bidx] + (row - l1);
-                    const double* __restrict__ xb = xs + l2;
-                    double s0 = 0.0, s1 = 0.0;
-                    int j = 0;
-                    for (; j + 1 < m2; j += 2) {
-                        s0 = __fma_rn(Ab[(int64_t)j * m1], xb[j], s0);
-                        s1 = __fma_rn(Ab[(int64_t)(j + 1) * m1], xb[j + 1], s1);
-                    }
-                    if (j < m2) s0 = __fma_rn(Ab[(int64_t)j * m1], xb[j], s0);
-                    acc += s0 + s1;
-                }
-            } else if (active) {
-                double s = 0.0;
-                for (int j = 0; j < m2; j++)
-                    s = __fma_rn(kernel_entry(kp, c1, row, c2, l2 + j), xs[l2 + j], s);
-                acc += s;
-            }
-        } else if (active) {
-            const int a = bidx - ndense;
-            const int ke = keff[a];
-            const double* __restrict__ Ub = U + uoff[a] + (row - l1);
-            const double* __restrict__ t = T + toff[a];
-            double s = 0.0;
-            for (int l = 0; l < ke; l++) s = __fma_rn(Ub[(int64_t)l * m1], t[l], s);
-            acc += s;
-        }
-    }
-    if (row < n1) y[order1[row]] = acc;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n1) return;
+    double s = 0.0;
+    for (int q = ptr[i]; q < ptr[i + 1]; q++) s += P[idx[q]];
+    y[order1[i]] = s;
 }
 
 // dense near-field fill: one block per dense block, column-major m1 x m2
@@ -178,6 +338,14 @@ __global__ void rhs_kernel(Coords c1, int dim, const uint64_t* __restrict__ orde
     b[order1[s]] = result;
 }
 
+__global__ void __launch_bounds__(kThreads)
+copy_probe_kernel(const double* __restrict__ src, double* __restrict__ dst, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads)
+        dst[i] = src[i];
+}
+
 __global__ void inv_kernel(const uint64_t* __restrict__ order, int64_t* __restrict__ inv, int64_t n)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -185,34 +353,6 @@ __global__ void inv_kernel(const uint64_t* __restrict__ order, int64_t* __restri
 }
 
 }  // namespace
-
-hmg_status build_schedule(Engine& eng)
-{
-    const int n1 = eng.n[0];
-    const int nt = (n1 + kTileRows - 1) / kTileRows;
-    eng.ntiles = nt;
-    std::vector<int> cnt(nt + 1, 0);
-    for (int b = 0; b < eng.nblocks; b++) {
-        const int4 bl = eng.h_blk[b];
-        for (int t = bl.x / kTileRows; t <= bl.y / kTileRows; t++) cnt[t + 1]++;
-    }
-    for (int t = 0; t < nt; t++) cnt[t + 1] += cnt[t];
-    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-    std::vector<int> lst((size_t)cnt[nt]);
-    for (int b = 0; b < eng.nblocks; b++) {
-        const int4 bl = eng.h_blk[b];
-        for (int t = bl.x / kTileRows; t <= bl.y / kTileRows; t++) lst[fill[t]++] = b;
-    }
-    hipStream_t s = eng.stream;
-    HMG_TRY(eng.tile_ptr.alloc(nt + 1));
-    HMG_TRY(eng.tile_blk.alloc(std::max<size_t>(lst.size(), 1)));
-    HMG_HIP(hipMemcpyAsync(eng.tile_ptr.p, cnt.data(), sizeof(int) * (nt + 1), hipMemcpyHostToDevice, s));
-    if (!lst.empty())
-        HMG_HIP(hipMemcpyAsync(eng.tile_blk.p, lst.data(), sizeof(int) * lst.size(),
-                               hipMemcpyHostToDevice, s));
-    HMG_HIP(hipStreamSynchronize(s));
-    return HMG_OK;
-}
 
 hmg_status dense_fill(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp)
 {
@@ -238,23 +378,43 @@ hmg_status dense_fill(Engine& eng, const Coords& c1, const Coords& c2, const Ker
     return HMG_OK;
 }
 
+namespace {
+
+template <int KIND, int DIM>
+void launch_near_fly(bool reg, unsigned grid, hipStream_t s, const Coords& c1, const Coords& c2,
+                     const KernelParams& kp, const Engine& eng, double* P)
+{
+    if (reg)
+        near_fly_kernel<KIND, DIM, true><<<grid, kThreads, eng.near_lds_pad, s>>>(
+            c1, c2, kp, eng.near_items.p, eng.n_near_items, eng.near_blk.p, eng.blk.p, eng.xs.p, P);
+    else
+        near_fly_kernel<KIND, DIM, false><<<grid, kThreads, eng.near_lds_pad, s>>>(
+            c1, c2, kp, eng.near_items.p, eng.n_near_items, eng.near_blk.p, eng.blk.p, eng.xs.p, P);
+}
+
+template <int KIND>
+void launch_near_fly_dim(bool reg, unsigned grid, hipStream_t s, const Coords& c1,
+                         const Coords& c2, const KernelParams& kp, const Engine& eng, double* P)
+{
+    if (kp.dim == 3)
+        launch_near_fly<KIND, 3>(reg, grid, s, c1, c2, kp, eng, P);
+    else if (kp.dim == 2)
+        launch_near_fly<KIND, 2>(reg, grid, s, c1, c2, kp, eng, P);
+    else
+        launch_near_fly<KIND, 0>(reg, grid, s, c1, c2, kp, eng, P);
+}
+
+}  // namespace
+
 hmg_status hmvp(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
                 const uint64_t* order1, const uint64_t* order2, const double* x, double* y, int nrhs,
                 const double* U, const double* V, const int* keff, bool dense_stored)
 {
-    hipStream_t s = eng.stream;
+    hipStream_t s = eng.stream, s2 = eng.serialize ? eng.stream : eng.stream2;
     const int n1 = eng.n[0], n2 = eng.n[1];
     const int na = eng.naca;
-    // T offsets (per ACA block, k_b slots)
-    if ((int)eng.h_toff.size() != na + 1) {
-        eng.h_toff.assign(na + 1, 0);
-        for (int a = 0; a < na; a++) eng.h_toff[a + 1] = eng.h_toff[a] + eng.h_kb[a];
-        HMG_TRY(eng.toff.alloc(na + 1));
-        HMG_HIP(hipMemcpyAsync(eng.toff.p, eng.h_toff.data(), sizeof(int64_t) * (na + 1),
-                               hipMemcpyHostToDevice, s));
-    }
     HMG_TRY(eng.xs.alloc(std::max(n2, 1)));
-    HMG_TRY(eng.T.alloc(std::max<int64_t>(eng.h_toff[na], 1)));
+    const bool reg = kp.reg != 0.0;
     for (int c = 0; c < nrhs; c++) {
         const double* xc = x + (int64_t)c * n2;
         double* yc = y + (int64_t)c * n1;
@@ -262,20 +422,58 @@ hmg_status hmvp(Engine& eng, const Coords& c1, const Coords& c2, const KernelPar
         gather_perm_kernel<<<grid_for(n2, kThreads), kThreads, 0, s>>>(xc, order2, eng.xs.p, n2);
         HMG_HIP(hipGetLastError());
         eng.prof.end(PROF_GATHER, s);
-        if (na > 0) {
+        // near field on the second stream
+        HMG_HIP(hipEventRecord(eng.ev_fork, s));
+        HMG_HIP(hipStreamWaitEvent(s2, eng.ev_fork, 0));
+        if (eng.n_near_items > 0) {
+            const unsigned g = grid_for(eng.n_near_items, kWaves);
+            eng.prof.begin(PROF_NEAR, s2);
+            if (dense_stored) {
+                near_stored_kernel<<<g, kThreads, eng.near_lds_pad, s2>>>(eng.near_items.p, eng.n_near_items,
+                                                          eng.near_blk.p, eng.blk.p, eng.aoff.p,
+                                                          eng.A.p, eng.xs.p, eng.P.p);
+            } else {
+                switch (kp.kind) {
+                case HMG_KERNEL_GAUSSIAN:
+                    launch_near_fly_dim<HMG_KERNEL_GAUSSIAN>(reg, g, s2, c1, c2, kp, eng, eng.P.p);
+                    break;
+                case HMG_KERNEL_EXPONENTIAL:
+                    launch_near_fly_dim<HMG_KERNEL_EXPONENTIAL>(reg, g, s2, c1, c2, kp, eng, eng.P.p);
+                    break;
+                case HMG_KERNEL_GAUSSIAN_REF:
+                    launch_near_fly_dim<HMG_KERNEL_GAUSSIAN_REF>(reg, g, s2, c1, c2, kp, eng, eng.P.p);
+                    break;
+                default:
+                    launch_near_fly_dim<HMG_KERNEL_MATERN_REF>(reg, g, s2, c1, c2, kp, eng, eng.P.p);
+                    break;
+                }
+            }
+            HMG_HIP(hipGetLastError());
+            eng.prof.end(PROF_NEAR, s2);
+        }
+        HMG_HIP(hipEventRecord(eng.ev_join, s2));
+        // low-rank part on the main stream
+        if (na > 0 && eng.n_lr_items > 0) {
             eng.prof.begin(PROF_LR_VTX, s);
-            lr_vtx_kernel<<<grid_for(na, kWaves), kThreads, 0, s>>>(
-                eng.blk.p + eng.ndense, na, keff, eng.voff.p, eng.toff.p, V, eng.xs.p, eng.T.p);
+            lr_v_kernel<<<grid_for(eng.n_lv_items, kWaves), kThreads, 0, s>>>(
+                eng.lv_items.p, eng.n_lv_items, V, eng.xs.p, eng.Tp.p);
+            HMG_HIP(hipGetLastError());
+            lr_v_finalize_kernel<<<grid_for(eng.n_lv_slices, kWaves), kThreads, 0, s>>>(
+                eng.lv_slices.p, eng.n_lv_slices, eng.Tp.p, eng.tmap.p, eng.T.p);
             HMG_HIP(hipGetLastError());
             eng.prof.end(PROF_LR_VTX, s);
+            eng.prof.begin(PROF_LR_U, s);
+            lr_u_kernel<<<grid_for(eng.n_lr_items, kWaves), kThreads, 0, s>>>(
+                eng.lr_items.p, eng.n_lr_items, U, eng.T.p, eng.P.p + (int64_t)eng.n_near_items * kNearRows, 0);
+            HMG_HIP(hipGetLastError());
+            eng.prof.end(PROF_LR_U, s);
         }
-        eng.prof.begin(PROF_ROW, s);
-        row_kernel<<<grid_for(eng.ntiles, kWaves), kThreads, 0, s>>>(
-            c1, c2, kp, eng.ntiles, n1, eng.ndense, eng.tile_ptr.p, eng.tile_blk.p, eng.blk.p,
-            eng.xs.p, dense_stored ? eng.aoff.p : nullptr, dense_stored ? eng.A.p : nullptr, keff,
-            eng.uoff.p, eng.toff.p, U, eng.T.p, order1, yc);
+        HMG_HIP(hipStreamWaitEvent(s, eng.ev_join, 0));
+        eng.prof.begin(PROF_COMBINE, s);
+        combine_kernel<<<grid_for(n1, kThreads), kThreads, 0, s>>>(eng.row_ptr.p, eng.row_idx.p,
+                                                                   eng.P.p, order1, yc, n1);
         HMG_HIP(hipGetLastError());
-        eng.prof.end(PROF_ROW, s);
+        eng.prof.end(PROF_COMBINE, s);
     }
     return HMG_OK;
 }
@@ -332,3 +530,17 @@ hmg_status inverse_perm(const uint64_t* order, int64_t* inv, int64_t n, hipStrea
 }
 
 }  // namespace hmg
+
+extern "C" hmg_status hmg_copy_probe(const double* src, double* dst, int64_t n)
+{
+    if (!src || !dst || n < 0) {
+        hmg::set_error("invalid copy probe arguments");
+        return HMG_ERR_INVALID;
+    }
+    if (n == 0) return HMG_OK;
+    const unsigned grid = (unsigned)std::min<int64_t>((n + hmg::kThreads - 1) / hmg::kThreads, 1 << 16);
+    hmg::copy_probe_kernel<<<grid, hmg::kThreads>>>(src, dst, n);
+    HMG_HIP(hipGetLastError());
+    HMG_HIP(hipDeviceSynchronize());
+    return HMG_OK;
+}

@@ -186,10 +186,17 @@ class HMatrix:
             memcpy_dtoh(out, ctypes.cast(self.data.mat_vec_data[0], ctypes.c_void_p).value)
         return out
 
+    def schedule(self):
+        out = (ctypes.c_int64 * 7)()
+        _abi.check(self.L.hmg_get_schedule(ctypes.byref(self.data), out), "hmg_get_schedule")
+        return {"near_items": out[0], "lr_items": out[1], "lr_rows": out[2], "lr_t_reads": out[3],
+                "lr_m1": out[4], "lr_m2": out[5], "partials": out[6]}
+
     def set_profiling(self, on=True):
         _abi.check(self.L.hmg_set_profiling(ctypes.byref(self.data), 1 if on else 0), "hmg_set_profiling")
 
-    PROFILE_CLASSES = {"gather": 0, "lr_vtx": 1, "row": 2, "aca": 3, "dense_fill": 4}
+    PROFILE_CLASSES = {"gather": 0, "lr_vtx": 1, "near": 2, "aca": 3, "dense_fill": 4, "lr_u": 5,
+                       "combine": 6}
 
     def profile(self):
         out = {}

@@ -138,13 +138,22 @@ hmg_status hmg_dense_rows(const double* X, int nrhs, const int64_t* rows, int nr
  * out[5] = levels on which a size-1 cluster changed the bounding-box table,
  * out[6] = sum_aca m1*rank, out[7] = sum_aca m2*rank, out[8] = sum_aca rank. */
 hmg_status hmg_get_work(const struct h_matrix_data* data, int64_t out[9]);
-/* Per-kernel timing with HIP events recorded on the engine's stream around each
+/* Shape of the H-MVP schedule: out[0] = near-field work items, out[1] = low-rank
+ * work items, out[2] = rows over low-rank items, out[3] = t values read by the
+ * low-rank items (sum of K over items), out[4] = sum_aca m1, out[5] = sum_aca m2,
+ * out[6] = partial sums combined per product (entries of the per-row lists). */
+hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[7]);
+/* Per-kernel timing with HIP events recorded on the engine's streams around each
  * launch.  enable != 0 starts (and clears) recording.  Kernel classes: 0 = x
- * permutation, 1 = low-rank V^T x, 2 = row tiles (near field + U t), 3 = ACA,
- * 4 = near-field fill.  Returns the summed milliseconds and the launch count. */
+ * permutation, 1 = low-rank V^T x, 2 = near field, 3 = ACA, 4 = near-field fill,
+ * 5 = low-rank U t, 6 = per-row combine.  Returns the summed milliseconds and the
+ * launch count. */
 hmg_status hmg_set_profiling(struct h_matrix_data* data, int enable);
 hmg_status hmg_get_profile(const struct h_matrix_data* data, int kernel_class, double* total_ms,
                            int64_t* launches);
+/* Measurement helper: a plain device copy dst[i] = src[i] of n doubles with the
+ * engine's streaming launch shape, used to calibrate HBM byte counters. */
+hmg_status hmg_copy_probe(const double* src, double* dst, int64_t n);
 /* last error message of this thread */
 const char* hmg_last_error(void);
 /* sizeof / field offsets of the public structs, for FFI layout checks */

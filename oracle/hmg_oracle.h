@@ -106,8 +106,8 @@ int orc_level_count(const orc_hm* hm);
 /* kernel entry on sorted indices (kernel_system_assembler.h:38-52,82-96) */
 double orc_kernel_entry(const orc_hm* hm, const orc_kernel* k, int i, int j);
 
-/* Batched ACA in reference mode (linear_algebra.cu:3684-4012, Appendix A.6 of the
- * survey): rank k_b = min(k, m1, m2), sequential row pivots skipping rows whose
+/* Batched ACA in reference mode (linear_algebra.cu:3684-4012, kernels :918-1147,
+ * :2009-2222): rank k_b = min(k, m1, m2), sequential row pivots skipping rows whose
  * residual 2-norm < 1e-13, column pivot = LAST argmax (tie_last=1) or FIRST argmax
  * (tie_last=0, the unbatched path linear_algebra.cu:473).  Factors are stored per
  * block, column-major: U_b (m1 x k_b) at uoff[b], V_b (m2 x k_b) at voff[b]. */
