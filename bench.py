@@ -49,10 +49,14 @@ def kernel_roofline(hm, prof, steps_prof):
     #   lr_u:   U once (8 B x sum m1*rank) + t once per 64-row item + partials
     bytes_vtx = 8 * int(w["lr_m2_rank"] + sch["lr_m2"] + w["rank_sum"])
     bytes_lru = 8 * int(w["lr_m1_rank"] + sch["lr_t_reads"] + sch["lr_rows"])
-    # FP64 flops of the matrix-free near field per entry: d subtractions + d FMAs
-    # (2 flops each) + sqrt (10) + exp (20) + scale (1) + accumulate FMA (2)
-    c_eval = 3 * hm.dim + 10 + 20 + 1 + 2
-    flops_near = c_eval * e_near
+    # FP64 flops of the matrix-free near field per evaluated entry: d subtractions
+    # + d FMAs (2 flops each) + sqrt (10) + exp (20) + scale (1); every matrix
+    # entry is also accumulated (FMA, 2 flops).  With the same points on both
+    # sides each transposed pair of DENSE blocks is evaluated once
+    # (near_evals < E_near) and feeds a row and a column sum.
+    c_eval = 3 * hm.dim + 10 + 20 + 1
+    evals = int(sch["near_evals"]) or e_near
+    flops_near = c_eval * evals + 2 * e_near
 
     def avg(name):
         ms, cnt = prof[name]
@@ -62,7 +66,8 @@ def kernel_roofline(hm, prof, steps_prof):
     res = {
         "lr_vtx": {"ms": t["lr_vtx"], "bytes": bytes_vtx, "GBs": bytes_vtx / (t["lr_vtx"] * 1e-3) / 1e9},
         "lr_u": {"ms": t["lr_u"], "bytes": bytes_lru, "GBs": bytes_lru / (t["lr_u"] * 1e-3) / 1e9},
-        "near": {"ms": t["near"], "flops": flops_near, "c_eval": c_eval,
+        "near": {"ms": t["near"], "flops": flops_near, "c_eval": c_eval, "evals": evals,
+                 "symmetric": bool(sch["near_symmetric"]),
                  "TFLOPs": flops_near / (t["near"] * 1e-3) / 1e12},
         "gather_ms": t["gather"], "combine_ms": t["combine"],
     }
@@ -75,7 +80,8 @@ def kernel_roofline(hm, prof, steps_prof):
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
                 "traffic": traffic,
                 "note": f"FP64 near field evaluated on the fly (VALU, not MFMA); {c_eval} FP64 "
-                        f"flops per entry; peak = MI355X FP64 (vector and matrix rates are equal)"}
+                        f"flops per evaluated entry ({evals} evaluations) + 2 per matrix entry "
+                        f"({e_near}); peak = MI355X FP64 (vector and matrix rates are equal)"}
     else:
         ach = res[dominant]["GBs"]
         roof = {"kernel": kname, "bound": "hbm", "achieved": ach,

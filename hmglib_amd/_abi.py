@@ -18,6 +18,10 @@ KERNEL_MATERN_REF = 2
 KERNEL_GAUSSIAN = 3
 KERNEL_EXPONENTIAL = 4
 
+OPT_NEAR_SYMMETRIC = 1
+OPT_NEAR_LDS_BYTES = 2
+OPT_SERIALIZE = 3
+
 HMG_OK = 0
 HMG_ERR_INVALID = 1
 HMG_ERR_OOM = 2
@@ -126,6 +130,7 @@ SIGNATURES = {
     "hmg_get_work": (ctypes.c_int, [_D, _P(ctypes.c_int64)]),
     "hmg_get_schedule": (ctypes.c_int, [_D, _P(ctypes.c_int64)]),
     "hmg_set_profiling": (ctypes.c_int, [_D, ctypes.c_int]),
+    "hmg_set_option": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int64]),
     "hmg_get_profile": (ctypes.c_int, [_D, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64)]),
     "hmg_copy_probe": (ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
     "hmg_last_error": (ctypes.c_char_p, []),

@@ -28,12 +28,13 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Factor layout: U_b is column-major m1 x k_b at U + uoff[a] (leading dim m1);
-// V_b is stored transposed, V_b[j, l] at V + voff[a] + j * ldv[a] + l, i.e. row j
-// of V_b is contiguous (ldv = total rank of the column cluster's blocks).
+// Factor layout: U_b[i, l] at U + uoff[a] + l * ldu[a] + i (column-major, ldu >= m1:
+// the row cluster's padded height); V_b is stored transposed, V_b[j, l] at
+// V + voff[a] + j * ldv[a] + l, i.e. row j of V_b is contiguous (ldv >= total rank
+// of the column cluster's blocks).
 __global__ void __launch_bounds__(kAcaThreads)
 aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, int naca,
-           const int* __restrict__ kb, const int64_t* __restrict__ uoff,
+           const int* __restrict__ kb, const int64_t* __restrict__ uoff, const int* __restrict__ ldu,
            const int64_t* __restrict__ voff, const int* __restrict__ ldv, double* __restrict__ U,
            double* __restrict__ V, int* __restrict__ keff, int tie_last)
 {
@@ -45,6 +46,7 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
     const int l2 = b.z, m2 = b.w - b.z + 1;
     const int k = kb[a];
     const int64_t ld = ldv[a];
+    const int64_t lu = ldu[a];
     double* __restrict__ Ub = U + uoff[a];
     double* __restrict__ Vb = V + voff[a];
     int i_r = -1;
@@ -62,7 +64,7 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
             for (int j = lane; j < m2; j += kWave) {
                 double val = kernel_entry(kp, c1, l1 + i_r, c2, l2 + j);
                 const double* __restrict__ vrow = Vb + (int64_t)j * ld;
-                for (int l = 0; l < r; l++) val = __fma_rn(-Ub[(int64_t)l * m1 + i_r], vrow[l], val);
+                for (int l = 0; l < r; l++) val = __fma_rn(-Ub[l * lu + i_r], vrow[l], val);
                 Vb[(int64_t)j * ld + r] = val;
                 nrm = __fma_rn(val, val, nrm);
             }
@@ -105,10 +107,10 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
         __threadfence_block();
         // residual column (batched_fill_kernel_vector_and_scaled_substraction_for_u_r)
         const double* __restrict__ vpiv = Vb + (int64_t)jr * ld;
-        double* __restrict__ u = Ub + (int64_t)r * m1;
+        double* __restrict__ u = Ub + r * lu;
         for (int i = lane; i < m1; i += kWave) {
             double val = kernel_entry(kp, c1, l1 + i, c2, l2 + jr);
-            for (int l = 0; l < r; l++) val = __fma_rn(-vpiv[l], Ub[(int64_t)l * m1 + i], val);
+            for (int l = 0; l < r; l++) val = __fma_rn(-vpiv[l], Ub[l * lu + i], val);
             u[i] = val;
         }
         __threadfence_block();
@@ -129,7 +131,7 @@ hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const Kern
     const unsigned grid = (unsigned)((na + kAcaWaves - 1) / kAcaWaves);
     eng.prof.begin(PROF_ACA, s);
     aca_kernel<<<grid, kAcaThreads, 0, s>>>(c1, c2, kp, eng.blk.p + eng.ndense, na, eng.kb.p,
-                                            eng.uoff.p, eng.voff.p, eng.ldv.p, U, V, keff, tie_last);
+                                            eng.uoff.p, eng.ldu.p, eng.voff.p, eng.ldv.p, U, V, keff, tie_last);
     HMG_HIP(hipGetLastError());
     eng.prof.end(PROF_ACA, s);
     return HMG_OK;

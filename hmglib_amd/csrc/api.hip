@@ -175,9 +175,25 @@ void release_build(Engine& e, h_matrix_data* d)
     e.n_lv_items = e.n_lv_slices = 0;
     e.near_items.release();
     e.near_blk.release();
+    e.h_near_items.clear();
+    e.fly_items.release();
+    e.fly_refs.release();
+    e.h_fly_items.clear();
+    e.h_fly_refs.clear();
+    e.h_col_groups.clear();
+    e.n_fly_items = 0;
+    e.n_col_slots = 0;
+    e.near_sym = false;
     e.lr_items.release();
-    e.row_ptr.release();
-    e.row_idx.release();
+    e.ldu.release();
+    e.h_ldu.clear();
+    for (Combine* c : {&e.comb_stored, &e.comb_fly}) {
+        c->ptr.release();
+        c->idx.release();
+        c->groups.release();
+        c->total = 0;
+        c->slots = 0;
+    }
     e.P.release();
     e.xs.release();
     e.T.release();
@@ -339,6 +355,22 @@ hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
         HMG_HIP(hipStreamSynchronize(s));
         tmp.release();
         e->perm32[i].release();
+    }
+    // the same point set on both sides (sorted coordinates bitwise equal) makes
+    // the kernel matrix bitwise symmetric outside the diagonal
+    e->same_sets = false;
+    if (data->point_count[0] == data->point_count[1] && data->point_count[0] > 0) {
+        DevBuf<int> flag;
+        HMG_TRY(flag.alloc(1));
+        HMG_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+        for (int d = 0; d < dim; d++)
+            HMG_TRY(flag_differ_f64(data->coords_d[0][d], data->coords_d[1][d], data->point_count[0],
+                                    flag.p, s));
+        int differ = 1;
+        HMG_HIP(hipMemcpyAsync(&differ, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HMG_HIP(hipStreamSynchronize(s));
+        flag.release();
+        e->same_sets = differ == 0;
     }
     // block-cluster tree (tree.cu:1429-1557) and DENSE-first partition
     TreeParams tp{data->eta, data->max_level, data->c_leaf, (int)data->root_level_set_1,
@@ -600,16 +632,47 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[7])
         m1s += b.y - b.x + 1;
         m2s += b.w - b.z + 1;
     }
-    int total = 0;
-    if (e->n[0] > 0) HMG_HIP(hipMemcpy(&total, e->row_ptr.p + e->n[0], sizeof(int), hipMemcpyDeviceToHost));
-    out[0] = e->n_near_items;
+    const bool stored = data->dA != nullptr && e->dense_stored;
+    out[0] = stored ? e->n_near_items : e->n_fly_items;
     out[1] = e->n_lr_items;
     out[2] = rows;
     out[3] = treads;
     out[4] = m1s;
     out[5] = m2s;
-    out[6] = total;
+    const Combine& cb = stored ? e->comb_stored : e->comb_fly;
+    out[6] = cb.total - cb.group_entries + cb.slots;
+    out[7] = e->near_sym ? 1 : 0;
+    out[8] = e->n_col_slots;
+    out[9] = e->same_sets ? 1 : 0;
+    out[10] = stored ? 0 : e->near_evals;
     return HMG_OK;
+}
+
+hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value)
+{
+    Engine* e = engine_of(data);
+    if (!e) {
+        set_error("h_matrix_data not initialised");
+        return HMG_ERR_NOT_READY;
+    }
+    switch (option) {
+    case HMG_OPT_NEAR_SYMMETRIC:
+        e->near_sym_enabled = value != 0;
+        return HMG_OK;
+    case HMG_OPT_NEAR_LDS_BYTES:
+        if (value < 0 || value > 128 * 1024) {
+            set_error("near-field LDS reservation must be in [0, 131072] bytes");
+            return HMG_ERR_INVALID;
+        }
+        e->near_lds_pad = (int)value;
+        return HMG_OK;
+    case HMG_OPT_SERIALIZE:
+        e->serialize = value != 0;
+        return HMG_OK;
+    default:
+        set_error("unknown option " + std::to_string(option));
+        return HMG_ERR_INVALID;
+    }
 }
 
 hmg_status hmg_set_profiling(struct h_matrix_data* data, int enable)

@@ -6,11 +6,11 @@
 //   codes      per set, N uint64 Morton codes (sorted), order: N uint64
 //   blocks     struct work_item[nblocks] (192 B, DENSE first, then ACA, BFS order)
 //   blk        int4 {l1,u1,l2,u2} per block, same order as `blocks`
-//   U_b, V_b   per ACA block, column-major m1 x k_b and m2 x k_b, contiguous,
-//              at uoff[a] / voff[a] (int64 element offsets)
+//   U, V       ACA factors grouped by row cluster (U, column-major) and by column
+//              cluster (V, row-major), leading dims padded to 16 doubles
 //   A_b        per DENSE block (when precomputed), column-major m1 x m2 at aoff[b]
-//   tiles      row tiles of 64 Morton-ordered rows; CSR list per tile of the blocks
-//              whose row range intersects it (block order)
+//   P          partial row / column sums of the MVP work items, combined per row
+//              through a CSR list in a fixed order
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -101,11 +101,12 @@ struct Profiler {
 };
 
 // low-rank work item: rows [row0, row0+nrows) of the row cluster's U matrix
-// (m1 rows, K columns, column-major, U + uoff points at row0), times t at T + toff
+// (K columns, column-major with leading dim ld, U + uoff points at row0), times
+// t at T + toff
 struct LrItem {
     int row0;
     int nrows;
-    int m1;
+    int ld;
     int K;
     int64_t uoff;
     int64_t toff;
@@ -129,6 +130,45 @@ struct LvSlice {
     int tcol0;
 };
 
+// Matrix-free near field.  A work item covers up to 128 rows [row0, row0+nrows)
+// of one row cluster against the column ranges of its refs.  Refs [ref0, ref_sym)
+// contribute row sums only; refs [ref_sym, ref1) are the upper halves of
+// transpose pairs (same point set on both sides, disjoint clusters): each entry
+// is evaluated once and feeds the row sum of its row and the column sum of its
+// column, written to P[cbase + column offset].  ncols_sym = total columns of
+// the symmetric refs (they are streamed as one column sequence).
+struct NearRef {
+    int l2;
+    int m2;
+    int64_t cbase;
+};
+struct NearItem {
+    int row0;
+    int nrows;
+    int ref0;
+    int ref_sym;
+    int ref1;
+    int ncols_sym;
+};
+
+// symmetric column slots of one column cluster: contribution k of point l2 + j
+// sits at P[base + k*m2 + j], k < count
+struct ColSlots {
+    int l2, m2;
+    int64_t base;
+    int count;
+};
+
+// per-point partial lists of one MVP schedule (CSR over sorted rows): an entry
+// v >= 0 is the P slot v, an entry v < 0 the column-slot group -(v+1)
+struct Combine {
+    DevBuf<int> ptr, idx;
+    DevBuf<ColSlots> groups;
+    int64_t total = 0;          // list entries
+    int64_t group_entries = 0;  // entries naming a column-slot group
+    int64_t slots = 0;          // column slots reached through the groups
+};
+
 // ---- engine state hung off h_matrix_data::engine ---------------------------
 struct Engine {
     Profiler prof;
@@ -147,14 +187,17 @@ struct Engine {
     int quirk_levels = 0;
 
     // ACA factors (FP64).  U of all ACA blocks that share a row cluster C is one
-    // column-major m1(C) x K(C) matrix (blocks in block order, k_b columns each);
+    // column-major m1(C) x K(C) matrix with leading dim ldu = m1 rounded up to 16
+    // (blocks in block order, k_b columns each; U_b[i,l] at uoff[a] + l*ldu[a] + i);
     // V of all ACA blocks that share a column cluster D is one row-major
-    // m2(D) x K(D) matrix (V_b[j,l] at voff[a] + j*ldv[a] + l).  T (V^T x) is laid
-    // out in U order; tmap sends V columns to T.
+    // m2(D) x K(D) matrix with leading dim ldv = K rounded up to 16 (V_b[j,l] at
+    // voff[a] + j*ldv[a] + l).  Row starts on 128-byte lines keep every 64-lane
+    // segment of the MVP streams on 4 lines.  T (V^T x) is laid out in U order;
+    // tmap sends V columns to T.
     int aca_k = -1;                             // k used for the stored factors
-    std::vector<int> h_kb, h_ldv;
-    std::vector<int64_t> h_uoff, h_voff, h_toff; // per ACA block
-    DevBuf<int> kb, keff, ldv;
+    std::vector<int> h_kb, h_ldv, h_ldu;
+    std::vector<int64_t> h_uoff, h_voff, h_toff; // per ACA block (+ total at [naca])
+    DevBuf<int> kb, keff, ldv, ldu;
     DevBuf<int64_t> uoff, voff, toff;
     DevBuf<double> U, V;
     int kmax = 0;
@@ -172,26 +215,41 @@ struct Engine {
     std::vector<int> h_agrp_ptr, h_agrp_blk;    // ACA groups (indices into the ACA list)
     std::vector<int> h_cgrp_ptr, h_cgrp_blk;    // ACA blocks grouped by column cluster
 
-    // MVP schedule: work items write 64-row partials into P; every row sums its
-    // partials in a fixed order (CSR rowptr/rowidx) and writes y once.
+    // MVP schedule: work items write partial row (and column) sums into P; every
+    // row sums its partials in a fixed order (CSR) and writes y once.  P layout:
+    // [low-rank items x 64][near items x 128][symmetric column slots].
     int n_near_items = 0, n_lr_items = 0, n_lv_items = 0, n_lv_slices = 0;
-    DevBuf<int4> near_items;                    // {row0, nrows, b0, b1} into near_blk
+    DevBuf<int4> near_items;                    // stored near field: {row0, nrows, b0, b1}
     DevBuf<int> near_blk;
+    std::vector<int4> h_near_items;
+    // matrix-free near field (symmetric when both sets are the same points)
+    bool same_sets = false;                     // sorted coordinates bitwise equal
+    bool near_sym_enabled = true;               // option: allow the symmetric form
+    bool near_sym = false;                      // the fly schedule uses transpose pairs
+    std::vector<ColSlots> h_col_groups;         // column slot groups (bases relative)
+    int n_fly_items = 0;
+    int64_t n_col_slots = 0;
+    int64_t near_evals = 0;                     // kernel entries evaluated per matrix-free pass
+    std::vector<NearItem> h_fly_items;
+    std::vector<NearRef> h_fly_refs;
+    DevBuf<NearItem> fly_items;
+    DevBuf<NearRef> fly_refs;
     DevBuf<LrItem> lr_items;
     DevBuf<LvItem> lv_items;
     DevBuf<LvSlice> lv_slices;
     DevBuf<int> tmap;
     DevBuf<double> Tp;                          // V^T x partials per row chunk
-    DevBuf<int> row_ptr, row_idx;
+    Combine comb_stored, comb_fly;              // stored near field / matrix-free
     DevBuf<double> P, xs, T;
 
     hipStream_t stream2 = nullptr;              // near field runs beside the low-rank part
     // tuning: LDS bytes reserved per near-field workgroup (caps its occupancy so
     // the low-rank streams keep wave slots on every CU); run the near field on
     // the main stream instead (no overlap)
-    // 40 KB + the kernel's own 12 KB leave room for 3 near-field workgroups per CU
-    // (measured best on MI355X at N = 2^20: 60.9 ms vs 68.9 ms without the cap)
-    int near_lds_pad = 40 * 1024;
+    // 64 KB + the kernel's own 12 KB leave room for 2 near-field workgroups per CU
+    // (measured on MI355X at N = 2^20, 3D, symmetric near field: 42.8 ms per
+    // product vs 46.2 ms with 3 workgroups per CU and 57.9 ms without overlap)
+    int near_lds_pad = 64 * 1024;
     bool serialize = false;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
@@ -209,6 +267,8 @@ hmg_status radix_sort_u64(uint64_t* keys, uint32_t* vals, int64_t n, int key_bit
                           hipStream_t s);
 // exact per-dimension min/max of a double array
 hmg_status minmax_f64(const double* x, int64_t n, double* dmin, double* dmax, hipStream_t s);
+// *differ = 1 if a[i] and b[i] differ bitwise for some i < n (left untouched otherwise)
+hmg_status flag_differ_f64(const double* a, const double* b, int64_t n, int* differ, hipStream_t s);
 
 // ---- Morton (morton.hip) ------------------------------------------------------
 hmg_status morton_codes(const Coords& c, int dim, int bits, const double* dmin,
@@ -236,7 +296,8 @@ hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const Kern
                      int k, int tie_last, double* U, double* V, int* keff);
 
 // ---- schedule (schedule.hip, host) ---------------------------------------------------
-// row-cluster groups and near-field work items (once per tree)
+// row-cluster groups and near-field work items (once per tree); the matrix-free
+// schedule pairs transposed DENSE blocks when eng.same_sets
 hmg_status build_groups(Engine& eng);
 // ranks k_b = min(k,m1,m2), factor offsets, low-rank work items, per-row partial lists
 hmg_status aca_layout(Engine& eng, int k);

@@ -99,10 +99,10 @@ lr_v_finalize_kernel(const LvSlice* __restrict__ slices, int nslices, const doub
     T[tmap[sl.tcol0 + lane]] = s;
 }
 
-// y_C rows [row0, row0+nrows) = U_C[rows, :] t_C  ->  P[(pbase + item) * 64 + lane]
+// y_C rows [row0, row0+nrows) = U_C[rows, :] t_C  ->  P[item * 64 + lane]
 __global__ void __launch_bounds__(kThreads)
 lr_u_kernel(const LrItem* __restrict__ items, int nitems, const double* __restrict__ U,
-            const double* __restrict__ T, double* __restrict__ P, int pbase)
+            const double* __restrict__ T, double* __restrict__ P)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int q = blockIdx.x * kWaves + wave_id();
@@ -111,7 +111,7 @@ lr_u_kernel(const LrItem* __restrict__ items, int nitems, const double* __restri
     const bool act = lane < it.nrows;
     const double* __restrict__ u = U + it.uoff + (act ? lane : 0);
     const double* __restrict__ t = T + it.toff;
-    const int64_t m1 = it.m1;
+    const int64_t m1 = it.ld;
     const int K = it.K;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int l = 0;
@@ -129,7 +129,7 @@ lr_u_kernel(const LrItem* __restrict__ items, int nitems, const double* __restri
         s3 = __fma_rn(u7, t[l + 7], s3);
     }
     for (; l < K; l++) s0 = __fma_rn(u[l * m1], t[l], s0);
-    if (act) P[(int64_t)(pbase + q) * kTileRows + lane] = (s0 + s1) + (s2 + s3);
+    if (act) P[(int64_t)q * kTileRows + lane] = (s0 + s1) + (s2 + s3);
 }
 
 template <int KIND>
@@ -151,12 +151,35 @@ __device__ __forceinline__ double kernel_value(const KernelParams& kp, double r2
 // evaluation chains per staged column.  DIM = 0 means "dim at run time".
 constexpr int kNearRows = 2 * kWave;
 
+// lane i receives lane (i + 1) mod 64 of v (DPP wave_rol:1)
+__device__ __forceinline__ double lane_rotate_down(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x134, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x134, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+template <int KIND, int DIM>
+__device__ __forceinline__ double entry_value(const KernelParams& kp, const double* xi, const double* xj)
+{
+    const int dim = DIM > 0 ? DIM : kp.dim;
+    double r2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < MAX_POINT_DIMENSION; d++) {
+        if (d < dim) {
+            const double t = xi[d] - xj[d];
+            r2 = __fma_rn(t, t, r2);
+        }
+    }
+    return kernel_value<KIND>(kp, r2);
+}
+
 template <int KIND, int DIM, bool REG, int NR>
 __device__ __forceinline__ void near_fly_item(const Coords& c1, const Coords& c2, const KernelParams& kp,
-                                              const int4 it, const int* __restrict__ near_blk,
-                                              const int4* __restrict__ blk,
+                                              const NearItem& it, const NearRef* __restrict__ refs,
                                               const double* __restrict__ xs,
-                                              double (*sh)[kWave], int lane, double* __restrict__ out)
+                                              double (*sh)[kWave], int lane, double* __restrict__ out,
+                                              double* __restrict__ P)
 {
     constexpr int kS = MAX_POINT_DIMENSION + 1;
     const int dim = DIM > 0 ? DIM : kp.dim;
@@ -166,15 +189,69 @@ __device__ __forceinline__ void near_fly_item(const Coords& c1, const Coords& c2
     double acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-        act[r] = lane + r * kWave < it.y;
-        row[r] = it.x + (act[r] ? lane + r * kWave : 0);
+        act[r] = lane + r * kWave < it.nrows;
+        row[r] = it.row0 + (act[r] ? lane + r * kWave : 0);
         acc[r] = 0.0;
 #pragma unroll
         for (int d = 0; d < MAX_POINT_DIMENSION; d++) xi[r][d] = d < dim ? c1.x[d][row[r]] : 0.0;
     }
-    for (int bi = it.z; bi < it.w; bi++) {
-        const int4 b = blk[near_blk[bi]];
-        const int l2 = b.z, m2 = b.w - b.z + 1;
+    // symmetric refs: one column stream over all their columns, 64 columns staged
+    // at a time (lane L stages column L).  Step s pairs lane L with staged column
+    // (L + s) mod 64; the column sums travel one lane down per step, so after 64
+    // steps lane L holds the sum of column L over this item's rows.
+    if (it.ncols_sym > 0) {
+        double xr[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) xr[r] = act[r] ? xs[row[r]] : 0.0;
+        int rc = it.ref_sym, rc_start = 0;  // first ref not yet fully staged, its stream offset
+        for (int t0 = 0; t0 < it.ncols_sym; t0 += kWave) {
+            const int pos = t0 + lane;
+            int j = -1;
+            int64_t slot = 0;
+            int r = rc, rs = rc_start;
+            while (r < it.ref1 && rs < t0 + kWave) {
+                const NearRef f = refs[r];
+                const int re = rs + f.m2;
+                if (pos >= rs && pos < re) {
+                    j = f.l2 + (pos - rs);
+                    slot = f.cbase + (pos - rs);
+                }
+                if (re <= t0 + kWave) {
+                    rc = r + 1;
+                    rc_start = re;
+                }
+                rs = re;
+                r++;
+            }
+            // past the stream's end: any valid point with x = 0 (contributes nothing)
+            const int jj = j >= 0 ? j : 0;
+            wave_barrier();
+#pragma unroll
+            for (int d = 0; d < MAX_POINT_DIMENSION; d++)
+                if (d < dim) sh[d][lane] = c2.x[d][jj];
+            sh[kS - 1][lane] = j >= 0 ? xs[jj] : 0.0;
+            wave_barrier();
+            double colacc = 0.0;
+            for (int s = 0; s < kWave; s++) {
+                const int c = (lane + s) & (kWave - 1);
+                double xj[MAX_POINT_DIMENSION];
+#pragma unroll
+                for (int d = 0; d < MAX_POINT_DIMENSION; d++) xj[d] = d < dim ? sh[d][c] : 0.0;
+                const double xv = sh[kS - 1][c];
+#pragma unroll
+                for (int q = 0; q < NR; q++) {
+                    const double kv = entry_value<KIND, DIM>(kp, xi[q], xj);
+                    acc[q] = __fma_rn(kv, xv, acc[q]);
+                    colacc = __fma_rn(kv, xr[q], colacc);
+                }
+                colacc = lane_rotate_down(colacc);
+            }
+            if (j >= 0) P[slot] = colacc;
+        }
+    }
+    for (int bi = it.ref0; bi < it.ref_sym; bi++) {
+        const NearRef f = refs[bi];
+        const int l2 = f.l2, m2 = f.m2;
         for (int j0 = 0; j0 < m2; j0 += kWave) {
             const int cnt = min(kWave, m2 - j0);
             wave_barrier();
@@ -212,24 +289,25 @@ __device__ __forceinline__ void near_fly_item(const Coords& c1, const Coords& c2
         if (act[r]) out[lane + r * kWave] = acc[r];
 }
 
-// near field evaluated on the fly; P[q*128 + row offset] for work item q
+// near field evaluated on the fly: row sums of item q at Pnear[q*128 + row offset],
+// symmetric column sums at P[ref cbase + column offset]
 template <int KIND, int DIM, bool REG>
 __global__ void __launch_bounds__(kThreads)
-near_fly_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ items, int nitems,
-                const int* __restrict__ near_blk, const int4* __restrict__ blk,
-                const double* __restrict__ xs, double* __restrict__ P)
+near_fly_kernel(Coords c1, Coords c2, KernelParams kp, const NearItem* __restrict__ items, int nitems,
+                const NearRef* __restrict__ refs, const double* __restrict__ xs,
+                double* __restrict__ Pnear, double* __restrict__ P)
 {
     __shared__ double sh[kWaves][MAX_POINT_DIMENSION + 1][kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = wave_id();
     const int q = blockIdx.x * kWaves + w;
     if (q >= nitems) return;
-    const int4 it = items[q];
-    double* out = P + (int64_t)q * kNearRows;
-    if (it.y > kWave)
-        near_fly_item<KIND, DIM, REG, 2>(c1, c2, kp, it, near_blk, blk, xs, sh[w], lane, out);
+    const NearItem it = items[q];
+    double* out = Pnear + (int64_t)q * kNearRows;
+    if (it.nrows > kWave)
+        near_fly_item<KIND, DIM, REG, 2>(c1, c2, kp, it, refs, xs, sh[w], lane, out, P);
     else
-        near_fly_item<KIND, DIM, REG, 1>(c1, c2, kp, it, near_blk, blk, xs, sh[w], lane, out);
+        near_fly_item<KIND, DIM, REG, 1>(c1, c2, kp, it, refs, xs, sh[w], lane, out, P);
 }
 
 // near field from the stored blocks (column-major m1 x m2 per dense block)
@@ -267,15 +345,25 @@ near_stored_kernel(const int4* __restrict__ items, int nitems, const int* __rest
     }
 }
 
-// y[order1[i]] = sum of row i's partials, in list order
+// y[order1[i]] = sum of row i's partials, in list order (a column-slot group
+// contributes its count slots of row i in order)
 __global__ void combine_kernel(const int* __restrict__ ptr, const int* __restrict__ idx,
-                               const double* __restrict__ P, const uint64_t* __restrict__ order1,
-                               double* __restrict__ y, int n1)
+                               const ColSlots* __restrict__ groups, const double* __restrict__ P,
+                               const uint64_t* __restrict__ order1, double* __restrict__ y, int n1)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n1) return;
     double s = 0.0;
-    for (int q = ptr[i]; q < ptr[i + 1]; q++) s += P[idx[q]];
+    for (int q = ptr[i]; q < ptr[i + 1]; q++) {
+        const int v = idx[q];
+        if (v >= 0) {
+            s += P[v];
+        } else {
+            const ColSlots g = groups[-v - 1];
+            const double* __restrict__ p = P + g.base + (i - g.l2);
+            for (int k = 0; k < g.count; k++) s += p[(int64_t)k * g.m2];
+        }
+    }
     y[order1[i]] = s;
 }
 
@@ -384,12 +472,13 @@ template <int KIND, int DIM>
 void launch_near_fly(bool reg, unsigned grid, hipStream_t s, const Coords& c1, const Coords& c2,
                      const KernelParams& kp, const Engine& eng, double* P)
 {
+    double* Pnear = P + (int64_t)eng.n_lr_items * kTileRows;
     if (reg)
         near_fly_kernel<KIND, DIM, true><<<grid, kThreads, eng.near_lds_pad, s>>>(
-            c1, c2, kp, eng.near_items.p, eng.n_near_items, eng.near_blk.p, eng.blk.p, eng.xs.p, P);
+            c1, c2, kp, eng.fly_items.p, eng.n_fly_items, eng.fly_refs.p, eng.xs.p, Pnear, P);
     else
         near_fly_kernel<KIND, DIM, false><<<grid, kThreads, eng.near_lds_pad, s>>>(
-            c1, c2, kp, eng.near_items.p, eng.n_near_items, eng.near_blk.p, eng.blk.p, eng.xs.p, P);
+            c1, c2, kp, eng.fly_items.p, eng.n_fly_items, eng.fly_refs.p, eng.xs.p, Pnear, P);
 }
 
 template <int KIND>
@@ -425,13 +514,14 @@ hmg_status hmvp(Engine& eng, const Coords& c1, const Coords& c2, const KernelPar
         // near field on the second stream
         HMG_HIP(hipEventRecord(eng.ev_fork, s));
         HMG_HIP(hipStreamWaitEvent(s2, eng.ev_fork, 0));
-        if (eng.n_near_items > 0) {
-            const unsigned g = grid_for(eng.n_near_items, kWaves);
+        const int n_items = dense_stored ? eng.n_near_items : eng.n_fly_items;
+        if (n_items > 0) {
+            const unsigned g = grid_for(n_items, kWaves);
             eng.prof.begin(PROF_NEAR, s2);
             if (dense_stored) {
-                near_stored_kernel<<<g, kThreads, eng.near_lds_pad, s2>>>(eng.near_items.p, eng.n_near_items,
-                                                          eng.near_blk.p, eng.blk.p, eng.aoff.p,
-                                                          eng.A.p, eng.xs.p, eng.P.p);
+                near_stored_kernel<<<g, kThreads, eng.near_lds_pad, s2>>>(
+                    eng.near_items.p, eng.n_near_items, eng.near_blk.p, eng.blk.p, eng.aoff.p, eng.A.p,
+                    eng.xs.p, eng.P.p + (int64_t)eng.n_lr_items * kTileRows);
             } else {
                 switch (kp.kind) {
                 case HMG_KERNEL_GAUSSIAN:
@@ -464,13 +554,14 @@ hmg_status hmvp(Engine& eng, const Coords& c1, const Coords& c2, const KernelPar
             eng.prof.end(PROF_LR_VTX, s);
             eng.prof.begin(PROF_LR_U, s);
             lr_u_kernel<<<grid_for(eng.n_lr_items, kWaves), kThreads, 0, s>>>(
-                eng.lr_items.p, eng.n_lr_items, U, eng.T.p, eng.P.p + (int64_t)eng.n_near_items * kNearRows, 0);
+                eng.lr_items.p, eng.n_lr_items, U, eng.T.p, eng.P.p);
             HMG_HIP(hipGetLastError());
             eng.prof.end(PROF_LR_U, s);
         }
         HMG_HIP(hipStreamWaitEvent(s, eng.ev_join, 0));
         eng.prof.begin(PROF_COMBINE, s);
-        combine_kernel<<<grid_for(n1, kThreads), kThreads, 0, s>>>(eng.row_ptr.p, eng.row_idx.p,
+        const Combine& cb = dense_stored ? eng.comb_stored : eng.comb_fly;
+        combine_kernel<<<grid_for(n1, kThreads), kThreads, 0, s>>>(cb.ptr.p, cb.idx.p, cb.groups.p,
                                                                    eng.P.p, order1, yc, n1);
         HMG_HIP(hipGetLastError());
         eng.prof.end(PROF_COMBINE, s);

@@ -280,6 +280,18 @@ minmax_final_kernel(const double* __restrict__ pmin, const double* __restrict__ 
     }
 }
 
+// bitwise comparison of two double arrays: any difference sets *differ = 1
+__global__ void __launch_bounds__(kRedThreads)
+differ_kernel(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b, int64_t n,
+              int* __restrict__ differ)
+{
+    bool d = false;
+    for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kRedThreads)
+        d |= a[i] != b[i];
+    if (__any(d) && (threadIdx.x & (kWave - 1)) == 0) *differ = 1;
+}
+
 }  // namespace
 
 hmg_status scan_i32(const int* in, int* out, int64_t n, bool inclusive, int* total, hipStream_t s)
@@ -332,6 +344,15 @@ hmg_status minmax_f64(const double* x, int64_t n, double* dmin, double* dmax, hi
     minmax_partial_kernel<<<nb, kRedThreads, 0, s>>>(x, n, pmin, pmax);
     HMG_HIP(hipGetLastError());
     minmax_final_kernel<<<1, kRedThreads, 0, s>>>(pmin, pmax, nb, dmin, dmax);
+    HMG_HIP(hipGetLastError());
+    return HMG_OK;
+}
+
+hmg_status flag_differ_f64(const double* a, const double* b, int64_t n, int* differ, hipStream_t s)
+{
+    if (n <= 0) return HMG_OK;
+    const int nb = (int)std::min<int64_t>(1024, (n + kRedThreads - 1) / kRedThreads);
+    differ_kernel<<<nb, kRedThreads, 0, s>>>((const uint64_t*)a, (const uint64_t*)b, n, differ);
     HMG_HIP(hipGetLastError());
     return HMG_OK;
 }

@@ -3,11 +3,22 @@
 // Blocks are grouped by their row cluster.  All ACA blocks of one row cluster C
 // keep their U factors side by side as one column-major m1(C) x K(C) matrix, so
 // the low-rank product of C is one contiguous GEMV; all DENSE blocks of C form
-// one near-field sweep over the concatenated column ranges.  Work items cover 64
-// rows (one wavefront) of one row cluster and write a partial row sum into P;
-// every row then adds its partials in a fixed order (near items first, then
-// low-rank items, each in item order), so results are deterministic.
+// one near-field sweep over the concatenated column ranges.  Work items write
+// partial sums into P; every row then adds its partials in a fixed order (near
+// items, then low-rank items, then symmetric column slots), so results are
+// deterministic.
+//
+// When both point sets are the same points, K(C,D) = K(D,C)^T bitwise for
+// disjoint clusters C, D (the distance is computed from coordinate differences
+// whose squares do not depend on the order).  The matrix-free near field then
+// evaluates each transposed pair of DENSE blocks once: the item of (C,D) with
+// C before D produces the row sums of C and the column sums of D; (D,C) is not
+// evaluated.  Diagonal and overlapping blocks keep the plain row-sum form (the
+// regularisation lives there).
+#include <algorithm>
+#include <map>
 #include <unordered_map>
+#include <unordered_set>
 
 #include "hmg_internal.h"
 
@@ -19,6 +30,22 @@ constexpr int kSlice = kTileRows;          // rows per low-rank work item (one w
 constexpr int kNearSlice = 2 * kTileRows;  // rows per near-field work item (two per lane)
 constexpr int kNearChunkCols = 4096;       // near-field columns per work item
 constexpr int kVChunkRows = 1024;          // points per V^T x work item
+constexpr int kPad = 16;                   // factor leading dims: multiples of 128 bytes
+
+inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+inline uint64_t range_key(int l, int u) { return ((uint64_t)(uint32_t)l << 32) | (uint32_t)u; }
+
+struct PairKey {
+    uint64_t a, b;
+    bool operator==(const PairKey& o) const { return a == o.a && b == o.b; }
+};
+struct PairHash {
+    size_t operator()(const PairKey& k) const
+    {
+        return (size_t)(k.a * 0x9E3779B97F4A7C15ull ^ (k.b + 0x632BE59BD9B4E019ull + (k.a << 6)));
+    }
+};
 
 // Group blocks [b0, b1) by row cluster (l1, u1): groups in order of first
 // appearance, blocks in block order inside a group; idx holds b - base.
@@ -30,8 +57,7 @@ void group_by_row(const std::vector<int4>& bl, int b0, int b1, int base, std::ve
     std::vector<int> g_of((size_t)(b1 - b0));
     std::vector<int> cnt;
     for (int b = b0; b < b1; b++) {
-        const uint64_t key = by_column ? (((uint64_t)(uint32_t)bl[b].z << 32) | (uint32_t)bl[b].w)
-                                       : (((uint64_t)(uint32_t)bl[b].x << 32) | (uint32_t)bl[b].y);
+        const uint64_t key = by_column ? range_key(bl[b].z, bl[b].w) : range_key(bl[b].x, bl[b].y);
         auto it = gid.find(key);
         int g;
         if (it == gid.end()) {
@@ -60,6 +86,64 @@ hmg_status upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s)
     return HMG_OK;
 }
 
+// cut a run of column counts into chunks of at most kNearChunkCols columns
+std::vector<int> column_chunks(const std::vector<int>& m2s)
+{
+    std::vector<int> cut{0};
+    int cols = 0;
+    for (int q = 0; q < (int)m2s.size(); q++) {
+        if (cols > 0 && cols + m2s[q] > kNearChunkCols) {
+            cut.push_back(q);
+            cols = 0;
+        }
+        cols += m2s[q];
+    }
+    cut.push_back((int)m2s.size());
+    return cut;
+}
+
+// Per-row partial lists of one schedule.  Near items own P slots
+// near0 + q*128 + r, low-rank items q*64 + r: one list entry each.  Symmetric
+// column slots come per column cluster as (l2, m2, base, count) (contribution k
+// of point l2 + j at base + k*m2 + j): one list entry -(g+1) per group, which
+// the combine expands, so neighbouring points read neighbouring slots.
+hmg_status build_combine(Combine& cb, int n1, const std::vector<int4>& near_rows, int64_t near0,
+                         const std::vector<LrItem>& lr, const std::vector<ColSlots>& cols,
+                         hipStream_t s)
+{
+    std::vector<int> ptr((size_t)n1 + 1, 0);
+    for (const int4& it : near_rows)
+        for (int r = 0; r < it.y; r++) ptr[it.x + r + 1]++;
+    for (const LrItem& it : lr)
+        for (int r = 0; r < it.nrows; r++) ptr[it.row0 + r + 1]++;
+    for (const ColSlots& g : cols)
+        for (int j = 0; j < g.m2; j++) ptr[g.l2 + j + 1]++;
+    for (int i = 0; i < n1; i++) ptr[i + 1] += ptr[i];
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    std::vector<int> idx((size_t)ptr[n1]);
+    for (size_t q = 0; q < near_rows.size(); q++) {
+        const int4& it = near_rows[q];
+        for (int r = 0; r < it.y; r++) idx[fill[it.x + r]++] = (int)(near0 + (int64_t)q * kNearSlice + r);
+    }
+    for (size_t q = 0; q < lr.size(); q++) {
+        const LrItem& it = lr[q];
+        for (int r = 0; r < it.nrows; r++) idx[fill[it.row0 + r]++] = (int)(q * kSlice + r);
+    }
+    for (size_t g = 0; g < cols.size(); g++)
+        for (int j = 0; j < cols[g].m2; j++) idx[fill[cols[g].l2 + j]++] = -(int)g - 1;
+    HMG_TRY(upload(cb.ptr, ptr, s));
+    HMG_TRY(upload(cb.idx, idx, s));
+    HMG_TRY(upload(cb.groups, cols, s));
+    cb.total = ptr[n1];
+    cb.slots = 0;
+    cb.group_entries = 0;
+    for (const ColSlots& g : cols) {
+        cb.slots += (int64_t)g.m2 * g.count;
+        cb.group_entries += g.m2;
+    }
+    return HMG_OK;
+}
+
 }  // namespace
 
 hmg_status build_groups(Engine& eng)
@@ -67,33 +151,114 @@ hmg_status build_groups(Engine& eng)
     group_by_row(eng.h_blk, 0, eng.ndense, 0, eng.h_dgrp_ptr, eng.h_dgrp_blk);
     group_by_row(eng.h_blk, eng.ndense, eng.nblocks, eng.ndense, eng.h_agrp_ptr, eng.h_agrp_blk);
     group_by_row(eng.h_blk, eng.ndense, eng.nblocks, eng.ndense, eng.h_cgrp_ptr, eng.h_cgrp_blk, true);
-    // near-field items: (dense row cluster, 64-row slice, chunk of its blocks)
-    std::vector<int4> items;
     const int ng = (int)eng.h_dgrp_ptr.size() - 1;
+
+    // stored near field: (dense row cluster, 128-row slice, chunk of its blocks)
+    std::vector<int4>& items = eng.h_near_items;
+    items.clear();
     for (int g = 0; g < ng; g++) {
         const int gb0 = eng.h_dgrp_ptr[g], gb1 = eng.h_dgrp_ptr[g + 1];
         const int4 first = eng.h_blk[eng.h_dgrp_blk[gb0]];
         const int l1 = first.x, m1 = first.y - first.x + 1;
-        // chunk boundaries over the group's blocks
-        std::vector<int> cut{gb0};
-        int cols = 0;
+        std::vector<int> m2s;
         for (int q = gb0; q < gb1; q++) {
             const int4 b = eng.h_blk[eng.h_dgrp_blk[q]];
-            const int m2 = b.w - b.z + 1;
-            if (cols > 0 && cols + m2 > kNearChunkCols) {
-                cut.push_back(q);
-                cols = 0;
-            }
-            cols += m2;
+            m2s.push_back(b.w - b.z + 1);
         }
-        cut.push_back(gb1);
+        const std::vector<int> cut = column_chunks(m2s);
         for (int r0 = 0; r0 < m1; r0 += kNearSlice)
             for (size_t c = 0; c + 1 < cut.size(); c++)
-                items.push_back(make_int4(l1 + r0, std::min(kNearSlice, m1 - r0), cut[c], cut[c + 1]));
+                items.push_back(make_int4(l1 + r0, std::min(kNearSlice, m1 - r0), gb0 + cut[c], gb0 + cut[c + 1]));
     }
     eng.n_near_items = (int)items.size();
     HMG_TRY(upload(eng.near_items, items, eng.stream));
     HMG_TRY(upload(eng.near_blk, eng.h_dgrp_blk, eng.stream));
+
+    // matrix-free near field: transposed pairs when both sides are the same points
+    std::unordered_set<PairKey, PairHash> dense_set;
+    eng.near_sym = eng.same_sets && eng.near_sym_enabled;
+    if (eng.near_sym) {
+        dense_set.reserve((size_t)eng.ndense * 2 + 16);
+        for (int b = 0; b < eng.ndense; b++) {
+            const int4 bl = eng.h_blk[b];
+            dense_set.insert(PairKey{range_key(bl.x, bl.y), range_key(bl.z, bl.w)});
+        }
+    }
+    eng.h_fly_items.clear();
+    eng.h_fly_refs.clear();
+    eng.near_evals = 0;
+    // kind: 0 = row sums only, 1 = symmetric (row + column sums), 2 = covered by its transpose
+    auto kind_of = [&](const int4& b) {
+        if (!eng.near_sym) return 0;
+        const bool disjoint = b.y < b.z || b.w < b.x;
+        if (!disjoint) return 0;
+        if (!dense_set.count(PairKey{range_key(b.z, b.w), range_key(b.x, b.y)})) return 0;
+        return b.x < b.z ? 1 : 2;
+    };
+    std::vector<std::pair<int, int>> sym_refs;  // (fly ref index, dense block) awaiting a column slot
+    for (int g = 0; g < ng; g++) {
+        const int gb0 = eng.h_dgrp_ptr[g], gb1 = eng.h_dgrp_ptr[g + 1];
+        const int4 first = eng.h_blk[eng.h_dgrp_blk[gb0]];
+        const int l1 = first.x, m1 = first.y - first.x + 1;
+        std::vector<int> plain, sym;
+        for (int q = gb0; q < gb1; q++) {
+            const int b = eng.h_dgrp_blk[q];
+            const int k = kind_of(eng.h_blk[b]);
+            if (k == 0) plain.push_back(b);
+            else if (k == 1) sym.push_back(b);
+        }
+        std::vector<int> order(plain);
+        order.insert(order.end(), sym.begin(), sym.end());
+        if (order.empty()) continue;
+        std::vector<int> m2s;
+        for (int b : order) m2s.push_back(eng.h_blk[b].w - eng.h_blk[b].z + 1);
+        const std::vector<int> cut = column_chunks(m2s);
+        const int nplain = (int)plain.size();
+        for (int r0 = 0; r0 < m1; r0 += kNearSlice) {
+            for (size_t c = 0; c + 1 < cut.size(); c++) {
+                NearItem it;
+                it.row0 = l1 + r0;
+                it.nrows = std::min(kNearSlice, m1 - r0);
+                it.ref0 = (int)eng.h_fly_refs.size();
+                it.ref_sym = -1;
+                it.ncols_sym = 0;
+                for (int q = cut[c]; q < cut[c + 1]; q++) {
+                    if (q >= nplain && it.ref_sym < 0) it.ref_sym = (int)eng.h_fly_refs.size();
+                    const int4 b = eng.h_blk[order[q]];
+                    if (q >= nplain) {
+                        it.ncols_sym += b.w - b.z + 1;
+                        sym_refs.emplace_back((int)eng.h_fly_refs.size(), order[q]);
+                    }
+                    eng.h_fly_refs.push_back(NearRef{b.z, b.w - b.z + 1, -1});
+                }
+                it.ref1 = (int)eng.h_fly_refs.size();
+                if (it.ref_sym < 0) it.ref_sym = it.ref1;
+                int cols = 0;
+                for (int q = cut[c]; q < cut[c + 1]; q++) cols += m2s[q];
+                eng.near_evals += (int64_t)it.nrows * cols;
+                eng.h_fly_items.push_back(it);
+            }
+        }
+    }
+    eng.n_fly_items = (int)eng.h_fly_items.size();
+    // column slots grouped by column cluster; contributions in item order.  Slot
+    // bases are relative to the column region of P (resolved by aca_layout).
+    std::map<uint64_t, std::vector<int>> by_col;  // column cluster -> refs in item order
+    for (const auto& pr : sym_refs) {
+        const int4 b = eng.h_blk[pr.second];
+        by_col[range_key(b.z, b.w)].push_back(pr.first);
+    }
+    int64_t run = 0;
+    eng.h_col_groups.clear();
+    for (const auto& kv : by_col) {
+        const int l2 = (int)(kv.first >> 32), u2 = (int)(uint32_t)kv.first;
+        const int m2 = u2 - l2 + 1;
+        for (size_t k = 0; k < kv.second.size(); k++)
+            eng.h_fly_refs[kv.second[k]].cbase = run + (int64_t)k * m2;
+        eng.h_col_groups.push_back(ColSlots{l2, m2, run, (int)kv.second.size()});
+        run += (int64_t)m2 * (int64_t)kv.second.size();
+    }
+    eng.n_col_slots = run;
     HMG_HIP(hipStreamSynchronize(eng.stream));
     return HMG_OK;
 }
@@ -116,6 +281,7 @@ hmg_status aca_layout(Engine& eng, int k)
     // U and T in row-cluster (U) order
     eng.h_uoff.assign(na + 1, 0);
     eng.h_toff.assign(na + 1, 0);
+    eng.h_ldu.assign(std::max(na, 1), 0);
     std::vector<LrItem> items;
     int64_t urun = 0, trun = 0;
     const int ng = (int)eng.h_agrp_ptr.size() - 1;
@@ -123,27 +289,30 @@ hmg_status aca_layout(Engine& eng, int k)
         const int gb0 = eng.h_agrp_ptr[g], gb1 = eng.h_agrp_ptr[g + 1];
         const int4 first = eng.h_blk[eng.ndense + eng.h_agrp_blk[gb0]];
         const int l1 = first.x, m1 = first.y - first.x + 1;
+        const int ldu = (int)round_up(m1, kPad);
         const int64_t uC = urun, tC = trun;
         int K = 0;
         for (int q = gb0; q < gb1; q++) {
             const int a = eng.h_agrp_blk[q];
-            eng.h_uoff[a] = urun;
+            eng.h_uoff[a] = uC + (int64_t)K * ldu;
             eng.h_toff[a] = trun;
-            urun += (int64_t)m1 * eng.h_kb[a];
+            eng.h_ldu[a] = ldu;
             trun += eng.h_kb[a];
             K += eng.h_kb[a];
         }
+        urun += (int64_t)ldu * K;
         if (K == 0) continue;
         for (int r0 = 0; r0 < m1; r0 += kSlice)
-            items.push_back(LrItem{l1 + r0, std::min(kSlice, m1 - r0), m1, K, uC + r0, tC});
+            items.push_back(LrItem{l1 + r0, std::min(kSlice, m1 - r0), ldu, K, uC + r0, tC});
     }
     eng.h_uoff[na] = urun;
     eng.h_toff[na] = trun;
     eng.tsize = trun;
     eng.n_lr_items = (int)items.size();
 
-    // V in column-cluster order, each cluster's V row-major (m2 x K); V^T x items
-    // cover 64 columns x up to kVChunkRows points, slices reduce their row chunks
+    // V in column-cluster order, each cluster's V row-major (m2 x K, ld = K padded);
+    // V^T x items cover 64 columns x up to kVChunkRows points, slices reduce their
+    // row chunks
     eng.h_voff.assign(na + 1, 0);
     eng.h_ldv.assign(std::max(na, 1), 0);
     std::vector<LvItem> vitems;
@@ -158,23 +327,24 @@ hmg_status aca_layout(Engine& eng, int k)
         const int l2 = first.z, m2 = first.w - first.z + 1;
         int K = 0;
         for (int q = gb0; q < gb1; q++) K += eng.h_kb[eng.h_cgrp_blk[q]];
+        const int ld = (int)round_up(K, kPad);
         const int64_t vD = vrun;
         int pos = 0;
         for (int q = gb0; q < gb1; q++) {
             const int a = eng.h_cgrp_blk[q];
             eng.h_voff[a] = vD + pos;
-            eng.h_ldv[a] = K;
+            eng.h_ldv[a] = ld;
             for (int l = 0; l < eng.h_kb[a]; l++) tmap[col + pos + l] = (int)(eng.h_toff[a] + l);
             pos += eng.h_kb[a];
         }
-        vrun += (int64_t)m2 * K;
+        vrun += (int64_t)m2 * ld;
         for (int s0 = 0; s0 < K; s0 += kSlice) {
             const int nc = std::min(kSlice, K - s0);
             const int item0 = (int)vitems.size();
             int nch = 0;
             for (int j0 = 0; j0 < m2; j0 += kVChunkRows, nch++)
-                vitems.push_back(LvItem{l2 + j0, std::min(kVChunkRows, m2 - j0), K, nc,
-                                        vD + s0 + (int64_t)j0 * K});
+                vitems.push_back(LvItem{l2 + j0, std::min(kVChunkRows, m2 - j0), ld, nc,
+                                        vD + s0 + (int64_t)j0 * ld});
             slices.push_back(LvSlice{item0, nch, nc, (int)(col + s0)});
         }
         col += K;
@@ -183,32 +353,28 @@ hmg_status aca_layout(Engine& eng, int k)
     eng.n_lv_items = (int)vitems.size();
     eng.n_lv_slices = (int)slices.size();
 
-    // per-row partial lists: near item q owns P slots q*128 .. q*128+127, low-rank
-    // item q owns n_near*128 + q*64 .. +63
-    const int n1 = eng.n[0];
-    std::vector<int> ptr((size_t)n1 + 1, 0);
-    std::vector<int4> near_items((size_t)eng.n_near_items);
-    if (eng.n_near_items > 0)
-        HMG_HIP(hipMemcpy(near_items.data(), eng.near_items.p, sizeof(int4) * eng.n_near_items,
-                          hipMemcpyDeviceToHost));
-    for (const int4& it : near_items)
-        for (int r = 0; r < it.y; r++) ptr[it.x + r + 1]++;
-    for (const LrItem& it : items)
-        for (int r = 0; r < it.nrows; r++) ptr[it.row0 + r + 1]++;
-    for (int i = 0; i < n1; i++) ptr[i + 1] += ptr[i];
-    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-    std::vector<int> idx((size_t)ptr[n1]);
-    for (int q = 0; q < eng.n_near_items; q++) {
-        const int4& it = near_items[q];
-        for (int r = 0; r < it.y; r++) idx[fill[it.x + r]++] = q * kNearSlice + r;
+    // P = [low-rank items x 64][near items x 128][symmetric column slots]
+    const int64_t near0 = (int64_t)eng.n_lr_items * kSlice;
+    const int64_t col0 = near0 + (int64_t)std::max(eng.n_near_items, eng.n_fly_items) * kNearSlice;
+    if (col0 + eng.n_col_slots > (int64_t)INT32_MAX) {
+        set_error("MVP partial buffer exceeds 2^31 slots");
+        return HMG_ERR_INVALID;
     }
-    for (int q = 0; q < eng.n_lr_items; q++) {
-        const LrItem& it = items[q];
-        for (int r = 0; r < it.nrows; r++)
-            idx[fill[it.row0 + r]++] = eng.n_near_items * kNearSlice + q * kSlice + r;
-    }
-
     hipStream_t s = eng.stream;
+    const int n1 = eng.n[0];
+    HMG_TRY(build_combine(eng.comb_stored, n1, eng.h_near_items, near0, items, {}, s));
+    std::vector<int4> fly_rows((size_t)eng.n_fly_items);
+    for (int q = 0; q < eng.n_fly_items; q++)
+        fly_rows[q] = make_int4(eng.h_fly_items[q].row0, eng.h_fly_items[q].nrows, 0, 0);
+    std::vector<ColSlots> cols(eng.h_col_groups);
+    for (ColSlots& c : cols) c.base += col0;
+    HMG_TRY(build_combine(eng.comb_fly, n1, fly_rows, near0, items, cols, s));
+    std::vector<NearRef> refs(eng.h_fly_refs);
+    for (NearRef& r : refs)
+        if (r.cbase >= 0) r.cbase += col0;
+    HMG_TRY(upload(eng.fly_items, eng.h_fly_items, s));
+    HMG_TRY(upload(eng.fly_refs, refs, s));
+
     HMG_TRY(eng.kb.alloc(std::max(na, 1)));
     HMG_TRY(eng.keff.alloc(std::max(na, 1)));
     if (na > 0) {
@@ -220,13 +386,12 @@ hmg_status aca_layout(Engine& eng, int k)
     HMG_TRY(upload(eng.toff, eng.h_toff, s));
     HMG_TRY(upload(eng.lr_items, items, s));
     HMG_TRY(upload(eng.ldv, eng.h_ldv, s));
+    HMG_TRY(upload(eng.ldu, eng.h_ldu, s));
     HMG_TRY(upload(eng.lv_items, vitems, s));
     HMG_TRY(upload(eng.lv_slices, slices, s));
     HMG_TRY(upload(eng.tmap, tmap, s));
     HMG_TRY(eng.Tp.alloc((size_t)std::max(eng.n_lv_items, 1) * kSlice));
-    HMG_TRY(upload(eng.row_ptr, ptr, s));
-    HMG_TRY(upload(eng.row_idx, idx, s));
-    HMG_TRY(eng.P.alloc((size_t)eng.n_near_items * kNearSlice + (size_t)eng.n_lr_items * kSlice));
+    HMG_TRY(eng.P.alloc((size_t)(col0 + eng.n_col_slots)));
     HMG_TRY(eng.T.alloc(std::max<int64_t>(eng.tsize, 1)));
     HMG_HIP(hipStreamSynchronize(s));
     eng.aca_k = k;

@@ -187,10 +187,17 @@ class HMatrix:
         return out
 
     def schedule(self):
-        out = (ctypes.c_int64 * 7)()
+        out = (ctypes.c_int64 * 11)()
         _abi.check(self.L.hmg_get_schedule(ctypes.byref(self.data), out), "hmg_get_schedule")
         return {"near_items": out[0], "lr_items": out[1], "lr_rows": out[2], "lr_t_reads": out[3],
-                "lr_m1": out[4], "lr_m2": out[5], "partials": out[6]}
+                "lr_m1": out[4], "lr_m2": out[5], "partials": out[6], "near_symmetric": bool(out[7]),
+                "col_slots": out[8], "same_sets": bool(out[9]), "near_evals": out[10]}
+
+    def set_option(self, option, value):
+        """hmg_set_option: option is one of _abi.OPT_* (near-field symmetry takes
+        effect at the next setup())."""
+        _abi.check(self.L.hmg_set_option(ctypes.byref(self.data), int(option), int(value)),
+                   "hmg_set_option")
 
     def set_profiling(self, on=True):
         _abi.check(self.L.hmg_set_profiling(ctypes.byref(self.data), 1 if on else 0), "hmg_set_profiling")

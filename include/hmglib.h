@@ -141,8 +141,22 @@ hmg_status hmg_get_work(const struct h_matrix_data* data, int64_t out[9]);
 /* Shape of the H-MVP schedule: out[0] = near-field work items, out[1] = low-rank
  * work items, out[2] = rows over low-rank items, out[3] = t values read by the
  * low-rank items (sum of K over items), out[4] = sum_aca m1, out[5] = sum_aca m2,
- * out[6] = partial sums combined per product (entries of the per-row lists). */
-hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[7]);
+ * out[6] = partial sums combined per product (entries of the per-row lists),
+ * out[7] = 1 if the near field evaluates transposed DENSE pairs once,
+ * out[8] = column-sum slots of that symmetric form, out[9] = 1 if both point
+ * sets are the same points (sorted coordinates bitwise equal). */
+hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[10]);
+/* Engine options (no counterpart in the reference; defaults in brackets):
+ * HMG_OPT_NEAR_SYMMETRIC [1]: when both sets are the same points, evaluate each
+ *   transposed pair of DENSE blocks once (row and column sums); takes effect at
+ *   the next setup_h_matrix.
+ * HMG_OPT_NEAR_LDS_BYTES [65536]: LDS reserved per near-field workgroup, which
+ *   caps its occupancy so the low-rank streams keep wave slots beside it.
+ * HMG_OPT_SERIALIZE [0]: run the near field on the main stream (no overlap). */
+#define HMG_OPT_NEAR_SYMMETRIC 1
+#define HMG_OPT_NEAR_LDS_BYTES 2
+#define HMG_OPT_SERIALIZE 3
+hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value);
 /* Per-kernel timing with HIP events recorded on the engine's streams around each
  * launch.  enable != 0 starts (and clears) recording.  Kernel classes: 0 = x
  * permutation, 1 = low-rank V^T x, 2 = near field, 3 = ACA, 4 = near-field fill,

@@ -217,3 +217,51 @@ def test_block_list_with_singleton_clusters_bit_exact(gpu):
     rng_pts = O.uniform(91, 2 * 3000).reshape(2, 3000) ** 3
     hm, _, om, _ = _pair(gpu, rng_pts, rng_pts, 32, 1.2, 2, 8, O.K_GAUSSIAN, 0.1, precompute=False)
     _assert_blocks_equal(hm.blocks(O.WORK_ITEM_DTYPE), om.blocks(), 2)
+
+
+def _setup_sym(gpu, X, c_leaf, k, kind, ls, sym, eta=1.0, bits=20, reg=0.0):
+    hm = gpu.HMatrix(X, X, bits, eta, 50, c_leaf, k)
+    hm.set_option(gpu.OPT_NEAR_SYMMETRIC, 1 if sym else 0)
+    hm.setup()
+    hm.set_kernel(gpu.Kernel(kind, reg, ls))
+    hm.precompute_aca()
+    return hm
+
+
+@pytest.mark.parametrize("dim,n,c_leaf,kind,ls,reg", [
+    (3, 12000, 128, O.K_EXPONENTIAL, 0.1, 0.0),   # clusters wider than a 64-column tile
+    (2, 3000, 8, O.K_GAUSSIAN, 0.1, 0.5),         # many small clusters per tile, regularised
+    (2, 4096, 64, O.K_MATERN_REF, 1.0, 0.0),
+])
+def test_symmetric_near_field(gpu, dim, n, c_leaf, kind, ls, reg):
+    """Same points on both sides: the near field evaluates each transposed pair of
+    DENSE blocks once.  Entries are bitwise those of the plain form, so the two
+    products differ only by summation order; both match the oracle."""
+    X = _points(81 + dim, dim, n)
+    bits = 20 if dim == 3 else 32
+    x = O.normal(83, n)
+    hs = _setup_sym(gpu, X, c_leaf, 12, kind, ls, True, bits=bits, reg=reg)
+    hp = _setup_sym(gpu, X, c_leaf, 12, kind, ls, False, bits=bits, reg=reg)
+    ss, sp = hs.schedule(), hp.schedule()
+    assert ss["same_sets"] and ss["near_symmetric"] and ss["col_slots"] > 0
+    assert sp["same_sets"] and not sp["near_symmetric"] and sp["col_slots"] == 0
+    ys, yp = hs.apply(x), hp.apply(x)
+    assert _rel(ys, yp) < 1e-13
+    np.testing.assert_array_equal(ys, hs.apply(x))  # deterministic
+    om = O.HMatrix(X, X, bits, 1.0, 50, c_leaf, 0, 0)
+    ok = O.kernel(kind, dim, reg, ls)
+    om.aca(ok, 12)
+    assert _rel(ys, om.hmvp(ok, x)) < TOL_FP64
+
+
+def test_same_sets_detection(gpu):
+    X1 = _points(91, 2, 2048)
+    X2 = X1.copy()
+    X2[1, 7] = np.nextafter(X2[1, 7], 2.0)   # one ulp apart: not the same points
+    h = gpu.HMatrix(X1, X2, 32, 1.5, 50, 64, 8)
+    h.setup()
+    s = h.schedule()
+    assert not s["same_sets"] and not s["near_symmetric"] and s["col_slots"] == 0
+    h2 = gpu.HMatrix(X1, X1.copy(), 32, 1.5, 50, 64, 8)
+    h2.setup()
+    assert h2.schedule()["same_sets"]

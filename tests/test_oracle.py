@@ -199,6 +199,33 @@ def test_tree_invariants_tiling_and_admissibility():
     assert np.all(wt[:hm.ndense] == O.WT_DENSE) and np.all(wt[hm.ndense:] == O.WT_ACA)
 
 
+@pytest.mark.parametrize("dim,bits,n,c_leaf,eta", [(2, 32, 4096, 64, 1.5), (3, 20, 6000, 128, 1.0),
+                                                   (2, 32, 3000, 8, 1.5)])
+def test_same_sets_block_list_is_transpose_closed(dim, bits, n, c_leaf, eta):
+    """With the same points on both sides the traversal is symmetric: (C,D) is a
+    leaf of a work type iff (D,C) is, and off-diagonal DENSE leaves have disjoint
+    clusters.  The engine's symmetric near field (each transposed DENSE pair
+    evaluated once) relies on exactly this; it also checks the kernel matrix is
+    bitwise symmetric on such a pair."""
+    X = O.uniform(7 + dim, dim * n).reshape(dim, n)
+    hm = O.HMatrix(X, X, bits, eta, 50, c_leaf)
+    b = hm.blocks()
+    key = {(int(w["set1_l"]), int(w["set1_u"]), int(w["set2_l"]), int(w["set2_u"])): int(w["work_type"])
+           for w in b}
+    offdiag = 0
+    for (l1, u1, l2, u2), wt in key.items():
+        assert key.get((l2, u2, l1, u1)) == wt
+        if (l1, u1) != (l2, u2):
+            assert u1 < l2 or u2 < l1
+            offdiag += wt == O.WT_DENSE
+    assert offdiag > 0
+    ok = O.kernel(O.K_EXPONENTIAL, dim, 0.0, 0.1)
+    for (l1, u1, l2, u2), wt in list(key.items())[:50]:
+        if wt == O.WT_DENSE and l1 != l2:
+            for i, j in ((l1, l2), (u1, u2), (l1, u2)):
+                assert hm.kernel_entry(ok, i, j) == hm.kernel_entry(ok, j, i)
+
+
 def test_single_block_when_sets_are_small():
     X = O.uniform(5, 2 * 40).reshape(2, 40)
     hm = O.HMatrix(X, X, 32, 1.5, 50, 64)
