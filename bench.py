@@ -1,13 +1,19 @@
 #!/usr/bin/env python3
 """bench.py -- H-MVP throughput of the MI355X H-matrix engine.
 
-Workload (default): N = 2^20 points i.i.d. uniform in [0,1]^3 (seed 2001), source
-= target, exponential kernel exp(-r/0.1), leaf 256, eta 1.0, ACA rank 24 (the
-reference's fixed-rank ACA), plain FP64, one right-hand side N(0,1) (seed 2002).
-A step is one H-MVP y = H x on device-resident x, y (x permuted, near field
-evaluated on the fly, low-rank factors streamed from HBM).  Build (Morton + tree +
-ACA) is timed separately.  The relative error against exact FP64 rows is measured
-on 2048 sampled target rows (seed 2003).
+Default workload (mixed vs plain): N = 2^20 points i.i.d. uniform in [0,1]^3
+(seed 2001), source = target, exponential kernel exp(-r/0.1), leaf 256, eta 1.0,
+ACA rank 24 (the reference's fixed-rank ACA).
+  value:  mixed precision with 32 right-hand sides N(0,1) (seed 3001): FP32 ACA
+          factors, FP32 MFMA low-rank products, near field evaluated in FP32 and
+          multiplied as BF16 hi+lo split MFMA GEMMs.  A step is one call of the
+          product on 32 device-resident right-hand sides; value = RHS x steps /
+          time.  The relative error against exact FP64 rows is measured per RHS on
+          2048 sampled target rows (seed 2003).
+  plain:  the same H-matrix in FP64 with one right-hand side N(0,1) (seed 2002),
+          timed the same way, reported beside it (config.plain_fp64).
+Build (Morton + tree + ACA) is timed separately.  `--workload c2` makes the plain
+FP64 single-RHS product the value instead.
 
 Prints one JSON line (see the bench contract in the repository docs).
 """
@@ -25,6 +31,7 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def log(*a):
@@ -37,26 +44,36 @@ def sample_rows(seed, n, count):
     return np.argsort(u, kind="stable")[:count].astype(np.int64)
 
 
-def kernel_roofline(hm, prof, steps_prof):
-    """Roofline of the dominant MVP kernel from HIP-event timings."""
+def kernel_roofline(hm, prof, nrhs_pass, mixed, traffic_name):
+    """Roofline of the dominant product kernel from HIP-event timings (events
+    recorded on the stream each kernel runs on)."""
     w = hm.work()
-    n1 = hm.n1
     e_near = int(w["E_near"])
     sch = hm.schedule()
-    # algorithmic bytes per launch
-    #   lr_vtx: V_b once (8 B x sum m2*rank) + x~_J once per block (8 B x sum m2)
-    #           + t written (8 B x sum rank)
-    #   lr_u:   U once (8 B x sum m1*rank) + t once per 64-row item + partials
-    bytes_vtx = 8 * int(w["lr_m2_rank"] + sch["lr_m2"] + w["rank_sum"])
-    bytes_lru = 8 * int(w["lr_m1_rank"] + sch["lr_t_reads"] + sch["lr_rows"])
-    # FP64 flops of the matrix-free near field per evaluated entry: d subtractions
-    # + d FMAs (2 flops each) + sqrt (10) + exp (20) + scale (1); every matrix
-    # entry is also accumulated (FMA, 2 flops).  With the same points on both
-    # sides each transposed pair of DENSE blocks is evaluated once
-    # (near_evals < E_near) and feeds a row and a column sum.
-    c_eval = 3 * hm.dim + 10 + 20 + 1
+    R = nrhs_pass
+    s_f = 4 if mixed else 8          # bytes per factor / vector element
+    # algorithmic bytes per launch (R right-hand sides per launch)
+    #   lr_vtx: V once (s_f x sum m2*rank) + X~_J once per block (s_f x R x sum m2)
+    #           + T written (s_f x R x sum rank)
+    #   lr_u:   U once (s_f x sum m1*rank) + T once per 64-row item (s_f x R x
+    #           sum K over items) + partial row sums written (s_f x R x rows)
+    bytes_vtx = s_f * int(w["lr_m2_rank"] + R * (sch["lr_m2"] + w["rank_sum"]))
+    bytes_lru = s_f * int(w["lr_m1_rank"] + R * (sch["lr_t_reads"] + sch["lr_rows"]))
+    flops_lr_v = 2 * R * int(w["lr_m2_rank"])
+    flops_lr_u = 2 * R * int(w["lr_m1_rank"])
     evals = int(sch["near_evals"]) or e_near
-    flops_near = c_eval * evals + 2 * e_near
+    if mixed:
+        # FP32 evaluation per entry: 3 subtractions + 3 FMAs (2 flops each) + sqrt
+        # + scale + exp2 (1 each); BF16 products: 3 MFMAs (hi*hi, hi*lo, lo*hi) of
+        # 2*R flops per evaluated entry
+        c_eval = 3 * hm.dim + 3
+        flops_near = c_eval * evals + 3 * 2 * R * evals
+    else:
+        # FP64: d subtractions + d FMAs + sqrt (10) + exp (20) + scale (1) per
+        # evaluated entry, and one FMA per matrix entry (symmetric pairs: each
+        # evaluation feeds a row and a column sum)
+        c_eval = 3 * hm.dim + 10 + 20 + 1
+        flops_near = c_eval * evals + 2 * e_near
 
     def avg(name):
         ms, cnt = prof[name]
@@ -64,24 +81,33 @@ def kernel_roofline(hm, prof, steps_prof):
 
     t = {k: avg(k) for k in ("gather", "lr_vtx", "lr_u", "near", "combine")}
     res = {
-        "lr_vtx": {"ms": t["lr_vtx"], "bytes": bytes_vtx, "GBs": bytes_vtx / (t["lr_vtx"] * 1e-3) / 1e9},
-        "lr_u": {"ms": t["lr_u"], "bytes": bytes_lru, "GBs": bytes_lru / (t["lr_u"] * 1e-3) / 1e9},
+        "lr_vtx": {"ms": t["lr_vtx"], "bytes": bytes_vtx, "GBs": bytes_vtx / (t["lr_vtx"] * 1e-3) / 1e9,
+                   "TFLOPs": flops_lr_v / (t["lr_vtx"] * 1e-3) / 1e12},
+        "lr_u": {"ms": t["lr_u"], "bytes": bytes_lru, "GBs": bytes_lru / (t["lr_u"] * 1e-3) / 1e9,
+                 "TFLOPs": flops_lr_u / (t["lr_u"] * 1e-3) / 1e12},
         "near": {"ms": t["near"], "flops": flops_near, "c_eval": c_eval, "evals": evals,
                  "symmetric": bool(sch["near_symmetric"]),
-                 "TFLOPs": flops_near / (t["near"] * 1e-3) / 1e12},
+                 "TFLOPs": flops_near / (t["near"] * 1e-3) / 1e12,
+                 "Geval_s": evals / (t["near"] * 1e-3) / 1e9},
         "gather_ms": t["gather"], "combine_ms": t["combine"],
     }
     dominant = max(("lr_vtx", "lr_u", "near"), key=lambda k: t[k])
-    kname = {"lr_vtx": "lr_v_kernel", "lr_u": "lr_u_kernel", "near": "near_fly_kernel"}[dominant]
-    traffic = measured_traffic(hm, kname)
+    if mixed:
+        kname = {"lr_vtx": "lr_v32_kernel", "lr_u": "lr_u32_kernel", "near": "near_mx_kernel"}[dominant]
+    else:
+        kname = {"lr_vtx": "lr_v_kernel", "lr_u": "lr_u_kernel", "near": "near_fly_kernel"}[dominant]
+    traffic = measured_traffic(hm, kname, traffic_name)
     if dominant == "near":
         ach = res["near"]["TFLOPs"]
-        roof = {"kernel": kname, "bound": "mfma", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
-                "traffic": traffic,
-                "note": f"FP64 near field evaluated on the fly (VALU, not MFMA); {c_eval} FP64 "
-                        f"flops per evaluated entry ({evals} evaluations) + 2 per matrix entry "
-                        f"({e_near}); peak = MI355X FP64 (vector and matrix rates are equal)"}
+        peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
+        note = (f"near field evaluated on the fly: {c_eval} FP32 VALU flops per evaluated entry "
+                f"+ 3 BF16 MFMAs x 2x{R} flops ({evals} evaluations incl. masked); peak = MI355X "
+                f"FP32 vector" if mixed else
+                f"FP64 near field evaluated on the fly (VALU, not MFMA); {c_eval} FP64 flops per "
+                f"evaluated entry ({evals} evaluations) + 2 per matrix entry ({e_near}); peak = "
+                f"MI355X FP64 (vector and matrix rates are equal)")
+        roof = {"kernel": kname, "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                "frac": ach / peak, "traffic": traffic, "note": note}
     else:
         ach = res[dominant]["GBs"]
         roof = {"kernel": kname, "bound": "hbm", "achieved": ach,
@@ -89,11 +115,11 @@ def kernel_roofline(hm, prof, steps_prof):
     return roof, res
 
 
-def measured_traffic(hm, kname):
+def measured_traffic(hm, kname, traffic_name):
     """HBM bytes per launch of `kname` from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, calibrated against a 1 GiB copy), when it was
     measured on this same workload; None otherwise."""
-    path = os.path.join(ROOT, "profiles", "hmvp_fp64_n2p20_hbm_traffic.json")
+    path = os.path.join(ROOT, "profiles", traffic_name)
     if hm.n1 != (1 << 20) or not os.path.exists(path):
         return None
     with open(path) as f:
@@ -137,15 +163,39 @@ def cpu_baseline(X, x, cfg, stride):
 
 
 WORKLOADS = {
-    # the plain method at scale (the headline workload)
+    # mixed precision, 32 right-hand sides, against the plain FP64 product (headline)
+    "c3": dict(dim=3, bits=20, n=1 << 20, kind=4, ls=0.1, c_leaf=256, eta=1.0, k=24, seed_pts=2001,
+               seed_x=2002, seed_X=3001, seed_rows=2003, nrhs=32, mixed=True,
+               desc="N=2^20 U[0,1]^3, exp(-r/0.1), leaf 256, eta 1.0, ACA rank 24; mixed precision "
+                    "(FP32 factors, FP32 MFMA low rank, BF16 hi+lo MFMA near field), 32 RHS"),
+    # the plain method at scale
     "c2": dict(dim=3, bits=20, n=1 << 20, kind=4, ls=0.1, c_leaf=256, eta=1.0, k=24, seed_pts=2001,
-               seed_x=2002, seed_rows=2003, desc="N=2^20 U[0,1]^3, exp(-r/0.1), leaf 256, eta 1.0, "
-                                                 "ACA rank 24, FP64, 1 RHS"),
+               seed_x=2002, seed_X=3001, seed_rows=2003, nrhs=1, mixed=False,
+               desc="N=2^20 U[0,1]^3, exp(-r/0.1), leaf 256, eta 1.0, ACA rank 24, FP64, 1 RHS"),
     # the small plain case (N=4096, 2D)
     "c1": dict(dim=2, bits=32, n=4096, kind=3, ls=0.1, c_leaf=64, eta=1.5, k=32, seed_pts=1001,
-               seed_x=1003, seed_rows=2003, desc="N=4096 U[0,1]^2, exp(-r^2/0.02), leaf 64, eta 1.5, "
-                                                 "ACA rank 32, FP64, 1 RHS"),
+               seed_x=1003, seed_X=1003, seed_rows=2003, nrhs=1, mixed=False,
+               desc="N=4096 U[0,1]^2, exp(-r^2/0.02), leaf 64, eta 1.5, ACA rank 32, FP64, 1 RHS"),
 }
+
+
+def timed_products(hm, xd, yd, nrhs, steps, warmup, hip):
+    for _ in range(warmup):
+        hm.apply_dev(xd, yd, nrhs)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hm.apply_dev(xd, yd, nrhs)
+    hip.synchronize()
+    return time.perf_counter() - t0
+
+
+def rel_errors(hm, Y, Xh, rows, nrhs):
+    """Relative 2-norm error per RHS on the sampled rows against exact FP64 rows."""
+    Yex = hm.dense_rows(rows, Xh)
+    Yex = Yex.reshape(rows.size, nrhs)
+    Ys = Y.reshape(-1, nrhs)[rows]
+    return [float(np.linalg.norm(Ys[:, c] - Yex[:, c]) / np.linalg.norm(Yex[:, c])) for c in range(nrhs)]
 
 
 def main():
@@ -153,9 +203,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=0, help="override the point count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 leg of c3")
+    ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--cpu-stride", type=int, default=16)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (calibrates HBM byte counters)")
@@ -188,9 +240,11 @@ def main():
     if args.n:
         cfg["n"] = args.n
     n = cfg["n"]
+    mixed = cfg["mixed"]
+    R = cfg["nrhs"]
     log(f"[bench] workload {args.workload}: {cfg['desc']} (n={n})")
     X = rng_uniform(cfg["seed_pts"], cfg["dim"] * n).reshape(cfg["dim"], n)
-    x = rng_normal(cfg["seed_x"], n)
+    x1 = rng_normal(cfg["seed_x"], n)
 
     t0 = time.perf_counter()
     hm = HMatrix(X, X, cfg["bits"], cfg["eta"], 50, cfg["c_leaf"], cfg["k"])
@@ -199,71 +253,112 @@ def main():
     t_setup = time.perf_counter() - t0
     kern = Kernel(cfg["kind"], 0.0, cfg["ls"])
     hm.set_kernel(kern)
-    t0 = time.perf_counter()
-    hm.precompute_aca()
-    hip.synchronize()
-    t_aca = time.perf_counter() - t0
-    w = hm.work()
-    log(f"[bench] setup {t_setup:.2f}s aca {t_aca:.2f}s blocks dense={w['dense']} aca={w['aca']} "
-        f"E_near={w['E_near']:.3e} E_lr={w['E_lr']:.3e} levels={w['levels']}")
-
-    xd = hip.DeviceArray(x)
-    yd = hip.DeviceArray.empty(n)
-    for _ in range(args.warmup):
-        hm.apply_dev(xd, yd)
-    hip.synchronize()
-    log("[bench] warmup done")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        hm.apply_dev(xd, yd)
-    hip.synchronize()
-    dt = time.perf_counter() - t0
-    ms = dt / args.steps * 1e3
-    value = args.steps / dt
-    log(f"[bench] {args.steps} H-MVPs: {ms:.3f} ms each, {value:.2f} H-MVP/s")
-
-    # per-kernel timing (HIP events on the engine's stream), separate pass
-    hm.set_profiling(True)
-    nprof = 3
-    for _ in range(nprof):
-        hm.apply_dev(xd, yd)
-    prof = hm.profile()
-    hm.set_profiling(False)
-    roof, kernels = kernel_roofline(hm, prof, nprof)
-    log(f"[bench] kernels {json.dumps(kernels)}")
-
-    # relative error against exact FP64 rows
-    y = yd.download()
     rows = sample_rows(cfg["seed_rows"], n, min(2048, n))
-    y_ex = hm.dense_rows(rows, x)
-    rel_err = float(np.linalg.norm(y[rows] - y_ex) / np.linalg.norm(y_ex))
-    log(f"[bench] rel err vs FP64 dense on {rows.size} rows: {rel_err:.3e}")
+
+    # ---- plain FP64, one right-hand side -----------------------------------------
+    plain = None
+    if not mixed or not args.no_plain:
+        hm.set_precision(hmglib_amd.PREC_FP64)
+        t0 = time.perf_counter()
+        hm.precompute_aca()
+        hip.synchronize()
+        t_aca64 = time.perf_counter() - t0
+        w = hm.work()
+        log(f"[bench] setup {t_setup:.2f}s FP64 aca {t_aca64:.2f}s blocks dense={w['dense']} "
+            f"aca={w['aca']} E_near={w['E_near']:.3e} E_lr={w['E_lr']:.3e} levels={w['levels']}")
+        xd = hip.DeviceArray(x1)
+        yd = hip.DeviceArray.empty(n)
+        steps = args.steps if not mixed else args.plain_steps
+        dt = timed_products(hm, xd, yd, 1, steps, args.warmup if not mixed else 1, hip)
+        err = rel_errors(hm, yd.download(), x1, rows, 1)[0]
+        plain = {"value": steps / dt, "ms_per_step": dt / steps * 1e3, "steps": steps,
+                 "rel_err_fp64_rows": err, "aca_s": t_aca64}
+        log(f"[bench] plain FP64: {plain['ms_per_step']:.3f} ms per H-MVP, {plain['value']:.2f} H-MVP/s, "
+            f"rel err {err:.3e}")
+        if not mixed:
+            hm.set_profiling(True)
+            for _ in range(3):
+                hm.apply_dev(xd, yd)
+            prof = hm.profile()
+            hm.set_profiling(False)
+            roof, kernels = kernel_roofline(hm, prof, 1, False, "hmvp_fp64_n2p20_hbm_traffic.json")
+        xd.free()
+        yd.free()
+
+    # ---- mixed precision, R right-hand sides ---------------------------------------
+    result = None
+    if mixed:
+        hm.set_precision(hmglib_amd.PREC_MIXED)
+        t0 = time.perf_counter()
+        hm.precompute_aca()
+        hip.synchronize()
+        t_aca = time.perf_counter() - t0
+        w = hm.work()
+        log(f"[bench] mixed aca {t_aca:.2f}s E_near={w['E_near']:.3e} E_lr={w['E_lr']:.3e}")
+        Xh = np.asfortranarray(rng_normal(cfg["seed_X"], n * R).reshape(n, R, order="F"))
+        Xd = hip.DeviceArray(Xh)
+        Yd = hip.DeviceArray.empty((n, R))
+        dt = timed_products(hm, Xd, Yd, R, args.steps, args.warmup, hip)
+        ms = dt / args.steps * 1e3
+        value = R * args.steps / dt
+        log(f"[bench] mixed: {args.steps} calls x {R} RHS: {ms:.3f} ms per call, {value:.1f} H-MVP/s")
+        hm.set_profiling(True)
+        for _ in range(3):
+            hm.apply_dev(Xd, Yd, R)
+        prof = hm.profile()
+        hm.set_profiling(False)
+        roof, kernels = kernel_roofline(hm, prof, R, True, "hmvp_mixed_n2p20_hbm_traffic.json")
+        errs = rel_errors(hm, np.asfortranarray(Yd.download()), Xh, rows, R)
+        log(f"[bench] mixed rel err per RHS: max {max(errs):.3e} median {float(np.median(errs)):.3e}")
+        result = {"value": value, "ms": ms, "aca_s": t_aca, "errs": errs}
+        Xd.free()
+        Yd.free()
+    log(f"[bench] kernels {json.dumps(kernels)}")
 
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(X, x, cfg, args.cpu_stride if n > 100000 else 1)
+            cpu = cpu_baseline(X, x1, cfg, args.cpu_stride if n > 100000 else 1)
             log(f"[bench] cpu baseline {cpu}")
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
 
+    config = {"workload": cfg["desc"], "n": n, "nrhs": R, "build_s": {"setup": t_setup},
+              "blocks": {"dense": w["dense"], "aca": w["aca"]}, "E_near": w["E_near"], "E_lr": w["E_lr"]}
+    if mixed:
+        config.update({"precision": "mixed: FP32 factors and low-rank MFMA, BF16x3 near-field MFMA, "
+                                    "FP64 ACA residuals",
+                       "rel_err_fp64_rows_per_rhs_max": max(result["errs"]),
+                       "rel_err_fp64_rows_per_rhs": result["errs"]})
+        config["build_s"]["aca_fp32"] = result["aca_s"]
+        if plain is not None:
+            config["plain_fp64"] = {"H-MVP/s": plain["value"], "ms_per_step": plain["ms_per_step"],
+                                    "steps": plain["steps"], "nrhs": 1,
+                                    "rel_err_fp64_rows": plain["rel_err_fp64_rows"]}
+            config["mixed_over_plain"] = result["value"] / plain["value"]
+            config["build_s"]["aca_fp64"] = plain["aca_s"]
+        value, ms, steps, dtype = result["value"], result["ms"], args.steps, "f32/bf16x3 (mixed)"
+        metric = ("H-MVP/s at N=2^20 3D, 32 RHS, mixed precision (rel.err<=1e-4 vs FP64), "
+                  "exp(-r/0.1), rank-24 ACA")
+    else:
+        config.update({"precision": "fp64", "rel_err_fp64_rows": plain["rel_err_fp64_rows"]})
+        config["build_s"]["aca_fp64"] = plain["aca_s"]
+        value, ms, steps, dtype = plain["value"], plain["ms_per_step"], plain["steps"], "f64"
+        metric = "H-MVP/s at N=2^20 3D (plain FP64, rank-24 ACA, exp(-r/0.1))"
     out = {
-        "metric": "H-MVP/s at N=2^20 3D (plain FP64, rank-24 ACA, exp(-r/0.1))",
+        "metric": metric,
         "value": value,
         "unit": "H-MVP/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype,
         "data": "synthetic (seeded SplitMix64/xoshiro256** points and RHS)",
-        "config": {"workload": cfg["desc"], "n": n, "nrhs": 1, "precision": "fp64",
-                   "build_s": {"setup": t_setup, "aca": t_aca}, "rel_err_fp64_rows": rel_err,
-                   "blocks": {"dense": w["dense"], "aca": w["aca"]}, "E_near": w["E_near"],
-                   "E_lr": w["E_lr"]},
+        "config": config,
         "roofline": roof,
         "cpu_baseline": (None if cpu is None else
                          {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}),

@@ -7,6 +7,7 @@ that drives the ABI like the reference's drivers do.  Importing it fails loudly
 when the library is missing; there is no CPU fallback.
 """
 from ._abi import (HMG_OK, KERNEL_EXPONENTIAL, KERNEL_GAUSSIAN, KERNEL_GAUSSIAN_REF,  # noqa: F401
-                   KERNEL_MATERN_REF, OPT_NEAR_LDS_BYTES, OPT_NEAR_SYMMETRIC, OPT_SERIALIZE,
-                   WT_ACA, WT_DENSE, HMGError, HMatrixData, WorkItem, lib)
+                   KERNEL_MATERN_REF, OPT_NEAR_LDS_BYTES, OPT_NEAR_SYMMETRIC, OPT_PRECISION,
+                   OPT_SERIALIZE, PREC_FP64, PREC_MIXED, WT_ACA, WT_DENSE, HMGError, HMatrixData,
+                   WorkItem, lib)
 from .hmatrix import HMatrix, Kernel, rng_normal, rng_uniform  # noqa: F401

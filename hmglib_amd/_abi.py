@@ -21,6 +21,10 @@ KERNEL_EXPONENTIAL = 4
 OPT_NEAR_SYMMETRIC = 1
 OPT_NEAR_LDS_BYTES = 2
 OPT_SERIALIZE = 3
+OPT_PRECISION = 4
+
+PREC_FP64 = 0
+PREC_MIXED = 1
 
 HMG_OK = 0
 HMG_ERR_INVALID = 1

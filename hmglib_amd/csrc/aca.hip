@@ -31,12 +31,15 @@ __device__ __forceinline__ double wave_sum(double v)
 // Factor layout: U_b[i, l] at U + uoff[a] + l * ldu[a] + i (column-major, ldu >= m1:
 // the row cluster's padded height); V_b is stored transposed, V_b[j, l] at
 // V + voff[a] + j * ldv[a] + l, i.e. row j of V_b is contiguous (ldv >= total rank
-// of the column cluster's blocks).
+// of the column cluster's blocks).  T is the storage type of the factors (double,
+// or float for mixed precision): every residual, norm and pivot division is
+// computed in FP64 from the stored values and rounded to T once when stored.
+template <class T>
 __global__ void __launch_bounds__(kAcaThreads)
 aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, int naca,
            const int* __restrict__ kb, const int64_t* __restrict__ uoff, const int* __restrict__ ldu,
-           const int64_t* __restrict__ voff, const int* __restrict__ ldv, double* __restrict__ U,
-           double* __restrict__ V, int* __restrict__ keff, int tie_last)
+           const int64_t* __restrict__ voff, const int* __restrict__ ldv, T* __restrict__ U,
+           T* __restrict__ V, int* __restrict__ keff, int tie_last)
 {
     const int lane = threadIdx.x % kWave;
     const int a = blockIdx.x * kAcaWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -47,8 +50,8 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
     const int k = kb[a];
     const int64_t ld = ldv[a];
     const int64_t lu = ldu[a];
-    double* __restrict__ Ub = U + uoff[a];
-    double* __restrict__ Vb = V + voff[a];
+    T* __restrict__ Ub = U + uoff[a];
+    T* __restrict__ Vb = V + voff[a];
     int i_r = -1;
     int ke = 0;
     for (int r = 0; r < k; r++) {
@@ -63,9 +66,10 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
             double nrm = 0.0;
             for (int j = lane; j < m2; j += kWave) {
                 double val = kernel_entry(kp, c1, l1 + i_r, c2, l2 + j);
-                const double* __restrict__ vrow = Vb + (int64_t)j * ld;
-                for (int l = 0; l < r; l++) val = __fma_rn(-Ub[l * lu + i_r], vrow[l], val);
-                Vb[(int64_t)j * ld + r] = val;
+                const T* __restrict__ vrow = Vb + (int64_t)j * ld;
+                for (int l = 0; l < r; l++)
+                    val = __fma_rn(-(double)Ub[l * lu + i_r], (double)vrow[l], val);
+                Vb[(int64_t)j * ld + r] = (T)val;
                 nrm = __fma_rn(val, val, nrm);
             }
             nrm = wave_sum(nrm);
@@ -74,14 +78,14 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
         if (stop) {
             // the rank stops here: column r of V (last rejected row) is cleared, column r
             // of U was never written; both stay zero in the product
-            for (int j = lane; j < m2; j += kWave) Vb[(int64_t)j * ld + r] = 0.0;
+            for (int j = lane; j < m2; j += kWave) Vb[(int64_t)j * ld + r] = (T)0;
             break;
         }
         // column pivot (tuple_absolute_maximum, :991-1018)
         double best = -1.0, bval = 0.0;
         int bj = -1;
         for (int j = lane; j < m2; j += kWave) {
-            const double x = Vb[(int64_t)j * ld + r];
+            const double x = (double)Vb[(int64_t)j * ld + r];
             const double ax = fabs(x);
             if (tie_last ? (ax >= best) : (ax > best)) {
                 best = ax;
@@ -103,15 +107,16 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
         }
         const int jr = bj;
         const double p = bval;
-        for (int j = lane; j < m2; j += kWave) Vb[(int64_t)j * ld + r] = Vb[(int64_t)j * ld + r] / p;
+        for (int j = lane; j < m2; j += kWave)
+            Vb[(int64_t)j * ld + r] = (T)((double)Vb[(int64_t)j * ld + r] / p);
         __threadfence_block();
         // residual column (batched_fill_kernel_vector_and_scaled_substraction_for_u_r)
-        const double* __restrict__ vpiv = Vb + (int64_t)jr * ld;
-        double* __restrict__ u = Ub + r * lu;
+        const T* __restrict__ vpiv = Vb + (int64_t)jr * ld;
+        T* __restrict__ u = Ub + r * lu;
         for (int i = lane; i < m1; i += kWave) {
             double val = kernel_entry(kp, c1, l1 + i, c2, l2 + jr);
-            for (int l = 0; l < r; l++) val = __fma_rn(-vpiv[l], Ub[l * lu + i], val);
-            u[i] = val;
+            for (int l = 0; l < r; l++) val = __fma_rn(-(double)vpiv[l], (double)Ub[l * lu + i], val);
+            u[i] = (T)val;
         }
         __threadfence_block();
         ke = r + 1;
@@ -121,8 +126,9 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
 
 }  // namespace
 
+template <class T>
 hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp, int k,
-                     int tie_last, double* U, double* V, int* keff)
+                     int tie_last, T* U, T* V, int* keff)
 {
     (void)k;
     const int na = eng.naca;
@@ -130,11 +136,16 @@ hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const Kern
     hipStream_t s = eng.stream;
     const unsigned grid = (unsigned)((na + kAcaWaves - 1) / kAcaWaves);
     eng.prof.begin(PROF_ACA, s);
-    aca_kernel<<<grid, kAcaThreads, 0, s>>>(c1, c2, kp, eng.blk.p + eng.ndense, na, eng.kb.p,
+    aca_kernel<T><<<grid, kAcaThreads, 0, s>>>(c1, c2, kp, eng.blk.p + eng.ndense, na, eng.kb.p,
                                             eng.uoff.p, eng.ldu.p, eng.voff.p, eng.ldv.p, U, V, keff, tie_last);
     HMG_HIP(hipGetLastError());
     eng.prof.end(PROF_ACA, s);
     return HMG_OK;
 }
+
+template hmg_status aca_build<double>(Engine&, const Coords&, const Coords&, const KernelParams&, int,
+                                      int, double*, double*, int*);
+template hmg_status aca_build<float>(Engine&, const Coords&, const Coords&, const KernelParams&, int,
+                                     int, float*, float*, int*);
 
 }  // namespace hmg

@@ -136,6 +136,8 @@ void release_factors(Engine& e, h_matrix_data* d)
 {
     e.U.release();
     e.V.release();
+    e.U32.release();
+    e.V32.release();
     e.aca_k = -1;
     d->U = nullptr;
     d->V = nullptr;
@@ -198,6 +200,26 @@ void release_build(Engine& e, h_matrix_data* d)
     e.xs.release();
     e.T.release();
     e.n_near_items = e.n_lr_items = 0;
+    e.mx_items.release();
+    e.mx_refs.release();
+    e.h_mx_items.clear();
+    e.h_mx_refs.clear();
+    e.n_mx_items = 0;
+    e.mx_evals = 0;
+    e.mx_sched = false;
+    e.comb_mx.ptr.release();
+    e.comb_mx.idx.release();
+    e.comb_mx.groups.release();
+    e.comb_mx.total = 0;
+    e.Xs32.release();
+    e.T32.release();
+    e.Tp32.release();
+    e.P32.release();
+    e.Xp.release();
+    e.cpan[0].release();
+    e.cpan[1].release();
+    e.cpan_ready = false;
+    e.layout_prec = HMG_PREC_FP64;
     e.built = false;
     if (d->mat_vec_data) {
         free(d->mat_vec_data);
@@ -209,17 +231,19 @@ void release_build(Engine& e, h_matrix_data* d)
     d->aca_work_size = nullptr;
 }
 
-// ACA factors into (U, V, keff) for the current block list and rank k
+// ACA factors into (U, V, keff) for the current block list and rank k; T = double
+// (FP64 layout) or float (mixed-precision layout)
+template <class T>
 hmg_status build_factors(Engine& e, h_matrix_data* d, const KernelParams& kp, int k, int tie_last,
-                         DevBuf<double>& U, DevBuf<double>& V)
+                         DevBuf<T>& U, DevBuf<T>& V)
 {
-    HMG_TRY(aca_layout(e, k));
+    HMG_TRY(aca_layout(e, k, sizeof(T) == sizeof(float) ? HMG_PREC_MIXED : HMG_PREC_FP64));
     const int na = e.naca;
     HMG_TRY(U.alloc(std::max<int64_t>(e.h_uoff[na], 1)));
     HMG_TRY(V.alloc(std::max<int64_t>(e.h_voff[na], 1)));
-    HMG_HIP(hipMemsetAsync(U.p, 0, sizeof(double) * U.n, e.stream));
-    HMG_HIP(hipMemsetAsync(V.p, 0, sizeof(double) * V.n, e.stream));
-    HMG_TRY(aca_build(e, coords_of(d, 0), coords_of(d, 1), kp, k, tie_last, U.p, V.p, e.keff.p));
+    HMG_HIP(hipMemsetAsync(U.p, 0, sizeof(T) * U.n, e.stream));
+    HMG_HIP(hipMemsetAsync(V.p, 0, sizeof(T) * V.n, e.stream));
+    HMG_TRY(aca_build<T>(e, coords_of(d, 0), coords_of(d, 1), kp, k, tie_last, U.p, V.p, e.keff.p));
     HMG_HIP(hipStreamSynchronize(e.stream));
     return HMG_OK;
 }
@@ -440,9 +464,16 @@ hmg_status hmg_precompute_aca(struct h_matrix_data* data)
     // The reference skips the whole far field when the first ACA block exceeds
     // max_batched_aca_size (hmglib.cu:232, linear_algebra.cu:4163-4171); the
     // engine has no batch cap and always builds every ACA block.
-    HMG_TRY(build_factors(*e, data, kp, std::max(data->k, 0), 1, e->U, e->V));
-    data->U = e->U.p;
-    data->V = e->V.p;
+    if (e->precision == HMG_PREC_MIXED) {
+        HMG_TRY(build_factors(*e, data, kp, std::max(data->k, 0), 1, e->U32, e->V32));
+        data->U = (double*)e->U32.p;  // FP32 factors in mixed precision
+        data->V = (double*)e->V32.p;
+    } else {
+        HMG_TRY(build_factors(*e, data, kp, std::max(data->k, 0), 1, e->U, e->V));
+        data->U = e->U.p;
+        data->V = e->V.p;
+    }
+    e->fact_prec = e->precision;
     return HMG_OK;
 }
 
@@ -484,6 +515,23 @@ hmg_status hmg_apply_h_matrix_mvp_multi(const double* X, double* Y, int nrhs, st
     KernelParams kp;
     HMG_TRY(read_kernel(data, &kp));
     const bool pre_aca = data->U != nullptr && e->aca_k >= 0;
+    // stored factors decide the product's precision; otherwise the option does
+    const int prec = pre_aca ? e->fact_prec : e->precision;
+    if (prec == HMG_PREC_MIXED) {
+        if (!pre_aca) {
+            HMG_TRY(build_factors(*e, data, kp, std::max(data->k, 0), 1, e->U32, e->V32));
+            e->fact_prec = HMG_PREC_MIXED;
+        }
+        HMG_TRY(hmvp_mixed(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[0], data->order[1],
+                           X, Y, nrhs));
+        HMG_HIP(hipStreamSynchronize(e->stream));
+        if (!pre_aca) {
+            e->U32.release();
+            e->V32.release();
+            e->aca_k = -1;
+        }
+        return HMG_OK;
+    }
     const bool pre_dense = data->dA != nullptr && e->dense_stored;
     DevBuf<double> tU, tV;
     const double* U = e->U.p;
@@ -536,7 +584,7 @@ hmg_status hmg_apply_h_matrix_mvp_without_batching(double* x, double* y, struct 
     tV.release();
     if (had) {
         // restore the stored factors' layout and ranks
-        HMG_TRY(aca_layout(*e, had_k));
+        HMG_TRY(aca_layout(*e, had_k, e->fact_prec));
         if (e->naca > 0)
             HMG_HIP(hipMemcpy(e->keff.p, keep_keff.data(), sizeof(int) * e->naca, hipMemcpyHostToDevice));
     } else {
@@ -614,7 +662,7 @@ hmg_status hmg_get_work(const struct h_matrix_data* data, int64_t out[9])
     return HMG_OK;
 }
 
-hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[7])
+hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11])
 {
     Engine* e;
     HMG_TRY(check_ready(data, &e));
@@ -632,19 +680,20 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[7])
         m1s += b.y - b.x + 1;
         m2s += b.w - b.z + 1;
     }
-    const bool stored = data->dA != nullptr && e->dense_stored;
-    out[0] = stored ? e->n_near_items : e->n_fly_items;
+    const bool mixed = e->layout_prec == HMG_PREC_MIXED;
+    const bool stored = !mixed && data->dA != nullptr && e->dense_stored;
+    out[0] = mixed ? e->n_mx_items : stored ? e->n_near_items : e->n_fly_items;
     out[1] = e->n_lr_items;
     out[2] = rows;
     out[3] = treads;
     out[4] = m1s;
     out[5] = m2s;
-    const Combine& cb = stored ? e->comb_stored : e->comb_fly;
+    const Combine& cb = mixed ? e->comb_mx : stored ? e->comb_stored : e->comb_fly;
     out[6] = cb.total - cb.group_entries + cb.slots;
-    out[7] = e->near_sym ? 1 : 0;
-    out[8] = e->n_col_slots;
+    out[7] = (!mixed && e->near_sym) ? 1 : 0;
+    out[8] = mixed ? 0 : e->n_col_slots;
     out[9] = e->same_sets ? 1 : 0;
-    out[10] = stored ? 0 : e->near_evals;
+    out[10] = mixed ? e->mx_evals : stored ? 0 : e->near_evals;
     return HMG_OK;
 }
 
@@ -668,6 +717,13 @@ hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value)
         return HMG_OK;
     case HMG_OPT_SERIALIZE:
         e->serialize = value != 0;
+        return HMG_OK;
+    case HMG_OPT_PRECISION:
+        if (value != HMG_PREC_FP64 && value != HMG_PREC_MIXED) {
+            set_error("precision must be HMG_PREC_FP64 or HMG_PREC_MIXED");
+            return HMG_ERR_INVALID;
+        }
+        e->precision = (int)value;
         return HMG_OK;
     default:
         set_error("unknown option " + std::to_string(option));

@@ -159,6 +159,28 @@ struct ColSlots {
     int count;
 };
 
+// Mixed-precision near field (many right-hand sides).  Points are handled in
+// aligned groups of 8 (the 8 columns one lane feeds to a BF16 MFMA).  A ref is
+// a run of groups [g0, g1) (g1 - g0 even: two groups per 16-column step) whose
+// columns outside [lo, hi) are masked; an item is up to 64 rows of one row
+// cluster against the refs [ref0, ref1).
+struct MxRef {
+    int g0, g1;
+    int lo, hi;
+};
+struct MxItem {
+    int row0;
+    int nrows;
+    int ref0;
+    int ref1;
+};
+
+// Number of right-hand sides one mixed-precision pass handles (the N of the
+// MFMA tiles); more right-hand sides run in several passes.
+constexpr int kMxRhs = 32;
+// Columns of one mixed-precision V^T X work item (four 32-column MFMA tiles).
+constexpr int kMxVCols = 128;
+
 // per-point partial lists of one MVP schedule (CSR over sorted rows): an entry
 // v >= 0 is the P slot v, an entry v < 0 the column-slot group -(v+1)
 struct Combine {
@@ -252,6 +274,30 @@ struct Engine {
     int near_lds_pad = 64 * 1024;
     bool serialize = false;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
+    // ---- mixed precision ------------------------------------------------------
+    // precision: requested mode for the next precompute_aca / apply (HMG_PREC_*);
+    // layout_prec: mode the factor layout and low-rank items were built for;
+    // fact_prec: mode of the stored factors (valid when aca_k >= 0).
+    int precision = HMG_PREC_FP64;
+    int layout_prec = HMG_PREC_FP64;
+    int fact_prec = HMG_PREC_FP64;
+    DevBuf<float> U32, V32;                     // FP32 factors (same grouped layout, pad 32)
+    // near field: 64-row items against merged, group-aligned column runs
+    std::vector<MxItem> h_mx_items;
+    std::vector<MxRef> h_mx_refs;
+    DevBuf<MxItem> mx_items;
+    DevBuf<MxRef> mx_refs;
+    int n_mx_items = 0;
+    int64_t mx_evals = 0;                       // kernel entries evaluated per pass (incl. masked)
+    bool mx_sched = false;                      // near items built for the current tree
+    Combine comb_mx;
+    // per pass: X as FP32 rows of kMxRhs values, as BF16 hi/lo panels (per group of
+    // 8 points: [hi|lo][rhs][8]), and coordinate panels per group ([dim][8] FP32)
+    DevBuf<float> Xs32, T32, Tp32, P32;
+    DevBuf<__bf16> Xp;
+    DevBuf<float> cpan[2];
+    bool cpan_ready = false;
 };
 
 Engine* engine_of(const h_matrix_data* d);
@@ -291,16 +337,29 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
                       const uint64_t* codes2, const TreeParams& tp);
 
 // ---- ACA (aca.hip) -------------------------------------------------------------
-// tie_last: 1 = last argmax (batched path), 0 = first argmax (unbatched path)
+// tie_last: 1 = last argmax (batched path), 0 = first argmax (unbatched path).
+// The factors are stored as T (double, or float for mixed precision); residuals,
+// norms and pivot scaling are computed in FP64 either way.
+template <class T>
 hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
-                     int k, int tie_last, double* U, double* V, int* keff);
+                     int k, int tie_last, T* U, T* V, int* keff);
 
 // ---- schedule (schedule.hip, host) ---------------------------------------------------
 // row-cluster groups and near-field work items (once per tree); the matrix-free
 // schedule pairs transposed DENSE blocks when eng.same_sets
 hmg_status build_groups(Engine& eng);
-// ranks k_b = min(k,m1,m2), factor offsets, low-rank work items, per-row partial lists
-hmg_status aca_layout(Engine& eng, int k);
+// ranks k_b = min(k,m1,m2), factor offsets, low-rank work items, per-row partial
+// lists, for factors of precision prec (HMG_PREC_*: leading dims are padded to
+// 128 bytes; mixed precision also builds its near-field items)
+hmg_status aca_layout(Engine& eng, int k, int prec = HMG_PREC_FP64);
+
+// ---- mixed precision (mixed.hip) -------------------------------------------------
+// Y = H X for nrhs right-hand sides (X, Y FP64 column-major in the caller's point
+// order) with FP32 factors, FP32 MFMA low-rank products and the near field
+// evaluated in FP32 and multiplied as BF16 hi+lo split MFMA GEMMs.
+hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
+                      const uint64_t* order1, const uint64_t* order2, const double* X, double* Y,
+                      int nrhs);
 
 // ---- near field / MVP (mvp.hip) ----------------------------------------------------
 hmg_status dense_fill(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp);

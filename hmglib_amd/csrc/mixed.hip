@@ -1,0 +1,468 @@
+// mixed.hip -- mixed-precision H-matrix-vector products for many right-hand sides.
+//
+// Same decomposition as the FP64 product (reference: linear_algebra.cu:4292-4379,
+// 1429-1591, 2817-3137), with up to kMxRhs = 32 right-hand sides per pass:
+//   prep      X (FP64, caller order) -> sorted FP32 rows Xs[p][32] for the
+//             low-rank product and BF16 hi/lo panels Xp for the near field
+//   lr_v32    T_D = V_D^T X_D per (column cluster, 128 columns, <= 1024 points):
+//             v_mfma_f32_32x32x2f32 tiles (M = columns, N = right-hand sides,
+//             K = points), partial sums per point chunk; lr_v32_finalize adds
+//             the chunks in order and scatters T into the U order
+//   lr_u32    Y_C = U_C T_C per 64 rows of a row cluster: two 32x32 FP32 MFMA
+//             tiles over the cluster's K columns
+//   near_mx   the near field evaluated on the fly in FP32: each lane evaluates
+//             the 8 kernel entries (its row, 8 consecutive columns) that it feeds
+//             to v_mfma_f32_32x32x16_bf16 as A operand, split into BF16 hi + lo;
+//             Y += A_hi X_hi + A_hi X_lo + A_lo X_hi, accumulated in FP32.  It
+//             runs on the second stream beside the low-rank kernels.
+//   combine   every row adds its partials in a fixed order (deterministic)
+// MFMA lane layouts (gfx950, wave64), for lane l, h = l / 32, c = l % 32:
+//   32x32x2 f32:  A[c][h], B[h][c], D[(r/4)*8 + 4h + r%4][c] for r < 16
+//   32x32x16 bf16: A[c][8h + e], B[8h + e][c] for e < 8, D as above
+#include "hmg_internal.h"
+
+namespace hmg {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+constexpr int kMxRowsItem = 64;  // rows per near-field and low-rank item (two 32-row tiles)
+constexpr int kGroup = 8;        // points per panel group
+inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+// kernel parameters in the form the FP32 evaluation uses: k = 2^(e0 * s) with
+// s = r^2 (Gaussian kinds) or s = r (exponential); Matérn stays FP64
+struct KernelParamsF {
+    int kind;
+    int dim;
+    float reg;
+    float e0;
+    double c1;
+};
+
+KernelParamsF make_params_f(const KernelParams& kp)
+{
+    const double log2e = 1.4426950408889634073599246810019;
+    KernelParamsF f{};
+    f.kind = kp.kind;
+    f.dim = kp.dim;
+    f.reg = (float)kp.reg;
+    f.c1 = kp.c1;
+    if (kp.kind == HMG_KERNEL_GAUSSIAN || kp.kind == HMG_KERNEL_EXPONENTIAL)
+        f.e0 = (float)(-kp.c0 * log2e);
+    else
+        f.e0 = (float)(-log2e);  // reference Gaussian exp(-r^2)
+    return f;
+}
+
+template <int KIND>
+__device__ __forceinline__ float kval_f32(const KernelParamsF& kp, float r2)
+{
+    if (KIND == HMG_KERNEL_GAUSSIAN || KIND == HMG_KERNEL_GAUSSIAN_REF)
+        return __builtin_amdgcn_exp2f(r2 * kp.e0);
+    if (KIND == HMG_KERNEL_EXPONENTIAL) return __builtin_amdgcn_exp2f(__builtin_amdgcn_sqrtf(r2) * kp.e0);
+    const double r = sqrt((double)r2);
+    return (float)((yn(1, r + 1.0e-15) * r) / kp.c1);
+}
+
+// ---- per-pass inputs --------------------------------------------------------------
+// Xs[p*32 + c] = X[order2[p] + (c0+c)*n2] (0 past n2 or nr); Xp panels: group
+// g = p/8 holds [hi | lo][32 right-hand sides][8 points] BF16
+__global__ void __launch_bounds__(kThreads)
+prep_x_kernel(const double* __restrict__ X, const uint64_t* __restrict__ order2, int n2, int n2p, int c0,
+              int nr, float* __restrict__ Xs, __bf16* __restrict__ Xp)
+{
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= (int64_t)n2p * kMxRhs) return;
+    const int p = (int)(t / kMxRhs), c = (int)(t % kMxRhs);
+    float v = 0.f;
+    if (p < n2 && c < nr) v = (float)X[order2[p] + (int64_t)(c0 + c) * n2];
+    Xs[t] = v;
+    const __bf16 h = (__bf16)v;
+    const __bf16 l = (__bf16)(v - (float)h);
+    const int64_t gbase = (int64_t)(p / kGroup) * 2 * kMxRhs * kGroup + c * kGroup + (p % kGroup);
+    Xp[gbase] = h;
+    Xp[gbase + kMxRhs * kGroup] = l;
+}
+
+// coordinate panels: cp[(g*dim + d)*8 + e] = x_d[8g + e] as FP32 (0 past n)
+__global__ void __launch_bounds__(kThreads)
+coord_panel_kernel(Coords c, int dim, int ngroups, float* __restrict__ cp)
+{
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= (int64_t)ngroups * kGroup) return;
+    const int p = (int)t;
+    const int g = p / kGroup, e = p % kGroup;
+    for (int d = 0; d < dim; d++) cp[((int64_t)g * dim + d) * kGroup + e] = p < c.n ? (float)c.x[d][p] : 0.f;
+}
+
+// ---- near field ----------------------------------------------------------------------
+template <int KIND, int DIM, bool REG>
+__device__ __forceinline__ f32x16 near_tile(const KernelParamsF& kp, int dim, const float* xr,
+                                            const f32x8* xc, int row, int col0, bf16x8 bh, bf16x8 bl,
+                                            f32x16 acc)
+{
+    constexpr int DM = DIM > 0 ? DIM : MAX_POINT_DIMENSION;
+    f32x8 kv;
+#pragma unroll
+    for (int e = 0; e < kGroup; e++) {
+        float r2 = 0.f;
+#pragma unroll
+        for (int d = 0; d < DM; d++) {
+            if (DIM > 0 || d < dim) {
+                const float t = xr[d] - xc[d][e];
+                r2 = __builtin_fmaf(t, t, r2);
+            }
+        }
+        float v = kval_f32<KIND>(kp, r2);
+        if (REG && row == col0 + e) v += kp.reg;
+        kv[e] = v;
+    }
+    const bf16x8 ah = __builtin_convertvector(kv, bf16x8);
+    const f32x8 hf = __builtin_convertvector(ah, f32x8);
+    const bf16x8 al = __builtin_convertvector(kv - hf, bf16x8);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+    return acc;
+}
+
+// One wavefront per item: up to 64 rows (two 32-row tiles) of one row cluster
+// against the item's column runs, 16 columns (two 8-point groups) per step.
+// Lane l evaluates row (l % 32) of each tile against the 8 columns of group
+// g + l/32, which is exactly its A operand; its B operand is the same group's
+// X panel for right-hand side l % 32.  Columns outside a run are masked by
+// zeroing their B values.  P[(q*64 + r)*32 + c] = row r's partial for rhs c.
+template <int KIND, int DIM, bool REG>
+__global__ void __launch_bounds__(kThreads)
+near_mx_kernel(const float* __restrict__ cp1, const float* __restrict__ cp2, KernelParamsF kp,
+               const MxItem* __restrict__ items, int nitems, const MxRef* __restrict__ refs,
+               const bf16x8* __restrict__ Xp, float* __restrict__ P)
+{
+    constexpr int DM = DIM > 0 ? DIM : MAX_POINT_DIMENSION;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = blockIdx.x * kWaves + wave_id();
+    if (q >= nitems) return;
+    const MxItem it = items[q];
+    const int dim = DIM > 0 ? DIM : kp.dim;
+    const int c32 = lane & 31, h = lane >> 5;
+    const bool two = it.nrows > 32;
+    float xr0[DM], xr1[DM];
+    const int rr0 = c32 < it.nrows ? c32 : 0;
+    const int rr1 = 32 + c32 < it.nrows ? 32 + c32 : 0;
+    const int row0 = it.row0 + rr0, row1 = it.row0 + rr1;
+#pragma unroll
+    for (int d = 0; d < DM; d++) {
+        if (DIM > 0 || d < dim) {
+            xr0[d] = cp1[((int64_t)(row0 >> 3) * dim + d) * kGroup + (row0 & 7)];
+            xr1[d] = cp1[((int64_t)(row1 >> 3) * dim + d) * kGroup + (row1 & 7)];
+        } else {
+            xr0[d] = xr1[d] = 0.f;
+        }
+    }
+    f32x16 acc0, acc1;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc0[r] = acc1[r] = 0.f;
+    for (int rf = it.ref0; rf < it.ref1; rf++) {
+        const MxRef f = refs[rf];
+        for (int g = f.g0; g < f.g1; g += 2) {
+            const int gg = g + h;
+            bf16x8 bh = Xp[((int64_t)gg * 2 + 0) * kMxRhs + c32];
+            bf16x8 bl = Xp[((int64_t)gg * 2 + 1) * kMxRhs + c32];
+            const int col0 = gg * kGroup;
+            if (col0 < f.lo || col0 + kGroup > f.hi) {
+#pragma unroll
+                for (int e = 0; e < kGroup; e++) {
+                    if (col0 + e < f.lo || col0 + e >= f.hi) {
+                        bh[e] = (__bf16)0.f;
+                        bl[e] = (__bf16)0.f;
+                    }
+                }
+            }
+            f32x8 xc[DM];
+#pragma unroll
+            for (int d = 0; d < DM; d++) {
+                if (DIM > 0 || d < dim)
+                    xc[d] = *(const f32x8*)(cp2 + ((int64_t)gg * dim + d) * kGroup);
+                else
+                    xc[d] = f32x8{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            }
+            acc0 = near_tile<KIND, DIM, REG>(kp, dim, xr0, xc, row0, col0, bh, bl, acc0);
+            if (two) acc1 = near_tile<KIND, DIM, REG>(kp, dim, xr1, xc, row1, col0, bh, bl, acc1);
+        }
+    }
+    float* __restrict__ out = P + (int64_t)q * kMxRowsItem * kMxRhs + c32;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int rr = (r >> 2) * 8 + h * 4 + (r & 3);
+        if (rr < it.nrows) out[(int64_t)rr * kMxRhs] = acc0[r];
+        if (32 + rr < it.nrows) out[(int64_t)(32 + rr) * kMxRhs] = acc1[r];
+    }
+}
+
+// ---- low rank ----------------------------------------------------------------------------
+// Tp[(q*128 + col)*32 + c] = sum over the item's points j of V[j][col] X[j][c]
+__global__ void __launch_bounds__(kThreads)
+lr_v32_kernel(const LvItem* __restrict__ items, int nitems, const float* __restrict__ V,
+              const float* __restrict__ Xs, float* __restrict__ Tp)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = blockIdx.x * kWaves + wave_id();
+    if (q >= nitems) return;
+    const LvItem it = items[q];
+    const int c32 = lane & 31, h = lane >> 5;
+    const int ntiles = (it.ncols + 31) >> 5;
+    const int64_t ld = it.ld;
+    const float* __restrict__ v = V + it.vbase + h * ld + c32;
+    const float* __restrict__ x = Xs + (int64_t)(it.j0 + h) * kMxRhs + c32;
+    f32x16 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[t][r] = 0.f;
+    const int n = it.nrows;
+    int j = 0;
+    for (; j + 8 <= n; j += 8) {
+        float b[4], a[4][4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            b[s] = x[(int64_t)(j + 2 * s) * kMxRhs];
+#pragma unroll
+            for (int t = 0; t < 4; t++) a[s][t] = t < ntiles ? v[(int64_t)(j + 2 * s) * ld + t * 32] : 0.f;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                if (t < ntiles) acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s][t], b[s], acc[t], 0, 0, 0);
+    }
+    for (; j < n; j += 2) {
+        const bool ok = j + h < n;
+        const float b = ok ? x[(int64_t)j * kMxRhs] : 0.f;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            if (t < ntiles) {
+                const float a = ok ? v[(int64_t)j * ld + t * 32] : 0.f;
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    float* __restrict__ out = Tp + (int64_t)q * kMxVCols * kMxRhs + c32;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        if (t < ntiles) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int col = t * 32 + (r >> 2) * 8 + h * 4 + (r & 3);
+                out[(int64_t)col * kMxRhs] = acc[t][r];
+            }
+        }
+    }
+}
+
+// T[tmap[tcol0 + col]*32 + c] = sum of the slice's point chunks, in chunk order
+__global__ void __launch_bounds__(kThreads)
+lr_v32_finalize_kernel(const LvSlice* __restrict__ slices, const float* __restrict__ Tp,
+                       const int* __restrict__ tmap, float* __restrict__ T)
+{
+    const LvSlice sl = slices[blockIdx.x];
+    for (int e = threadIdx.x; e < sl.ncols * kMxRhs; e += kThreads) {
+        const int col = e / kMxRhs, c = e % kMxRhs;
+        float s = 0.f;
+        for (int ch = 0; ch < sl.nchunks; ch++)
+            s += Tp[((int64_t)(sl.item0 + ch) * kMxVCols + col) * kMxRhs + c];
+        T[(int64_t)tmap[sl.tcol0 + col] * kMxRhs + c] = s;
+    }
+}
+
+// P[(q*64 + r)*32 + c] = sum_k U_C[row0 + r][k] T_C[k][c]
+__global__ void __launch_bounds__(kThreads)
+lr_u32_kernel(const LrItem* __restrict__ items, int nitems, const float* __restrict__ U,
+              const float* __restrict__ T, float* __restrict__ P)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = blockIdx.x * kWaves + wave_id();
+    if (q >= nitems) return;
+    const LrItem it = items[q];
+    const int c32 = lane & 31, h = lane >> 5;
+    const bool two = it.nrows > 32;
+    const int64_t ld = it.ld;
+    const float* __restrict__ u = U + it.uoff + h * ld + c32;
+    const float* __restrict__ t = T + (it.toff + h) * kMxRhs + c32;
+    f32x16 acc0, acc1;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc0[r] = acc1[r] = 0.f;
+    const int K = it.K;
+    int k = 0;
+    for (; k + 8 <= K; k += 8) {
+        float b[4], a0[4], a1[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            b[s] = t[(int64_t)(k + 2 * s) * kMxRhs];
+            a0[s] = u[(int64_t)(k + 2 * s) * ld];
+            a1[s] = two ? u[(int64_t)(k + 2 * s) * ld + 32] : 0.f;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[s], b[s], acc0, 0, 0, 0);
+            if (two) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[s], b[s], acc1, 0, 0, 0);
+        }
+    }
+    for (; k < K; k += 2) {
+        const bool ok = k + h < K;
+        const float b = ok ? t[(int64_t)k * kMxRhs] : 0.f;
+        const float a0 = ok ? u[(int64_t)k * ld] : 0.f;
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b, acc0, 0, 0, 0);
+        if (two) {
+            const float a1 = ok ? u[(int64_t)k * ld + 32] : 0.f;
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b, acc1, 0, 0, 0);
+        }
+    }
+    float* __restrict__ out = P + (int64_t)q * kMxRowsItem * kMxRhs + c32;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int rr = (r >> 2) * 8 + h * 4 + (r & 3);
+        if (rr < it.nrows) out[(int64_t)rr * kMxRhs] = acc0[r];
+        if (32 + rr < it.nrows) out[(int64_t)(32 + rr) * kMxRhs] = acc1[r];
+    }
+}
+
+// Y[order1[i] + (c0 + c)*n1] = sum of row i's partials (list order), c < nr
+__global__ void __launch_bounds__(kThreads)
+combine_mx_kernel(const int* __restrict__ ptr, const int* __restrict__ idx, const float* __restrict__ P,
+                  const uint64_t* __restrict__ order1, double* __restrict__ Y, int n1, int c0, int nr)
+{
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int i = (int)(t / kMxRhs), c = (int)(t % kMxRhs);
+    if (i >= n1) return;
+    float s = 0.f;
+    for (int q = ptr[i]; q < ptr[i + 1]; q++) s += P[(int64_t)idx[q] * kMxRhs + c];
+    if (c < nr) Y[order1[i] + (int64_t)(c0 + c) * n1] = (double)s;
+}
+
+template <int KIND, int DIM>
+void launch_near_mx(bool reg, unsigned grid, hipStream_t s, const Engine& eng, const KernelParamsF& kp,
+                    float* Pn)
+{
+    const bf16x8* xp = (const bf16x8*)eng.Xp.p;
+    if (reg)
+        near_mx_kernel<KIND, DIM, true><<<grid, kThreads, 0, s>>>(
+            eng.cpan[0].p, eng.cpan[1].p, kp, eng.mx_items.p, eng.n_mx_items, eng.mx_refs.p, xp, Pn);
+    else
+        near_mx_kernel<KIND, DIM, false><<<grid, kThreads, 0, s>>>(
+            eng.cpan[0].p, eng.cpan[1].p, kp, eng.mx_items.p, eng.n_mx_items, eng.mx_refs.p, xp, Pn);
+}
+
+template <int KIND>
+void launch_near_mx_dim(bool reg, unsigned grid, hipStream_t s, const Engine& eng, const KernelParamsF& kp,
+                        float* Pn)
+{
+    if (kp.dim == 3)
+        launch_near_mx<KIND, 3>(reg, grid, s, eng, kp, Pn);
+    else if (kp.dim == 2)
+        launch_near_mx<KIND, 2>(reg, grid, s, eng, kp, Pn);
+    else
+        launch_near_mx<KIND, 0>(reg, grid, s, eng, kp, Pn);
+}
+
+inline int64_t groups_of(int n) { return (n + kGroup - 1) / kGroup + 2; }
+
+}  // namespace
+
+hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
+                      const uint64_t* order1, const uint64_t* order2, const double* X, double* Y,
+                      int nrhs)
+{
+    if (eng.layout_prec != HMG_PREC_MIXED || !eng.mx_sched) {
+        set_error("mixed-precision product without a mixed-precision layout");
+        return HMG_ERR_NOT_READY;
+    }
+    hipStream_t s = eng.stream, s2 = eng.serialize ? eng.stream : eng.stream2;
+    const int n1 = eng.n[0], n2 = eng.n[1];
+    const int dim = kp.dim;
+    const int64_t g1 = groups_of(n1), g2 = groups_of(n2);
+    if (!eng.cpan_ready) {
+        HMG_TRY(eng.cpan[0].alloc(g1 * dim * kGroup));
+        HMG_TRY(eng.cpan[1].alloc(g2 * dim * kGroup));
+        coord_panel_kernel<<<grid_for(g1 * kGroup, kThreads), kThreads, 0, s>>>(c1, dim, (int)g1, eng.cpan[0].p);
+        HMG_HIP(hipGetLastError());
+        coord_panel_kernel<<<grid_for(g2 * kGroup, kThreads), kThreads, 0, s>>>(c2, dim, (int)g2, eng.cpan[1].p);
+        HMG_HIP(hipGetLastError());
+        eng.cpan_ready = true;
+    }
+    const int n2p = (int)(g2 * kGroup);
+    const int64_t near0 = (int64_t)eng.n_lr_items * kMxRowsItem;
+    const int64_t slots = near0 + (int64_t)eng.n_mx_items * kMxRowsItem;
+    HMG_TRY(eng.Xs32.alloc((size_t)n2p * kMxRhs));
+    HMG_TRY(eng.Xp.alloc((size_t)g2 * 2 * kMxRhs * kGroup));
+    HMG_TRY(eng.T32.alloc((size_t)std::max<int64_t>(eng.tsize, 1) * kMxRhs));
+    HMG_TRY(eng.Tp32.alloc((size_t)std::max(eng.n_lv_items, 1) * kMxVCols * kMxRhs));
+    HMG_TRY(eng.P32.alloc((size_t)std::max<int64_t>(slots, 1) * kMxRhs));
+    const KernelParamsF kpf = make_params_f(kp);
+    const bool reg = kp.reg != 0.0;
+    const float* U = eng.U32.p;
+    const float* V = eng.V32.p;
+    for (int c0 = 0; c0 < nrhs; c0 += kMxRhs) {
+        const int nr = std::min(kMxRhs, nrhs - c0);
+        eng.prof.begin(PROF_GATHER, s);
+        prep_x_kernel<<<grid_for((int64_t)n2p * kMxRhs, kThreads), kThreads, 0, s>>>(
+            X, order2, n2, n2p, c0, nr, eng.Xs32.p, eng.Xp.p);
+        HMG_HIP(hipGetLastError());
+        eng.prof.end(PROF_GATHER, s);
+        HMG_HIP(hipEventRecord(eng.ev_fork, s));
+        HMG_HIP(hipStreamWaitEvent(s2, eng.ev_fork, 0));
+        if (eng.n_mx_items > 0) {
+            const unsigned g = grid_for(eng.n_mx_items, kWaves);
+            float* Pn = eng.P32.p + near0 * kMxRhs;
+            eng.prof.begin(PROF_NEAR, s2);
+            switch (kp.kind) {
+            case HMG_KERNEL_GAUSSIAN:
+                launch_near_mx_dim<HMG_KERNEL_GAUSSIAN>(reg, g, s2, eng, kpf, Pn);
+                break;
+            case HMG_KERNEL_EXPONENTIAL:
+                launch_near_mx_dim<HMG_KERNEL_EXPONENTIAL>(reg, g, s2, eng, kpf, Pn);
+                break;
+            case HMG_KERNEL_GAUSSIAN_REF:
+                launch_near_mx_dim<HMG_KERNEL_GAUSSIAN_REF>(reg, g, s2, eng, kpf, Pn);
+                break;
+            default:
+                launch_near_mx_dim<HMG_KERNEL_MATERN_REF>(reg, g, s2, eng, kpf, Pn);
+                break;
+            }
+            HMG_HIP(hipGetLastError());
+            eng.prof.end(PROF_NEAR, s2);
+        }
+        HMG_HIP(hipEventRecord(eng.ev_join, s2));
+        if (eng.naca > 0 && eng.n_lr_items > 0) {
+            eng.prof.begin(PROF_LR_VTX, s);
+            lr_v32_kernel<<<grid_for(eng.n_lv_items, kWaves), kThreads, 0, s>>>(eng.lv_items.p, eng.n_lv_items,
+                                                                                  V, eng.Xs32.p, eng.Tp32.p);
+            HMG_HIP(hipGetLastError());
+            lr_v32_finalize_kernel<<<eng.n_lv_slices, kThreads, 0, s>>>(eng.lv_slices.p, eng.Tp32.p,
+                                                                        eng.tmap.p, eng.T32.p);
+            HMG_HIP(hipGetLastError());
+            eng.prof.end(PROF_LR_VTX, s);
+            eng.prof.begin(PROF_LR_U, s);
+            lr_u32_kernel<<<grid_for(eng.n_lr_items, kWaves), kThreads, 0, s>>>(eng.lr_items.p, eng.n_lr_items,
+                                                                                  U, eng.T32.p, eng.P32.p);
+            HMG_HIP(hipGetLastError());
+            eng.prof.end(PROF_LR_U, s);
+        }
+        HMG_HIP(hipStreamWaitEvent(s, eng.ev_join, 0));
+        eng.prof.begin(PROF_COMBINE, s);
+        combine_mx_kernel<<<grid_for((int64_t)n1 * kMxRhs, kThreads), kThreads, 0, s>>>(
+            eng.comb_mx.ptr.p, eng.comb_mx.idx.p, eng.P32.p, order1, Y, n1, c0, nr);
+        HMG_HIP(hipGetLastError());
+        eng.prof.end(PROF_COMBINE, s);
+    }
+    return HMG_OK;
+}
+
+}  // namespace hmg

@@ -30,7 +30,8 @@ constexpr int kSlice = kTileRows;          // rows per low-rank work item (one w
 constexpr int kNearSlice = 2 * kTileRows;  // rows per near-field work item (two per lane)
 constexpr int kNearChunkCols = 4096;       // near-field columns per work item
 constexpr int kVChunkRows = 1024;          // points per V^T x work item
-constexpr int kPad = 16;                   // factor leading dims: multiples of 128 bytes
+constexpr int kMxRows = 64;                // rows per mixed-precision near-field item
+constexpr int kMxChunkCols = 4096;         // columns per mixed-precision near-field item
 
 inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
@@ -109,7 +110,7 @@ std::vector<int> column_chunks(const std::vector<int>& m2s)
 // the combine expands, so neighbouring points read neighbouring slots.
 hmg_status build_combine(Combine& cb, int n1, const std::vector<int4>& near_rows, int64_t near0,
                          const std::vector<LrItem>& lr, const std::vector<ColSlots>& cols,
-                         hipStream_t s)
+                         hipStream_t s, int near_slice = kNearSlice)
 {
     std::vector<int> ptr((size_t)n1 + 1, 0);
     for (const int4& it : near_rows)
@@ -123,7 +124,7 @@ hmg_status build_combine(Combine& cb, int n1, const std::vector<int4>& near_rows
     std::vector<int> idx((size_t)ptr[n1]);
     for (size_t q = 0; q < near_rows.size(); q++) {
         const int4& it = near_rows[q];
-        for (int r = 0; r < it.y; r++) idx[fill[it.x + r]++] = (int)(near0 + (int64_t)q * kNearSlice + r);
+        for (int r = 0; r < it.y; r++) idx[fill[it.x + r]++] = (int)(near0 + (int64_t)q * near_slice + r);
     }
     for (size_t q = 0; q < lr.size(); q++) {
         const LrItem& it = lr[q];
@@ -142,6 +143,68 @@ hmg_status build_combine(Combine& cb, int n1, const std::vector<int4>& near_rows
         cb.group_entries += g.m2;
     }
     return HMG_OK;
+}
+
+// Mixed-precision near field: per dense row cluster, the column ranges of its
+// DENSE blocks sorted by start and merged where contiguous, cut into pieces of at
+// most kMxChunkCols columns; each piece becomes a run of 8-point groups (an even
+// number of groups, masked outside the piece).  Items are 64-row slices of the
+// row cluster against consecutive pieces of at most kMxChunkCols columns.
+void build_mixed_near(Engine& eng)
+{
+    eng.h_mx_items.clear();
+    eng.h_mx_refs.clear();
+    eng.mx_evals = 0;
+    const int ng = (int)eng.h_dgrp_ptr.size() - 1;
+    for (int g = 0; g < ng; g++) {
+        const int gb0 = eng.h_dgrp_ptr[g], gb1 = eng.h_dgrp_ptr[g + 1];
+        const int4 first = eng.h_blk[eng.h_dgrp_blk[gb0]];
+        const int l1 = first.x, m1 = first.y - first.x + 1;
+        std::vector<std::pair<int, int>> rng;  // [lo, hi)
+        for (int q = gb0; q < gb1; q++) {
+            const int4 b = eng.h_blk[eng.h_dgrp_blk[q]];
+            rng.emplace_back(b.z, b.w + 1);
+        }
+        std::sort(rng.begin(), rng.end());
+        std::vector<std::pair<int, int>> merged;
+        for (const auto& r : rng) {
+            if (!merged.empty() && r.first <= merged.back().second)
+                merged.back().second = std::max(merged.back().second, r.second);
+            else
+                merged.push_back(r);
+        }
+        std::vector<MxRef> pieces;
+        for (const auto& r : merged) {
+            for (int lo = r.first; lo < r.second; lo += kMxChunkCols) {
+                const int hi = std::min(r.second, lo + kMxChunkCols);
+                const int g0 = lo / 8;
+                const int gend = (hi + 7) / 8;
+                const int g1 = g0 + (int)round_up(gend - g0, 2);
+                pieces.push_back(MxRef{g0, g1, lo, hi});
+            }
+        }
+        // items: consecutive pieces up to kMxChunkCols evaluated columns
+        size_t p0 = 0;
+        while (p0 < pieces.size()) {
+            size_t p1 = p0;
+            int cols = 0;
+            while (p1 < pieces.size()) {
+                const int c = (pieces[p1].g1 - pieces[p1].g0) * 8;
+                if (p1 > p0 && cols + c > kMxChunkCols) break;
+                cols += c;
+                p1++;
+            }
+            const int ref0 = (int)eng.h_mx_refs.size();
+            eng.h_mx_refs.insert(eng.h_mx_refs.end(), pieces.begin() + p0, pieces.begin() + p1);
+            const int ref1 = (int)eng.h_mx_refs.size();
+            for (int r0 = 0; r0 < m1; r0 += kMxRows) {
+                eng.h_mx_items.push_back(MxItem{l1 + r0, std::min(kMxRows, m1 - r0), ref0, ref1});
+                eng.mx_evals += (int64_t)kMxRows * cols;
+            }
+            p0 = p1;
+        }
+    }
+    eng.n_mx_items = (int)eng.h_mx_items.size();
 }
 
 }  // namespace
@@ -263,9 +326,15 @@ hmg_status build_groups(Engine& eng)
     return HMG_OK;
 }
 
-hmg_status aca_layout(Engine& eng, int k)
+hmg_status aca_layout(Engine& eng, int k, int prec)
 {
     const int na = eng.naca;
+    const bool mixed = prec == HMG_PREC_MIXED;
+    // leading dims in elements: multiples of 128 bytes
+    const int kPad = mixed ? 32 : 16;
+    // columns per V^T x item: one wavefront of columns (FP64), four 32-column MFMA
+    // tiles (mixed)
+    const int vslice = mixed ? kMxVCols : kSlice;
     eng.h_kb.assign(na, 0);
     eng.kmax = 0;
     for (int a = 0; a < na; a++) {
@@ -338,8 +407,8 @@ hmg_status aca_layout(Engine& eng, int k)
             pos += eng.h_kb[a];
         }
         vrun += (int64_t)m2 * ld;
-        for (int s0 = 0; s0 < K; s0 += kSlice) {
-            const int nc = std::min(kSlice, K - s0);
+        for (int s0 = 0; s0 < K; s0 += vslice) {
+            const int nc = std::min(vslice, K - s0);
             const int item0 = (int)vitems.size();
             int nch = 0;
             for (int j0 = 0; j0 < m2; j0 += kVChunkRows, nch++)
@@ -374,6 +443,24 @@ hmg_status aca_layout(Engine& eng, int k)
         if (r.cbase >= 0) r.cbase += col0;
     HMG_TRY(upload(eng.fly_items, eng.h_fly_items, s));
     HMG_TRY(upload(eng.fly_refs, refs, s));
+    if (mixed) {
+        // P (mixed, kMxRhs values per slot) = [low-rank items x 64][near items x 64]
+        if (!eng.mx_sched) {
+            build_mixed_near(eng);
+            HMG_TRY(upload(eng.mx_items, eng.h_mx_items, s));
+            HMG_TRY(upload(eng.mx_refs, eng.h_mx_refs, s));
+            eng.mx_sched = true;
+        }
+        std::vector<int4> mx_rows((size_t)eng.n_mx_items);
+        for (int q = 0; q < eng.n_mx_items; q++)
+            mx_rows[q] = make_int4(eng.h_mx_items[q].row0, eng.h_mx_items[q].nrows, 0, 0);
+        const int64_t mslots = near0 + (int64_t)eng.n_mx_items * kMxRows;
+        if (mslots > (int64_t)INT32_MAX) {
+            set_error("mixed-precision partial buffer too large");
+            return HMG_ERR_INVALID;
+        }
+        HMG_TRY(build_combine(eng.comb_mx, n1, mx_rows, near0, items, {}, s, kMxRows));
+    }
 
     HMG_TRY(eng.kb.alloc(std::max(na, 1)));
     HMG_TRY(eng.keff.alloc(std::max(na, 1)));
@@ -395,6 +482,7 @@ hmg_status aca_layout(Engine& eng, int k)
     HMG_TRY(eng.T.alloc(std::max<int64_t>(eng.tsize, 1)));
     HMG_HIP(hipStreamSynchronize(s));
     eng.aca_k = k;
+    eng.layout_prec = prec;
     return HMG_OK;
 }
 

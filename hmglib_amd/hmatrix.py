@@ -199,6 +199,11 @@ class HMatrix:
         _abi.check(self.L.hmg_set_option(ctypes.byref(self.data), int(option), int(value)),
                    "hmg_set_option")
 
+    def set_precision(self, prec):
+        """_abi.PREC_FP64 or _abi.PREC_MIXED: factor precision of the next
+        precompute_aca(), and product precision when no factors are stored."""
+        self.set_option(_abi.OPT_PRECISION, prec)
+
     def set_profiling(self, on=True):
         _abi.check(self.L.hmg_set_profiling(ctypes.byref(self.data), 1 if on else 0), "hmg_set_profiling")
 

@@ -138,24 +138,35 @@ hmg_status hmg_dense_rows(const double* X, int nrhs, const int64_t* rows, int nr
  * out[5] = levels on which a size-1 cluster changed the bounding-box table,
  * out[6] = sum_aca m1*rank, out[7] = sum_aca m2*rank, out[8] = sum_aca rank. */
 hmg_status hmg_get_work(const struct h_matrix_data* data, int64_t out[9]);
-/* Shape of the H-MVP schedule: out[0] = near-field work items, out[1] = low-rank
+/* Shape of the H-MVP schedule (out holds 11 values): out[0] = near-field work items, out[1] = low-rank
  * work items, out[2] = rows over low-rank items, out[3] = t values read by the
  * low-rank items (sum of K over items), out[4] = sum_aca m1, out[5] = sum_aca m2,
  * out[6] = partial sums combined per product (entries of the per-row lists),
  * out[7] = 1 if the near field evaluates transposed DENSE pairs once,
  * out[8] = column-sum slots of that symmetric form, out[9] = 1 if both point
- * sets are the same points (sorted coordinates bitwise equal). */
-hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[10]);
+ * sets are the same points (sorted coordinates bitwise equal), out[10] = kernel
+ * entries the matrix-free near field evaluates per product. */
+hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
 /* Engine options (no counterpart in the reference; defaults in brackets):
  * HMG_OPT_NEAR_SYMMETRIC [1]: when both sets are the same points, evaluate each
  *   transposed pair of DENSE blocks once (row and column sums); takes effect at
  *   the next setup_h_matrix.
  * HMG_OPT_NEAR_LDS_BYTES [65536]: LDS reserved per near-field workgroup, which
  *   caps its occupancy so the low-rank streams keep wave slots beside it.
- * HMG_OPT_SERIALIZE [0]: run the near field on the main stream (no overlap). */
+ * HMG_OPT_SERIALIZE [0]: run the near field on the main stream (no overlap).
+ * HMG_OPT_PRECISION [HMG_PREC_FP64]: arithmetic of the next precompute_aca and
+ *   of every product.  HMG_PREC_MIXED stores the ACA factors in FP32 (residuals,
+ *   norms and pivot scaling in FP64), runs the low-rank products as FP32 MFMA
+ *   GEMMs and evaluates the near field in FP32, multiplied as BF16 hi+lo split
+ *   MFMA GEMMs accumulated in FP32, for up to 32 right-hand sides per pass.
+ *   X and Y stay FP64 at the boundary.  In this mode data->U and data->V point at
+ *   the FP32 factors (float arrays) and a stored FP64 near field is not used. */
 #define HMG_OPT_NEAR_SYMMETRIC 1
 #define HMG_OPT_NEAR_LDS_BYTES 2
 #define HMG_OPT_SERIALIZE 3
+#define HMG_OPT_PRECISION 4
+#define HMG_PREC_FP64 0
+#define HMG_PREC_MIXED 1
 hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value);
 /* Per-kernel timing with HIP events recorded on the engine's streams around each
  * launch.  enable != 0 starts (and clears) recording.  Kernel classes: 0 = x

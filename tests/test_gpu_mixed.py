@@ -1,0 +1,199 @@
+"""GPU parity of the mixed-precision H-MVP (FP32 factors, FP32 MFMA low-rank
+products, near field evaluated in FP32 and multiplied as BF16 hi+lo split MFMA
+GEMMs) against the FP64 engine on the same H-matrix and against the dense FP64
+product of the oracle.
+
+Tolerances (relative 2-norm error per right-hand side):
+  mixed vs dense FP64: 1e-4, the stated mixed-precision bar;
+  mixed vs the FP64 H-MVP of the same tree: 1e-4 in general.  FP32-stored
+    residuals can flip an ACA column pivot where two candidates are within
+    rounding of each other, which moves the product by the ACA truncation error at
+    that rank (up to ~3e-5 at rank 16 for a 2D Gaussian of length scale 0.1);
+  arithmetic alone, where truncation cannot matter (every block dense, or ranks
+    high enough that the ACA is exact to ~1e-9): 2e-5 (FP32 rounding of factors
+    and partial sums, ~2^-17 per entry for the BF16 hi+lo split).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_MIXED_ARITH = 2e-5
+TOL_MIXED_VS_FP64 = 1e-4
+TOL_MIXED_VS_DENSE = 1e-4
+
+
+def _points(seed, dim, n):
+    return O.uniform(seed, dim * n).reshape(dim, n)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _build(gpu, X1, X2, bits, eta, c_leaf, k, kind, ls, reg=0.0, prec=None, precompute=True):
+    hm = gpu.HMatrix(X1, X2, bits, eta, 50, c_leaf, k)
+    hm.setup()
+    kern = gpu.Kernel(kind, reg, ls)
+    hm.set_kernel(kern)
+    hm.set_precision(gpu.PREC_MIXED if prec is None else prec)
+    if precompute:
+        hm.precompute_aca()
+    return hm, kern
+
+
+def _check_columns(Ymx, Yref, tol):
+    for c in range(Yref.shape[1]):
+        e = _rel(Ymx[:, c], Yref[:, c])
+        assert e < tol, f"rhs {c}: rel err {e:.3e} >= {tol:.1e}"
+
+
+def test_mixed_c1_32_rhs_against_fp64_and_dense(gpu):
+    """Config 1 points and kernel (N=4096 uniform [0,1]^2, exp(-r^2/0.02), leaf
+    64, eta 1.5, rank 32) with 32 right-hand sides U[-1,1]."""
+    n = 4096
+    X = _points(1001, 2, n)
+    R = 32
+    Xr = (O.uniform(1003, n * R) * 2.0 - 1.0).reshape(n, R, order="F")
+    h64, _ = _build(gpu, X, X, 32, 1.5, 64, 32, O.K_GAUSSIAN, 0.1, prec=gpu.PREC_FP64)
+    hmx, _ = _build(gpu, X, X, 32, 1.5, 64, 32, O.K_GAUSSIAN, 0.1)
+    Y64 = h64.apply(Xr)
+    Ymx = hmx.apply(Xr)
+    _check_columns(Ymx, Y64, TOL_MIXED_ARITH)
+    om = O.HMatrix(X, X, 32, 1.5, 50, 64)
+    ok = O.kernel(O.K_GAUSSIAN, 2, 0.0, 0.1)
+    for c in (0, 13, 31):
+        yd = om.dense_mvp(ok, Xr[:, c])
+        assert _rel(Ymx[:, c], yd) < TOL_MIXED_VS_DENSE
+    # bitwise deterministic
+    np.testing.assert_array_equal(Ymx, hmx.apply(Xr))
+    s = hmx.schedule()
+    assert s["near_items"] > 0 and s["near_evals"] >= hmx.work()["E_near"]
+
+
+@pytest.mark.parametrize("nrhs", [1, 5, 40, 65])
+def test_mixed_rhs_counts(gpu, nrhs):
+    """Right-hand sides run in passes of 32 (zero-padded); every column matches
+    the FP64 product and the single-RHS mixed product."""
+    n = 6000
+    X = _points(2001, 3, n)
+    Xr = O.normal(2002, n * nrhs).reshape(n, nrhs, order="F")
+    h64, _ = _build(gpu, X, X, 20, 1.0, 128, 16, O.K_EXPONENTIAL, 0.1, prec=gpu.PREC_FP64)
+    hmx, _ = _build(gpu, X, X, 20, 1.0, 128, 16, O.K_EXPONENTIAL, 0.1)
+    Ymx = hmx.apply(Xr)
+    _check_columns(Ymx, h64.apply(Xr), TOL_MIXED_ARITH)
+    for c in sorted({0, nrhs - 1, nrhs // 2}):
+        y1 = hmx.apply(Xr[:, c])
+        assert _rel(Ymx[:, c], y1) < 1e-6
+
+
+@pytest.mark.parametrize("kind,ls,reg,same", [
+    (O.K_EXPONENTIAL, 0.1, 0.0, False),
+    (O.K_GAUSSIAN, 0.1, 0.0, False),
+    (O.K_GAUSSIAN_REF, 1.0, 0.0, False),
+    (O.K_MATERN_REF, 1.0, 0.0, False),
+    (O.K_GAUSSIAN, 0.1, 0.5, True),
+    (O.K_EXPONENTIAL, 0.1, 0.25, True),
+])
+def test_mixed_kernels_and_regularization(gpu, kind, ls, reg, same):
+    n1, n2 = 3000, (3000 if same else 2200)
+    X1 = _points(3001, 2, n1)
+    X2 = X1 if same else _points(3002, 2, n2)
+    Xr = O.normal(3003, n2 * 8).reshape(n2, 8, order="F")
+    h64, _ = _build(gpu, X1, X2, 32, 1.5, 64, 16, kind, ls, reg=reg, prec=gpu.PREC_FP64)
+    hmx, _ = _build(gpu, X1, X2, 32, 1.5, 64, 16, kind, ls, reg=reg)
+    Ymx = hmx.apply(Xr)
+    Y64 = h64.apply(Xr)
+    _check_columns(Ymx, Y64, TOL_MIXED_VS_FP64)
+    om = O.HMatrix(X1, X2, 32, 1.5, 50, 64)
+    ok = O.kernel(kind, 2, reg, ls)
+    for c in (0, 7):
+        yd = om.dense_mvp(ok, Xr[:, c])
+        # rank 16 leaves an H-approximation error of up to ~1e-4 here in FP64 as
+        # well: mixed precision may add at most the arithmetic tolerance to it
+        assert _rel(Ymx[:, c], yd) < _rel(Y64[:, c], yd) + TOL_MIXED_ARITH
+
+
+@pytest.mark.parametrize("kind,ls,reg", [(O.K_EXPONENTIAL, 0.1, 0.0), (O.K_GAUSSIAN, 0.1, 0.3),
+                                         (O.K_MATERN_REF, 1.0, 0.0)])
+def test_mixed_near_field_arithmetic(gpu, kind, ls, reg):
+    """Every block dense (leaf size above N): the product is the near field alone,
+    evaluated in FP32 and multiplied as BF16 hi+lo MFMAs.  The error of a sum of
+    rounded terms is bounded relative to sum |K_ij| |x_j|, not to |y| (the
+    reference Matern kernel is almost constant on the unit cube, so its y cancels
+    heavily); every kernel here is single-signed on these inputs (Y_1 < 0 below
+    r = 2.19), so |K| |x| = |K |x||."""
+    n1, n2 = 2500, 2300
+    X1 = _points(6001, 3, n1)
+    X2 = X1 if reg else _points(6002, 3, n2)
+    n2 = X2.shape[1]
+    Xr = O.normal(6003, n2 * 32).reshape(n2, 32, order="F")
+    hmx, _ = _build(gpu, X1, X2, 20, 1.0, 4096, 8, kind, ls, reg=reg)
+    assert hmx.work()["aca"] == 0
+    Ymx = hmx.apply(Xr)
+    om = O.HMatrix(X1, X2, 20, 1.0, 50, 4096)
+    ok = O.kernel(kind, 3, reg, ls)
+    for c in (0, 11, 31):
+        yd = om.dense_mvp(ok, Xr[:, c])
+        mag = np.linalg.norm(om.dense_mvp(ok, np.abs(Xr[:, c])))
+        assert np.linalg.norm(Ymx[:, c] - yd) / mag < TOL_MIXED_ARITH
+
+
+def test_mixed_low_rank_arithmetic(gpu):
+    """Ranks high enough that the ACA truncation error is far below FP32 rounding
+    (rank 64 on leaves of 32): mixed and FP64 products differ by FP32 arithmetic
+    only."""
+    n = 4000
+    X = _points(7001, 2, n)
+    Xr = O.normal(7002, n * 32).reshape(n, 32, order="F")
+    h64, _ = _build(gpu, X, X, 32, 1.5, 32, 64, O.K_EXPONENTIAL, 0.1, prec=gpu.PREC_FP64)
+    hmx, _ = _build(gpu, X, X, 32, 1.5, 32, 64, O.K_EXPONENTIAL, 0.1)
+    assert hmx.work()["aca"] > 0
+    _check_columns(hmx.apply(Xr), h64.apply(Xr), TOL_MIXED_ARITH)
+
+
+def test_mixed_3d_against_dense(gpu):
+    """3D exponential kernel (the scale configuration's kernel and parameters at a
+    size the oracle's dense product finishes quickly)."""
+    n = 8192
+    X = _points(2001, 3, n)
+    Xr = O.normal(2002, n * 4).reshape(n, 4, order="F")
+    hmx, _ = _build(gpu, X, X, 20, 1.0, 256, 24, O.K_EXPONENTIAL, 0.1)
+    Ymx = hmx.apply(Xr)
+    om = O.HMatrix(X, X, 20, 1.0, 50, 256)
+    ok = O.kernel(O.K_EXPONENTIAL, 3, 0.0, 0.1)
+    for c in range(4):
+        assert _rel(Ymx[:, c], om.dense_mvp(ok, Xr[:, c])) < TOL_MIXED_VS_DENSE
+
+
+def test_mixed_not_precomputed_matches_precomputed(gpu):
+    n = 5000
+    X = _points(4001, 3, n)
+    Xr = O.normal(4002, n * 3).reshape(n, 3, order="F")
+    hmx, _ = _build(gpu, X, X, 20, 1.0, 128, 12, O.K_EXPONENTIAL, 0.1, precompute=False)
+    Y_fly = hmx.apply(Xr)
+    hmx.precompute_aca()
+    np.testing.assert_array_equal(Y_fly, hmx.apply(Xr))
+
+
+def test_switching_precision_back_to_fp64(gpu):
+    """Stored factors decide a product's precision; rebuilding them in FP64 gives
+    exactly the FP64 engine's product again."""
+    n = 4000
+    X = _points(5001, 2, n)
+    x = O.normal(5002, n)
+    h64, _ = _build(gpu, X, X, 32, 1.5, 64, 16, O.K_GAUSSIAN, 0.1, prec=gpu.PREC_FP64)
+    hmx, _ = _build(gpu, X, X, 32, 1.5, 64, 16, O.K_GAUSSIAN, 0.1)
+    y_mx = hmx.apply(x)
+    hmx.set_precision(gpu.PREC_FP64)
+    # factors are still FP32: the product stays mixed until precompute_aca runs again
+    np.testing.assert_array_equal(hmx.apply(x), y_mx)
+    hmx.precompute_aca()
+    np.testing.assert_array_equal(hmx.apply(x), h64.apply(x))
+    # the unbatched reference path is FP64 and leaves the stored factors usable
+    hmx.set_precision(gpu.PREC_MIXED)
+    hmx.precompute_aca()
+    hmx.apply_without_batching(x)
+    np.testing.assert_array_equal(hmx.apply(x), y_mx)
