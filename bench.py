@@ -179,6 +179,99 @@ WORKLOADS = {
 }
 
 
+def serving_requests(seed_arrivals, seed_params, qps, duration):
+    """The serving workload: Poisson arrivals at `qps` for `duration` seconds; each
+    request a point cloud with N = floor(2^(12 + 6u)), uniform or an 8-blob
+    Gaussian mixture (centres U[0,1]^d, sigma 0.05, clipped), 2D or 3D, exponential
+    or Gaussian kernel (length scale 0.1), 1-16 right-hand sides; 70% applies
+    against a cached H-matrix, 30% builds.  Tree parameters follow the small and
+    the scale configuration: 2D leaf 64, eta 1.5, rank 32; 3D leaf 256, eta 1.0,
+    rank 24."""
+    ra = np.random.default_rng(seed_arrivals)
+    rp = np.random.default_rng(seed_params)
+    t, out = 0.0, []
+    while True:
+        t += ra.exponential(1.0 / qps)
+        if t > duration:
+            break
+        u = rp.random()
+        req = {"t": t, "build": rp.random() < 0.3, "n": int(2 ** (12 + 6 * u)),
+               "dim": 2 if rp.random() < 0.5 else 3, "blobs": rp.random() < 0.5,
+               "kind": 4 if rp.random() < 0.5 else 3, "nrhs": int(rp.integers(1, 17)),
+               "pick": rp.random(), "seed": int(rp.integers(1 << 30))}
+        out.append(req)
+    return out
+
+
+def cloud(rng, n, dim, blobs):
+    if not blobs:
+        return rng.random((dim, n))
+    centres = rng.random((8, dim))
+    lab = rng.integers(0, 8, n)
+    return np.clip(centres[lab].T + 0.05 * rng.standard_normal((dim, n)), 0.0, 1.0)
+
+
+def serve_session(reqs, precision, budget, slots, log_tag):
+    """Replays `reqs` open-loop against a native server; returns latency stats."""
+    import hmglib_amd
+    from hmglib_amd import hip
+    from hmglib_amd.server import Server
+    nmax = max(r["n"] for r in reqs)
+    # right-hand sides: one device pool read by every apply (prefix of n x nrhs);
+    # outputs: a ring of buffers, each large enough for any request
+    Xpool = hip.DeviceArray(np.asfortranarray(np.random.default_rng(7).standard_normal((nmax * 16,))))
+    ring = [hip.DeviceArray.empty(nmax * 16) for _ in range(96)]
+    srv = Server(budget_bytes=budget, slots=slots, max_rhs_per_pass=32, precision=precision)
+    ids = []
+    mat_n = {}
+    t0 = time.perf_counter()
+    k_ring = 0
+    for r in reqs:
+        dt = r["t"] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        resident = srv.resident()
+        if r["build"] or not resident:
+            rng = np.random.default_rng(r["seed"])
+            X = cloud(rng, r["n"], r["dim"], r["blobs"])
+            bits, leaf, eta, k = (32, 64, 1.5, 32) if r["dim"] == 2 else (20, 256, 1.0, 24)
+            rid, mid = srv.submit_build(X, X, bits, eta, 50, leaf, k, r["kind"], 0.0, 0.1)
+            mat_n[mid] = r["n"]
+            ids.append((rid, "build", r["n"]))
+        else:
+            m = resident[int(r["pick"] * len(resident))]
+            y = ring[k_ring % len(ring)]
+            k_ring += 1
+            rid = srv.submit_apply_ptr(m, Xpool.ptr, y.ptr, r["nrhs"])
+            ids.append((rid, "apply", mat_n[m]))
+    srv.drain()
+    wall = time.perf_counter() - t0
+    lat = {"build": [], "apply": []}
+    fails = 0
+    for rid, kind, n in ids:
+        info = srv.query(rid)
+        if info["status"] != 0:
+            fails += 1
+            continue
+        lat[kind].append((info["t_end"] - info["t_submit"]) * 1e3)
+    st = srv.stats()
+    srv.destroy()
+    for b in ring:
+        b.free()
+    Xpool.free()
+    allv = np.array(lat["build"] + lat["apply"])
+    res = {"requests": len(ids), "failed": fails, "wall_s": wall, "throughput_rps": len(allv) / wall,
+           "p50_ms": float(np.percentile(allv, 50)), "p99_ms": float(np.percentile(allv, 99)),
+           "apply_p50_ms": float(np.percentile(lat["apply"], 50)) if lat["apply"] else None,
+           "apply_p99_ms": float(np.percentile(lat["apply"], 99)) if lat["apply"] else None,
+           "build_p50_ms": float(np.percentile(lat["build"], 50)) if lat["build"] else None,
+           "build_p99_ms": float(np.percentile(lat["build"], 99)) if lat["build"] else None,
+           "builds": len(lat["build"]), "applies": len(lat["apply"]), "evictions": st["evictions"],
+           "passes": st["passes"], "shared_requests": st["shared"]}
+    log(f"[bench] serving {log_tag}: {json.dumps(res)}")
+    return res
+
+
 def timed_products(hm, xd, yd, nrhs, steps, warmup, hip):
     for _ in range(warmup):
         hm.apply_dev(xd, yd, nrhs)
@@ -209,6 +302,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 leg of c3")
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--cpu-stride", type=int, default=16)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-qps", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -314,6 +411,17 @@ def main():
         Xd.free()
         Yd.free()
     log(f"[bench] kernels {json.dumps(kernels)}")
+    hm.destroy()
+
+    served = None
+    if mixed and not args.no_serve:
+        reqs = serving_requests(5001, 5002, args.serve_qps, args.serve_seconds)
+        served = {"qps": args.serve_qps, "seconds": args.serve_seconds, "budget_GB": 200,
+                  "slots": args.serve_slots,
+                  "workload": "Poisson arrivals; N=2^(12+6u); uniform or 8-blob; 2D/3D; exp or Gaussian "
+                              "l=0.1; 1-16 RHS; 70% applies to cached H-matrices, 30% builds"}
+        for name, prec in (("fp64", hmglib_amd.PREC_FP64), ("mixed", hmglib_amd.PREC_MIXED)):
+            served[name] = serve_session(reqs, prec, 200e9, args.serve_slots, name)
 
     cpu = None
     if not args.no_cpu:
@@ -337,6 +445,8 @@ def main():
                                     "rel_err_fp64_rows": plain["rel_err_fp64_rows"]}
             config["mixed_over_plain"] = result["value"] / plain["value"]
             config["build_s"]["aca_fp64"] = plain["aca_s"]
+        if served is not None:
+            config["served"] = served
         value, ms, steps, dtype = result["value"], result["ms"], args.steps, "f32/bf16x3 (mixed)"
         metric = ("H-MVP/s at N=2^20 3D, 32 RHS, mixed precision (rel.err<=1e-4 vs FP64), "
                   "exp(-r/0.1), rank-24 ACA")

@@ -103,6 +103,26 @@ class HMatrixData(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    """hmg_server_config (include/hmglib_server.h)."""
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("slots", ctypes.c_int),
+                ("max_rhs_per_pass", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+class BuildRequest(ctypes.Structure):
+    """hmg_build_request (include/hmglib_server.h)."""
+    _fields_ = [("dim", ctypes.c_int), ("bits", ctypes.c_int), ("n", ctypes.c_int * 2),
+                ("coords", c_double_p * 2), ("eta", ctypes.c_double), ("max_level", ctypes.c_int),
+                ("c_leaf", ctypes.c_int), ("k", ctypes.c_int), ("kernel_kind", ctypes.c_int),
+                ("regularization", ctypes.c_double), ("length_scale", ctypes.c_double)]
+
+
+class RequestInfo(ctypes.Structure):
+    """hmg_request_info (include/hmglib_server.h)."""
+    _fields_ = [("status", ctypes.c_int), ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
+                ("t_end", ctypes.c_double), ("merged", ctypes.c_int), ("matrix", ctypes.c_int64)]
+
+
 # name -> (restype, argtypes) for every symbol declared in include/*.h
 _P = ctypes.POINTER
 _D = ctypes.POINTER(HMatrixData)
@@ -137,6 +157,20 @@ SIGNATURES = {
     "hmg_set_option": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int64]),
     "hmg_get_profile": (ctypes.c_int, [_D, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64)]),
     "hmg_copy_probe": (ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
+    "hmg_memory": (ctypes.c_int, [_D, _P(ctypes.c_int64)]),
+    # serving (include/hmglib_server.h)
+    "hmg_server_create": (ctypes.c_int, [_P(ServerConfig), _P(_vp)]),
+    "hmg_server_destroy": (ctypes.c_int, [_vp]),
+    "hmg_server_submit_build": (ctypes.c_int, [_vp, _P(BuildRequest), _P(ctypes.c_int64),
+                                               _P(ctypes.c_int64)]),
+    "hmg_server_submit_apply": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, ctypes.c_int,
+                                               _P(ctypes.c_int64)]),
+    "hmg_server_wait": (ctypes.c_int, [_vp, ctypes.c_int64, _P(RequestInfo)]),
+    "hmg_server_query": (ctypes.c_int, [_vp, ctypes.c_int64, _P(RequestInfo)]),
+    "hmg_server_drain": (ctypes.c_int, [_vp]),
+    "hmg_server_release": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "hmg_server_stats": (ctypes.c_int, [_vp, _P(ctypes.c_int64)]),
+    "hmg_server_resident": (ctypes.c_int, [_vp, _P(ctypes.c_int64), ctypes.c_int]),
     "hmg_last_error": (ctypes.c_char_p, []),
     "hmg_struct_layout": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int]),
     "hmg_rng_uniform": (None, [ctypes.c_uint64, ctypes.c_int64, c_double_p]),

@@ -10,6 +10,7 @@
 #include <cstddef>
 
 #include "hmg_internal.h"
+#include "../../include/hmglib_server.h"
 
 namespace hmg {
 
@@ -76,7 +77,12 @@ hmg_status read_kernel(const h_matrix_data* d, KernelParams* kp)
         return HMG_ERR_NOT_READY;
     }
     system_assembler a{};
-    HMG_HIP(hipMemcpy(&a, d->assem, sizeof(a), hipMemcpyDeviceToHost));
+    // on the engine's own (non-blocking) stream: a null-stream copy would wait
+    // for every other H-matrix's work on the device
+    Engine* e = engine_of(d);
+    hipStream_t s = e ? e->stream : nullptr;
+    HMG_HIP(hipMemcpyAsync(&a, d->assem, sizeof(a), hipMemcpyDeviceToHost, s));
+    HMG_HIP(hipStreamSynchronize(s));
     if (a.kind < HMG_KERNEL_GAUSSIAN_REF || a.kind > HMG_KERNEL_EXPONENTIAL) {
         set_error("unknown kernel kind " + std::to_string(a.kind));
         return HMG_ERR_INVALID;
@@ -239,8 +245,10 @@ hmg_status build_factors(Engine& e, h_matrix_data* d, const KernelParams& kp, in
 {
     HMG_TRY(aca_layout(e, k, sizeof(T) == sizeof(float) ? HMG_PREC_MIXED : HMG_PREC_FP64));
     const int na = e.naca;
-    HMG_TRY(U.alloc(std::max<int64_t>(e.h_uoff[na], 1)));
-    HMG_TRY(V.alloc(std::max<int64_t>(e.h_voff[na], 1)));
+    // 256 elements of zero padding past the end: the mixed-precision products
+    // read whole 128-element rows or columns, up to 32 elements beyond the last
+    HMG_TRY(U.alloc(e.h_uoff[na] + 256));
+    HMG_TRY(V.alloc(e.h_voff[na] + 256));
     HMG_HIP(hipMemsetAsync(U.p, 0, sizeof(T) * U.n, e.stream));
     HMG_HIP(hipMemsetAsync(V.p, 0, sizeof(T) * V.n, e.stream));
     HMG_TRY(aca_build<T>(e, coords_of(d, 0), coords_of(d, 1), kp, k, tie_last, U.p, V.p, e.keff.p));
@@ -277,8 +285,10 @@ hmg_status hmg_init_h_matrix_data(struct h_matrix_data* data, int point_count[2]
         return HMG_ERR_INVALID;
     }
     Engine* e = new Engine();
-    HMG_HIP(hipStreamCreate(&e->stream));
-    HMG_HIP(hipStreamCreate(&e->stream2));
+    // non-blocking streams: no implicit ordering with the null stream, so the work
+    // of independent H-matrices driven from different host threads overlaps
+    HMG_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HMG_HIP(hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking));
     HMG_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
     HMG_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
     e->dim = dim;
@@ -310,17 +320,18 @@ hmg_status hmg_init_h_matrix_data(struct h_matrix_data* data, int point_count[2]
         HMG_HIP(hipMalloc((void**)&data->max_per_dim_d[i], sizeof(double) * dim));
         HMG_HIP(hipMalloc((void**)&data->min_per_dim_d[i], sizeof(double) * dim));
         HMG_HIP(hipMalloc((void**)&data->coords_device[i], sizeof(double*) * dim));
-        HMG_HIP(hipMemcpy(data->coords_device[i], data->coords_d[i], sizeof(double*) * dim,
-                          hipMemcpyHostToDevice));
+        HMG_HIP(hipMemcpyAsync(data->coords_device[i], data->coords_d[i], sizeof(double*) * dim,
+                               hipMemcpyHostToDevice, e->stream));
         HMG_HIP(hipMalloc((void**)&data->code_d[i], sizeof(uint64_t) * n));
         HMG_HIP(hipMalloc((void**)&data->order[i], sizeof(uint64_t) * n));
         point_set ps{data->coords_device[i], dim, n, data->max_per_dim_d[i], data->min_per_dim_d[i],
                      data->point_ids_d[i]};
         HMG_HIP(hipMalloc((void**)&data->points_d[i], sizeof(point_set)));
-        HMG_HIP(hipMemcpy(data->points_d[i], &ps, sizeof(ps), hipMemcpyHostToDevice));
+        HMG_HIP(hipMemcpyAsync(data->points_d[i], &ps, sizeof(ps), hipMemcpyHostToDevice, e->stream));
         morton_code mc{data->code_d[i], n, dim, bits, nullptr, nullptr};
         HMG_HIP(hipMalloc((void**)&data->morton_d[i], sizeof(morton_code)));
-        HMG_HIP(hipMemcpy(data->morton_d[i], &mc, sizeof(mc), hipMemcpyHostToDevice));
+        HMG_HIP(hipMemcpyAsync(data->morton_d[i], &mc, sizeof(mc), hipMemcpyHostToDevice, e->stream));
+        HMG_HIP(hipStreamSynchronize(e->stream));
     }
     return HMG_OK;
 }
@@ -573,7 +584,9 @@ hmg_status hmg_apply_h_matrix_mvp_without_batching(double* x, double* y, struct 
     std::vector<int> keep_keff;
     if (had && e->naca > 0) {
         keep_keff.resize(e->naca);
-        HMG_HIP(hipMemcpy(keep_keff.data(), e->keff.p, sizeof(int) * e->naca, hipMemcpyDeviceToHost));
+        HMG_HIP(hipMemcpyAsync(keep_keff.data(), e->keff.p, sizeof(int) * e->naca, hipMemcpyDeviceToHost,
+                               e->stream));
+        HMG_HIP(hipStreamSynchronize(e->stream));
     }
     const int had_k = e->aca_k;
     HMG_TRY(build_factors(*e, data, kp, std::max(data->k, 0), 0, tU, tV));
@@ -586,7 +599,11 @@ hmg_status hmg_apply_h_matrix_mvp_without_batching(double* x, double* y, struct 
         // restore the stored factors' layout and ranks
         HMG_TRY(aca_layout(*e, had_k, e->fact_prec));
         if (e->naca > 0)
-            HMG_HIP(hipMemcpy(e->keff.p, keep_keff.data(), sizeof(int) * e->naca, hipMemcpyHostToDevice));
+        {
+            HMG_HIP(hipMemcpyAsync(e->keff.p, keep_keff.data(), sizeof(int) * e->naca, hipMemcpyHostToDevice,
+                                   e->stream));
+            HMG_HIP(hipStreamSynchronize(e->stream));
+        }
     } else {
         e->aca_k = -1;
     }
@@ -643,7 +660,10 @@ hmg_status hmg_get_work(const struct h_matrix_data* data, int64_t out[9])
     }
     std::vector<int> ranks(e->h_kb);
     if (data->U && e->aca_k >= 0 && e->naca > 0)
-        HMG_HIP(hipMemcpy(ranks.data(), e->keff.p, sizeof(int) * e->naca, hipMemcpyDeviceToHost));
+    {
+        HMG_HIP(hipMemcpyAsync(ranks.data(), e->keff.p, sizeof(int) * e->naca, hipMemcpyDeviceToHost, e->stream));
+        HMG_HIP(hipStreamSynchronize(e->stream));
+    }
     for (int a = 0; a < e->naca; a++) {
         const int4 bl = e->h_blk[e->ndense + a];
         el1 += (int64_t)(bl.y - bl.x + 1) * ranks[a];
@@ -668,8 +688,11 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11])
     HMG_TRY(check_ready(data, &e));
     std::vector<LrItem> items((size_t)e->n_lr_items);
     if (!items.empty())
-        HMG_HIP(hipMemcpy(items.data(), e->lr_items.p, sizeof(LrItem) * items.size(),
-                          hipMemcpyDeviceToHost));
+    {
+        HMG_HIP(hipMemcpyAsync(items.data(), e->lr_items.p, sizeof(LrItem) * items.size(),
+                               hipMemcpyDeviceToHost, e->stream));
+        HMG_HIP(hipStreamSynchronize(e->stream));
+    }
     int64_t rows = 0, treads = 0, m1s = 0, m2s = 0;
     for (const LrItem& it : items) {
         rows += it.nrows;
@@ -694,6 +717,95 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11])
     out[8] = mixed ? 0 : e->n_col_slots;
     out[9] = e->same_sets ? 1 : 0;
     out[10] = mixed ? e->mx_evals : stored ? 0 : e->near_evals;
+    return HMG_OK;
+}
+
+extern "C++" {
+namespace {
+
+template <class T>
+int64_t bytes_of(const DevBuf<T>& b)
+{
+    return (int64_t)(b.p ? b.n : 0) * (int64_t)sizeof(T);
+}
+
+int64_t engine_bytes(const Engine& e)
+{
+    int64_t t = 0;
+    t += bytes_of(e.perm32[0]) + bytes_of(e.perm32[1]) + bytes_of(e.blocks) + bytes_of(e.blk);
+    t += bytes_of(e.kb) + bytes_of(e.keff) + bytes_of(e.ldv) + bytes_of(e.ldu);
+    t += bytes_of(e.uoff) + bytes_of(e.voff) + bytes_of(e.toff) + bytes_of(e.U) + bytes_of(e.V);
+    t += bytes_of(e.aoff) + bytes_of(e.A) + bytes_of(e.near_items) + bytes_of(e.near_blk);
+    t += bytes_of(e.fly_items) + bytes_of(e.fly_refs) + bytes_of(e.lr_items) + bytes_of(e.lv_items);
+    t += bytes_of(e.lv_slices) + bytes_of(e.tmap) + bytes_of(e.Tp) + bytes_of(e.P) + bytes_of(e.xs);
+    t += bytes_of(e.T) + bytes_of(e.U32) + bytes_of(e.V32) + bytes_of(e.mx_items) + bytes_of(e.mx_refs);
+    t += bytes_of(e.Xs32) + bytes_of(e.T32) + bytes_of(e.Tp32) + bytes_of(e.P32) + bytes_of(e.Xp);
+    t += bytes_of(e.cpan[0]) + bytes_of(e.cpan[1]);
+    for (const Combine* c : {&e.comb_stored, &e.comb_fly, &e.comb_mx})
+        t += bytes_of(c->ptr) + bytes_of(c->idx) + bytes_of(c->groups);
+    return t;
+}
+
+}  // namespace
+}  // extern "C++"
+
+hmg_status hmg_memory(const struct h_matrix_data* data, int64_t out[2])
+{
+    Engine* e = engine_of(data);
+    if (!e || !out) {
+        set_error("h_matrix_data not initialised");
+        return HMG_ERR_NOT_READY;
+    }
+    // buffers of init_h_matrix_data: coordinates, ids, extents, codes, orders
+    int64_t base = 0;
+    for (int i = 0; i < 2; i++)
+        base += (int64_t)data->point_count[i] * (8 * data->dim + 4 + 8 + 8) + 16 * data->dim + 256;
+    out[0] = base + engine_bytes(*e);
+    out[1] = 0;
+    if (!e->built) return HMG_OK;
+    const int na = e->naca;
+    const bool have_factors = e->aca_k >= 0 && data->U != nullptr;
+    const bool mixed = e->precision == HMG_PREC_MIXED;
+    int64_t add = 0;
+    if (!have_factors && na > 0) {
+        // factor elements of the current layout; the mixed layout pads to 32 floats
+        // instead of 16 doubles
+        const double elems = (double)(e->h_uoff[na] + e->h_voff[na]);
+        add += (int64_t)(elems * (mixed ? 4.0 * 1.03 : 8.0));
+    }
+    if (mixed && e->layout_prec != HMG_PREC_MIXED) {
+        const int64_t n1 = e->n[0], n2 = e->n[1];
+        const int64_t g1 = (n1 + 7) / 8 + 2, g2 = (n2 + 7) / 8 + 2;
+        // near-field items: 64-row slices x 4096-column chunks per dense row cluster
+        int64_t mx_items = 0;
+        const int ng = (int)e->h_dgrp_ptr.size() - 1;
+        for (int g = 0; g < ng; g++) {
+            int64_t cols = 0;
+            int m1 = 0;
+            for (int q = e->h_dgrp_ptr[g]; q < e->h_dgrp_ptr[g + 1]; q++) {
+                const int4 b = e->h_blk[e->h_dgrp_blk[q]];
+                cols += b.w - b.z + 1;
+                m1 = b.y - b.x + 1;
+            }
+            mx_items += (int64_t)((m1 + 63) / 64) * ((cols + 4095) / 4096 + 1);
+        }
+        // partial V^T X slots of column clusters longer than one 4096-point chunk
+        int64_t ntp = 0;
+        const int ncg = (int)e->h_cgrp_ptr.size() - 1;
+        for (int g = 0; g < ncg; g++) {
+            int K = 0;
+            for (int q = e->h_cgrp_ptr[g]; q < e->h_cgrp_ptr[g + 1]; q++) K += e->h_kb[e->h_cgrp_blk[q]];
+            const int4 b = e->h_blk[e->ndense + e->h_cgrp_blk[e->h_cgrp_ptr[g]]];
+            const int64_t nch = (b.w - b.z + 1 + 4095) / 4096;
+            if (nch > 1) ntp += nch * ((K + 127) / 128);
+        }
+        const int64_t slots = (int64_t)e->n_lr_items * 64 + mx_items * 64;
+        add += g2 * 8 * kMxRhs * 4 + g2 * 2 * kMxRhs * 8 * 2;     // Xs32, Xp
+        add += e->tsize * kMxRhs * 4 + ntp * kMxVCols * kMxRhs * 4;  // T32, Tp32
+        add += slots * kMxRhs * 4 + slots * 4 + n1 * 4;              // P32, combine list
+        add += (g1 + g2) * e->dim * 8 * 4 + mx_items * (16 + 16);    // panels, items
+    }
+    out[1] = add + add / 10 + (64 << 20);
     return HMG_OK;
 }
 
@@ -906,6 +1018,27 @@ int64_t hmg_struct_layout(int which, int field)
         return field < 0 ? (int64_t)sizeof(system_assembler) : -1;
     case 5:
         return field < 0 ? (int64_t)sizeof(mat_vec_data_info) : -1;
+    case 6:
+        return field < 0 ? (int64_t)sizeof(hmg_server_config) : -1;
+    case 7: {
+        const int64_t offs[] = {OFF(hmg_build_request, dim), OFF(hmg_build_request, bits),
+                                OFF(hmg_build_request, n), OFF(hmg_build_request, coords),
+                                OFF(hmg_build_request, eta), OFF(hmg_build_request, max_level),
+                                OFF(hmg_build_request, c_leaf), OFF(hmg_build_request, k),
+                                OFF(hmg_build_request, kernel_kind), OFF(hmg_build_request, regularization),
+                                OFF(hmg_build_request, length_scale)};
+        if (field < 0) return (int64_t)sizeof(hmg_build_request);
+        if (field < (int)(sizeof(offs) / sizeof(offs[0]))) return offs[field];
+        return -1;
+    }
+    case 8: {
+        const int64_t offs[] = {OFF(hmg_request_info, status), OFF(hmg_request_info, t_submit),
+                                OFF(hmg_request_info, t_start), OFF(hmg_request_info, t_end),
+                                OFF(hmg_request_info, merged), OFF(hmg_request_info, matrix)};
+        if (field < 0) return (int64_t)sizeof(hmg_request_info);
+        if (field < (int)(sizeof(offs) / sizeof(offs[0]))) return offs[field];
+        return -1;
+    }
     }
     return -1;
 #undef OFF

@@ -114,15 +114,20 @@ struct LrItem {
 
 // V^T x work item: columns [0, ncols) of a 64-column slice of a column cluster's
 // transposed V (leading dim ld), rows (points) j0 .. j0+nrows-1, V + vbase at row j0
+// tcol0: the slice's first column in the tmap order; tp: partial-sum slot of this
+// chunk, or -1 when the slice has a single chunk and the item writes T itself
+// (mixed precision; the FP64 kernel always writes partials at its item index)
 struct LvItem {
     int j0;
     int nrows;
     int ld;
     int ncols;
     int64_t vbase;
+    int tcol0;
+    int tp;
 };
-// reduction of the row chunks of one 64-column slice: items item0 .. item0+nchunks-1,
-// slice columns map to T through tmap[tcol0 + l]
+// reduction of the row chunks of one column slice: partial slots
+// item0 .. item0+nchunks-1, slice columns map to T through tmap[tcol0 + l]
 struct LvSlice {
     int item0;
     int nchunks;
@@ -178,8 +183,10 @@ struct MxItem {
 // Number of right-hand sides one mixed-precision pass handles (the N of the
 // MFMA tiles); more right-hand sides run in several passes.
 constexpr int kMxRhs = 32;
-// Columns of one mixed-precision V^T X work item (four 32-column MFMA tiles).
+// Columns of one mixed-precision V^T X work item and rows of one mixed-precision
+// U T work item (up to four interleaved 32-wide MFMA tiles).
 constexpr int kMxVCols = 128;
+constexpr int kMxLrRows = 128;
 
 // per-point partial lists of one MVP schedule (CSR over sorted rows): an entry
 // v >= 0 is the P slot v, an entry v < 0 the column-slot group -(v+1)
@@ -241,6 +248,12 @@ struct Engine {
     // row sums its partials in a fixed order (CSR) and writes y once.  P layout:
     // [low-rank items x 64][near items x 128][symmetric column slots].
     int n_near_items = 0, n_lr_items = 0, n_lv_items = 0, n_lv_slices = 0;
+    int n_tp = 0;                               // partial-sum slots of the V^T x items
+    // mixed-precision layout: V^T x and U t items grouped by the number of
+    // interleaved 32-wide MFMA tiles they use (offsets of the 4-, 2- and 1-tile
+    // groups and the end)
+    int lv_nt_off[4] = {0, 0, 0, 0};
+    int lr_nt_off[4] = {0, 0, 0, 0};
     DevBuf<int4> near_items;                    // stored near field: {row0, nrows, b0, b1}
     DevBuf<int> near_blk;
     std::vector<int4> h_near_items;

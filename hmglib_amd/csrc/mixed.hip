@@ -208,60 +208,139 @@ near_mx_kernel(const float* __restrict__ cp1, const float* __restrict__ cp2, Ker
 }
 
 // ---- low rank ----------------------------------------------------------------------------
-// Tp[(q*128 + col)*32 + c] = sum over the item's points j of V[j][col] X[j][c]
+// Both low-rank products are skinny FP32 GEMMs whose A operand streams from HBM:
+//   V^T X: A[col][point] = V[point][col]   (V row-major: a point's columns contiguous)
+//   U T:   A[row][col]   = U[col][row]     (U column-major: a column's rows contiguous)
+// Lane (c, h) of a wavefront loads NT consecutive A values of the contiguous
+// direction in one vector load: for k-step k, V[j+h][NT*c .. NT*c+NT-1] or
+// U[k+h][NT*c .. NT*c+NT-1].  Element e of that load is the A operand of MFMA
+// tile e, so tile e covers columns (rows) NT*m + e, m < 32, and one 32x32x2 FP32
+// MFMA per tile and k-step consumes the whole load.  With NT = 4 a load
+// instruction reads two contiguous 512-byte segments.
+template <int NT>
+struct VecT;
+template <>
+struct VecT<1> {
+    typedef float type;
+};
+template <>
+struct VecT<2> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct VecT<4> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+
+template <int NT>
+__device__ __forceinline__ float vget(const typename VecT<NT>::type& v, int e)
+{
+    if constexpr (NT == 1)
+        return v;
+    else
+        return v[e];
+}
+
+// operands of 4 k-steps: 4 B values and 4 NT-wide A vectors
+template <int NT>
+struct Batch {
+    float b[4];
+    typename VecT<NT>::type a[4];
+};
+
+// a: this lane's A stream (advances by lda per k), b: its B stream (32 floats
+// per k); k-steps k0, k0+2, k0+4, k0+6
+template <int NT>
+__device__ __forceinline__ void lr_load(Batch<NT>& st, const float* __restrict__ a, int64_t lda,
+                                        const float* __restrict__ b, int k0)
+{
+    typedef typename VecT<NT>::type V;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        st.b[s] = b[(int64_t)(k0 + 2 * s) * kMxRhs];
+        st.a[s] = *(const V*)(a + (int64_t)(k0 + 2 * s) * lda);
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void lr_mfma(f32x16 (&acc)[NT], const Batch<NT>& st)
+{
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int e = 0; e < NT; e++)
+            acc[e] = __builtin_amdgcn_mfma_f32_32x32x2f32(vget<NT>(st.a[s], e), st.b[s], acc[e], 0, 0, 0);
+}
+
+// acc[e] += sum over k < K of A-stream x B-stream; the next batch of operands is
+// loaded while the current batch's MFMAs run (two register sets).  The next batch
+// is always loaded (on the last step from a clamped, valid position) so wait
+// counts stay path-independent; sched_barrier keeps the loads ahead of the MFMAs.
+template <int NT>
+__device__ __forceinline__ void lr_body(f32x16 (&acc)[NT], const float* __restrict__ a, int64_t lda,
+                                        const float* __restrict__ b, int K, int h)
+{
+    typedef typename VecT<NT>::type V;
+    const int kfull = K & ~7;
+    Batch<NT> A, B;
+    if (kfull > 0) lr_load<NT>(A, a, lda, b, 0);
+    for (int k = 0; k < kfull; k += 16) {
+        lr_load<NT>(B, a, lda, b, min(k + 8, kfull - 8));
+        __builtin_amdgcn_sched_barrier(0);
+        lr_mfma<NT>(acc, A);
+        __builtin_amdgcn_sched_barrier(0);
+        if (k + 8 >= kfull) break;
+        lr_load<NT>(A, a, lda, b, min(k + 16, kfull - 8));
+        __builtin_amdgcn_sched_barrier(0);
+        lr_mfma<NT>(acc, B);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int k = kfull; k < K; k += 2) {
+        const bool ok = k + h < K;
+        const float bv = ok ? b[(int64_t)k * kMxRhs] : 0.f;
+        V av;
+        if (ok)
+            av = *(const V*)(a + (int64_t)k * lda);
+        else
+            av = V{};
+#pragma unroll
+        for (int e = 0; e < NT; e++)
+            acc[e] = __builtin_amdgcn_mfma_f32_32x32x2f32(vget<NT>(av, e), bv, acc[e], 0, 0, 0);
+    }
+}
+
+// V^T X for the item's column slice (ncols <= 32*NT) over its points:
+//   tp < 0:  T[tmap[tcol0 + col]*32 + c] = result (single-chunk slice)
+//   tp >= 0: Tp[(tp*128 + col)*32 + c] = partial for this chunk of points
+template <int NT>
 __global__ void __launch_bounds__(kThreads)
 lr_v32_kernel(const LvItem* __restrict__ items, int nitems, const float* __restrict__ V,
-              const float* __restrict__ Xs, float* __restrict__ Tp)
+              const float* __restrict__ Xs, const int* __restrict__ tmap, float* __restrict__ T,
+              float* __restrict__ Tp)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int q = blockIdx.x * kWaves + wave_id();
     if (q >= nitems) return;
     const LvItem it = items[q];
     const int c32 = lane & 31, h = lane >> 5;
-    const int ntiles = (it.ncols + 31) >> 5;
     const int64_t ld = it.ld;
-    const float* __restrict__ v = V + it.vbase + h * ld + c32;
-    const float* __restrict__ x = Xs + (int64_t)(it.j0 + h) * kMxRhs + c32;
-    f32x16 acc[4];
+    f32x16 acc[NT];
 #pragma unroll
-    for (int t = 0; t < 4; t++)
+    for (int e = 0; e < NT; e++)
 #pragma unroll
-        for (int r = 0; r < 16; r++) acc[t][r] = 0.f;
-    const int n = it.nrows;
-    int j = 0;
-    for (; j + 8 <= n; j += 8) {
-        float b[4], a[4][4];
+        for (int r = 0; r < 16; r++) acc[e][r] = 0.f;
+    lr_body<NT>(acc, V + it.vbase + h * ld + NT * c32, ld, Xs + (int64_t)(it.j0 + h) * kMxRhs + c32,
+                it.nrows, h);
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            b[s] = x[(int64_t)(j + 2 * s) * kMxRhs];
+    for (int e = 0; e < NT; e++) {
 #pragma unroll
-            for (int t = 0; t < 4; t++) a[s][t] = t < ntiles ? v[(int64_t)(j + 2 * s) * ld + t * 32] : 0.f;
-        }
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-                if (t < ntiles) acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s][t], b[s], acc[t], 0, 0, 0);
-    }
-    for (; j < n; j += 2) {
-        const bool ok = j + h < n;
-        const float b = ok ? x[(int64_t)j * kMxRhs] : 0.f;
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            if (t < ntiles) {
-                const float a = ok ? v[(int64_t)j * ld + t * 32] : 0.f;
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[t], 0, 0, 0);
-            }
-        }
-    }
-    float* __restrict__ out = Tp + (int64_t)q * kMxVCols * kMxRhs + c32;
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        if (t < ntiles) {
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int col = t * 32 + (r >> 2) * 8 + h * 4 + (r & 3);
-                out[(int64_t)col * kMxRhs] = acc[t][r];
+        for (int r = 0; r < 16; r++) {
+            const int col = NT * ((r >> 2) * 8 + h * 4 + (r & 3)) + e;
+            if (col < it.ncols) {
+                if (it.tp < 0)
+                    T[(int64_t)tmap[it.tcol0 + col] * kMxRhs + c32] = acc[e][r];
+                else
+                    Tp[((int64_t)it.tp * kMxVCols + col) * kMxRhs + c32] = acc[e][r];
             }
         }
     }
@@ -282,7 +361,8 @@ lr_v32_finalize_kernel(const LvSlice* __restrict__ slices, const float* __restri
     }
 }
 
-// P[(q*64 + r)*32 + c] = sum_k U_C[row0 + r][k] T_C[k][c]
+// U T: P[(q*128 + r)*32 + c] = sum_k U_C[row0 + r][k] T_C[k][c], r < nrows <= 32*NT
+template <int NT>
 __global__ void __launch_bounds__(kThreads)
 lr_u32_kernel(const LrItem* __restrict__ items, int nitems, const float* __restrict__ U,
               const float* __restrict__ T, float* __restrict__ P)
@@ -292,45 +372,21 @@ lr_u32_kernel(const LrItem* __restrict__ items, int nitems, const float* __restr
     if (q >= nitems) return;
     const LrItem it = items[q];
     const int c32 = lane & 31, h = lane >> 5;
-    const bool two = it.nrows > 32;
     const int64_t ld = it.ld;
-    const float* __restrict__ u = U + it.uoff + h * ld + c32;
-    const float* __restrict__ t = T + (it.toff + h) * kMxRhs + c32;
-    f32x16 acc0, acc1;
+    f32x16 acc[NT];
 #pragma unroll
-    for (int r = 0; r < 16; r++) acc0[r] = acc1[r] = 0.f;
-    const int K = it.K;
-    int k = 0;
-    for (; k + 8 <= K; k += 8) {
-        float b[4], a0[4], a1[4];
+    for (int e = 0; e < NT; e++)
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            b[s] = t[(int64_t)(k + 2 * s) * kMxRhs];
-            a0[s] = u[(int64_t)(k + 2 * s) * ld];
-            a1[s] = two ? u[(int64_t)(k + 2 * s) * ld + 32] : 0.f;
+        for (int r = 0; r < 16; r++) acc[e][r] = 0.f;
+    lr_body<NT>(acc, U + it.uoff + h * ld + NT * c32, ld, T + (it.toff + h) * kMxRhs + c32, it.K, h);
+    float* __restrict__ out = P + (int64_t)q * kMxLrRows * kMxRhs + c32;
+#pragma unroll
+    for (int e = 0; e < NT; e++) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int rr = NT * ((r >> 2) * 8 + h * 4 + (r & 3)) + e;
+            if (rr < it.nrows) out[(int64_t)rr * kMxRhs] = acc[e][r];
         }
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[s], b[s], acc0, 0, 0, 0);
-            if (two) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[s], b[s], acc1, 0, 0, 0);
-        }
-    }
-    for (; k < K; k += 2) {
-        const bool ok = k + h < K;
-        const float b = ok ? t[(int64_t)k * kMxRhs] : 0.f;
-        const float a0 = ok ? u[(int64_t)k * ld] : 0.f;
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b, acc0, 0, 0, 0);
-        if (two) {
-            const float a1 = ok ? u[(int64_t)k * ld + 32] : 0.f;
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b, acc1, 0, 0, 0);
-        }
-    }
-    float* __restrict__ out = P + (int64_t)q * kMxRowsItem * kMxRhs + c32;
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int rr = (r >> 2) * 8 + h * 4 + (r & 3);
-        if (rr < it.nrows) out[(int64_t)rr * kMxRhs] = acc0[r];
-        if (32 + rr < it.nrows) out[(int64_t)(32 + rr) * kMxRhs] = acc1[r];
     }
 }
 
@@ -398,12 +454,12 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
         eng.cpan_ready = true;
     }
     const int n2p = (int)(g2 * kGroup);
-    const int64_t near0 = (int64_t)eng.n_lr_items * kMxRowsItem;
+    const int64_t near0 = (int64_t)eng.n_lr_items * kMxLrRows;
     const int64_t slots = near0 + (int64_t)eng.n_mx_items * kMxRowsItem;
     HMG_TRY(eng.Xs32.alloc((size_t)n2p * kMxRhs));
     HMG_TRY(eng.Xp.alloc((size_t)g2 * 2 * kMxRhs * kGroup));
     HMG_TRY(eng.T32.alloc((size_t)std::max<int64_t>(eng.tsize, 1) * kMxRhs));
-    HMG_TRY(eng.Tp32.alloc((size_t)std::max(eng.n_lv_items, 1) * kMxVCols * kMxRhs));
+    HMG_TRY(eng.Tp32.alloc((size_t)std::max(eng.n_tp, 1) * kMxVCols * kMxRhs));
     HMG_TRY(eng.P32.alloc((size_t)std::max<int64_t>(slots, 1) * kMxRhs));
     const KernelParamsF kpf = make_params_f(kp);
     const bool reg = kp.reg != 0.0;
@@ -442,17 +498,42 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
         HMG_HIP(hipEventRecord(eng.ev_join, s2));
         if (eng.naca > 0 && eng.n_lr_items > 0) {
             eng.prof.begin(PROF_LR_VTX, s);
-            lr_v32_kernel<<<grid_for(eng.n_lv_items, kWaves), kThreads, 0, s>>>(eng.lv_items.p, eng.n_lv_items,
-                                                                                  V, eng.Xs32.p, eng.Tp32.p);
-            HMG_HIP(hipGetLastError());
-            lr_v32_finalize_kernel<<<eng.n_lv_slices, kThreads, 0, s>>>(eng.lv_slices.p, eng.Tp32.p,
-                                                                        eng.tmap.p, eng.T32.p);
-            HMG_HIP(hipGetLastError());
+            // items grouped by interleaved tile count: [4 | 2 | 1]
+            for (int g = 0; g < 3; g++) {
+                const int b = eng.lv_nt_off[g], e = eng.lv_nt_off[g + 1];
+                if (e <= b) continue;
+                const LvItem* li = eng.lv_items.p + b;
+                const unsigned gr = grid_for(e - b, kWaves);
+                if (g == 0)
+                    lr_v32_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xs32.p, eng.tmap.p, eng.T32.p, eng.Tp32.p);
+                else if (g == 1)
+                    lr_v32_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xs32.p, eng.tmap.p, eng.T32.p, eng.Tp32.p);
+                else
+                    lr_v32_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xs32.p, eng.tmap.p, eng.T32.p, eng.Tp32.p);
+                HMG_HIP(hipGetLastError());
+            }
+            if (eng.n_lv_slices > 0) {
+                lr_v32_finalize_kernel<<<eng.n_lv_slices, kThreads, 0, s>>>(eng.lv_slices.p, eng.Tp32.p,
+                                                                            eng.tmap.p, eng.T32.p);
+                HMG_HIP(hipGetLastError());
+            }
             eng.prof.end(PROF_LR_VTX, s);
             eng.prof.begin(PROF_LR_U, s);
-            lr_u32_kernel<<<grid_for(eng.n_lr_items, kWaves), kThreads, 0, s>>>(eng.lr_items.p, eng.n_lr_items,
-                                                                                  U, eng.T32.p, eng.P32.p);
-            HMG_HIP(hipGetLastError());
+            // items grouped by interleaved tile count: [4 | 2 | 1]
+            for (int g = 0; g < 3; g++) {
+                const int b = eng.lr_nt_off[g], e = eng.lr_nt_off[g + 1];
+                if (e <= b) continue;
+                const LrItem* li = eng.lr_items.p + b;
+                float* Pg = eng.P32.p + (int64_t)b * kMxLrRows * kMxRhs;
+                const unsigned gr = grid_for(e - b, kWaves);
+                if (g == 0)
+                    lr_u32_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.T32.p, Pg);
+                else if (g == 1)
+                    lr_u32_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.T32.p, Pg);
+                else
+                    lr_u32_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.T32.p, Pg);
+                HMG_HIP(hipGetLastError());
+            }
             eng.prof.end(PROF_LR_U, s);
         }
         HMG_HIP(hipStreamWaitEvent(s, eng.ev_join, 0));

@@ -29,7 +29,8 @@ namespace {
 constexpr int kSlice = kTileRows;          // rows per low-rank work item (one wavefront)
 constexpr int kNearSlice = 2 * kTileRows;  // rows per near-field work item (two per lane)
 constexpr int kNearChunkCols = 4096;       // near-field columns per work item
-constexpr int kVChunkRows = 1024;          // points per V^T x work item
+constexpr int kVChunkRows = 1024;          // points per V^T x work item (FP64)
+constexpr int kMxVChunkRows = 4096;        // points per V^T x work item (mixed precision)
 constexpr int kMxRows = 64;                // rows per mixed-precision near-field item
 constexpr int kMxChunkCols = 4096;         // columns per mixed-precision near-field item
 
@@ -110,7 +111,7 @@ std::vector<int> column_chunks(const std::vector<int>& m2s)
 // the combine expands, so neighbouring points read neighbouring slots.
 hmg_status build_combine(Combine& cb, int n1, const std::vector<int4>& near_rows, int64_t near0,
                          const std::vector<LrItem>& lr, const std::vector<ColSlots>& cols,
-                         hipStream_t s, int near_slice = kNearSlice)
+                         hipStream_t s, int near_slice = kNearSlice, int lr_slice = kSlice)
 {
     std::vector<int> ptr((size_t)n1 + 1, 0);
     for (const int4& it : near_rows)
@@ -128,7 +129,7 @@ hmg_status build_combine(Combine& cb, int n1, const std::vector<int4>& near_rows
     }
     for (size_t q = 0; q < lr.size(); q++) {
         const LrItem& it = lr[q];
-        for (int r = 0; r < it.nrows; r++) idx[fill[it.row0 + r]++] = (int)(q * kSlice + r);
+        for (int r = 0; r < it.nrows; r++) idx[fill[it.row0 + r]++] = (int)(q * lr_slice + r);
     }
     for (size_t g = 0; g < cols.size(); g++)
         for (int j = 0; j < cols[g].m2; j++) idx[fill[cols[g].l2 + j]++] = -(int)g - 1;
@@ -335,6 +336,8 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
     // columns per V^T x item: one wavefront of columns (FP64), four 32-column MFMA
     // tiles (mixed)
     const int vslice = mixed ? kMxVCols : kSlice;
+    const int vchunk = mixed ? kMxVChunkRows : kVChunkRows;
+    int ntp = 0;
     eng.h_kb.assign(na, 0);
     eng.kmax = 0;
     for (int a = 0; a < na; a++) {
@@ -352,6 +355,7 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
     eng.h_toff.assign(na + 1, 0);
     eng.h_ldu.assign(std::max(na, 1), 0);
     std::vector<LrItem> items;
+    const int lrows = mixed ? kMxLrRows : kSlice;
     int64_t urun = 0, trun = 0;
     const int ng = (int)eng.h_agrp_ptr.size() - 1;
     for (int g = 0; g < ng; g++) {
@@ -371,13 +375,30 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
         }
         urun += (int64_t)ldu * K;
         if (K == 0) continue;
-        for (int r0 = 0; r0 < m1; r0 += kSlice)
-            items.push_back(LrItem{l1 + r0, std::min(kSlice, m1 - r0), ldu, K, uC + r0, tC});
+        for (int r0 = 0; r0 < m1; r0 += lrows)
+            items.push_back(LrItem{l1 + r0, std::min(lrows, m1 - r0), ldu, K, uC + r0, tC});
     }
     eng.h_uoff[na] = urun;
     eng.h_toff[na] = trun;
     eng.tsize = trun;
     eng.n_lr_items = (int)items.size();
+    // interleaved 32-wide MFMA tiles an item of n rows or columns uses
+    auto tiles_of = [](int n) { return n <= 32 ? 1 : n <= 64 ? 2 : 4; };
+    for (int g = 0; g < 4; g++) eng.lr_nt_off[g] = eng.lv_nt_off[g] = 0;
+    if (mixed) {
+        // one kernel instance per tile count: items grouped 4, 2, 1 (each item
+        // carries its own rows and output slots)
+        std::vector<LrItem> grouped;
+        grouped.reserve(items.size());
+        int g = 0;
+        for (int nt : {4, 2, 1}) {
+            eng.lr_nt_off[g++] = (int)grouped.size();
+            for (const LrItem& it : items)
+                if (tiles_of(it.nrows) == nt) grouped.push_back(it);
+        }
+        eng.lr_nt_off[3] = (int)grouped.size();
+        items.swap(grouped);
+    }
 
     // V in column-cluster order, each cluster's V row-major (m2 x K, ld = K padded);
     // V^T x items cover 64 columns x up to kVChunkRows points, slices reduce their
@@ -407,44 +428,68 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
             pos += eng.h_kb[a];
         }
         vrun += (int64_t)m2 * ld;
+        const int nch = (m2 + vchunk - 1) / vchunk;
         for (int s0 = 0; s0 < K; s0 += vslice) {
             const int nc = std::min(vslice, K - s0);
-            const int item0 = (int)vitems.size();
-            int nch = 0;
-            for (int j0 = 0; j0 < m2; j0 += kVChunkRows, nch++)
-                vitems.push_back(LvItem{l2 + j0, std::min(kVChunkRows, m2 - j0), ld, nc,
-                                        vD + s0 + (int64_t)j0 * ld});
-            slices.push_back(LvSlice{item0, nch, nc, (int)(col + s0)});
+            // mixed precision: a single-chunk slice writes T directly
+            const bool direct = mixed && nch == 1;
+            const int tp0 = direct ? -1 : (mixed ? ntp : (int)vitems.size());
+            for (int c = 0; c < nch; c++) {
+                const int j0 = c * vchunk;
+                vitems.push_back(LvItem{l2 + j0, std::min(vchunk, m2 - j0), ld, nc,
+                                        vD + s0 + (int64_t)j0 * ld, (int)(col + s0),
+                                        direct ? -1 : tp0 + c});
+            }
+            if (!direct) {
+                slices.push_back(LvSlice{tp0, nch, nc, (int)(col + s0)});
+                if (mixed) ntp += nch;
+            }
         }
         col += K;
     }
     eng.h_voff[na] = vrun;
+    if (mixed) {
+        std::vector<LvItem> grouped;
+        grouped.reserve(vitems.size());
+        int g = 0;
+        for (int nt : {4, 2, 1}) {
+            eng.lv_nt_off[g++] = (int)grouped.size();
+            for (const LvItem& it : vitems)
+                if (tiles_of(it.ncols) == nt) grouped.push_back(it);
+        }
+        eng.lv_nt_off[3] = (int)grouped.size();
+        vitems.swap(grouped);
+    }
     eng.n_lv_items = (int)vitems.size();
     eng.n_lv_slices = (int)slices.size();
+    eng.n_tp = mixed ? ntp : eng.n_lv_items;
 
-    // P = [low-rank items x 64][near items x 128][symmetric column slots]
-    const int64_t near0 = (int64_t)eng.n_lr_items * kSlice;
-    const int64_t col0 = near0 + (int64_t)std::max(eng.n_near_items, eng.n_fly_items) * kNearSlice;
-    if (col0 + eng.n_col_slots > (int64_t)INT32_MAX) {
-        set_error("MVP partial buffer exceeds 2^31 slots");
-        return HMG_ERR_INVALID;
-    }
+    // FP64: P = [low-rank items x 64][near items x 128][symmetric column slots];
+    // mixed precision: P32 = [low-rank items x 128][near items x 64], kMxRhs values
+    // per slot (allocated by the product)
+    const int64_t near0 = (int64_t)eng.n_lr_items * lrows;
     hipStream_t s = eng.stream;
     const int n1 = eng.n[0];
-    HMG_TRY(build_combine(eng.comb_stored, n1, eng.h_near_items, near0, items, {}, s));
-    std::vector<int4> fly_rows((size_t)eng.n_fly_items);
-    for (int q = 0; q < eng.n_fly_items; q++)
-        fly_rows[q] = make_int4(eng.h_fly_items[q].row0, eng.h_fly_items[q].nrows, 0, 0);
-    std::vector<ColSlots> cols(eng.h_col_groups);
-    for (ColSlots& c : cols) c.base += col0;
-    HMG_TRY(build_combine(eng.comb_fly, n1, fly_rows, near0, items, cols, s));
-    std::vector<NearRef> refs(eng.h_fly_refs);
-    for (NearRef& r : refs)
-        if (r.cbase >= 0) r.cbase += col0;
-    HMG_TRY(upload(eng.fly_items, eng.h_fly_items, s));
-    HMG_TRY(upload(eng.fly_refs, refs, s));
-    if (mixed) {
-        // P (mixed, kMxRhs values per slot) = [low-rank items x 64][near items x 64]
+    int64_t col0 = 0;
+    if (!mixed) {
+        col0 = near0 + (int64_t)std::max(eng.n_near_items, eng.n_fly_items) * kNearSlice;
+        if (col0 + eng.n_col_slots > (int64_t)INT32_MAX) {
+            set_error("MVP partial buffer exceeds 2^31 slots");
+            return HMG_ERR_INVALID;
+        }
+        HMG_TRY(build_combine(eng.comb_stored, n1, eng.h_near_items, near0, items, {}, s));
+        std::vector<int4> fly_rows((size_t)eng.n_fly_items);
+        for (int q = 0; q < eng.n_fly_items; q++)
+            fly_rows[q] = make_int4(eng.h_fly_items[q].row0, eng.h_fly_items[q].nrows, 0, 0);
+        std::vector<ColSlots> cols(eng.h_col_groups);
+        for (ColSlots& c : cols) c.base += col0;
+        HMG_TRY(build_combine(eng.comb_fly, n1, fly_rows, near0, items, cols, s));
+        std::vector<NearRef> refs(eng.h_fly_refs);
+        for (NearRef& r : refs)
+            if (r.cbase >= 0) r.cbase += col0;
+        HMG_TRY(upload(eng.fly_items, eng.h_fly_items, s));
+        HMG_TRY(upload(eng.fly_refs, refs, s));
+    } else {
         if (!eng.mx_sched) {
             build_mixed_near(eng);
             HMG_TRY(upload(eng.mx_items, eng.h_mx_items, s));
@@ -459,7 +504,7 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
             set_error("mixed-precision partial buffer too large");
             return HMG_ERR_INVALID;
         }
-        HMG_TRY(build_combine(eng.comb_mx, n1, mx_rows, near0, items, {}, s, kMxRows));
+        HMG_TRY(build_combine(eng.comb_mx, n1, mx_rows, near0, items, {}, s, kMxRows, lrows));
     }
 
     HMG_TRY(eng.kb.alloc(std::max(na, 1)));
@@ -477,9 +522,11 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
     HMG_TRY(upload(eng.lv_items, vitems, s));
     HMG_TRY(upload(eng.lv_slices, slices, s));
     HMG_TRY(upload(eng.tmap, tmap, s));
-    HMG_TRY(eng.Tp.alloc((size_t)std::max(eng.n_lv_items, 1) * kSlice));
-    HMG_TRY(eng.P.alloc((size_t)(col0 + eng.n_col_slots)));
-    HMG_TRY(eng.T.alloc(std::max<int64_t>(eng.tsize, 1)));
+    if (!mixed) {
+        HMG_TRY(eng.Tp.alloc((size_t)std::max(eng.n_lv_items, 1) * kSlice));
+        HMG_TRY(eng.P.alloc((size_t)(col0 + eng.n_col_slots)));
+        HMG_TRY(eng.T.alloc(std::max<int64_t>(eng.tsize, 1)));
+    }
     HMG_HIP(hipStreamSynchronize(s));
     eng.aca_k = k;
     eng.layout_prec = prec;

@@ -453,7 +453,9 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
     eng.naca = nout - ndense;
     eng.h_blk.resize(nout);
     if (nout > 0)
-        HMG_HIP(hipMemcpy(eng.h_blk.data(), eng.blk.p, sizeof(int4) * nout, hipMemcpyDeviceToHost));
+        HMG_HIP(hipMemcpyAsync(eng.h_blk.data(), eng.blk.p, sizeof(int4) * nout, hipMemcpyDeviceToHost,
+                               eng.stream));
+        HMG_HIP(hipStreamSynchronize(eng.stream));
     for (DevBuf<int>* b : {&flags, &rank, &ub, &keys, &S, &runid, &Lc, &Uc, &rstart, &rend, &scal,
                            &nch, &nmv, &choff, &mvoff, &adm})
         b->release();

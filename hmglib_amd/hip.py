@@ -44,6 +44,7 @@ def _load():
         _hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
                                              ctypes.c_void_p]
         _hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        _hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
     return _hip
 
 
@@ -105,6 +106,7 @@ class DeviceArray:
         assert a.nbytes <= self.nbytes
         _chk(_load().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, hipMemcpyHostToDevice),
              "hipMemcpy H2D")
+        _null_stream_sync()
 
     def download(self):
         out = np.empty(self.shape, dtype=self.dtype, order="F")
@@ -114,6 +116,7 @@ class DeviceArray:
 
     def zero(self):
         _chk(_load().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
+        _null_stream_sync()
 
     def free(self):
         if self.ptr:
@@ -127,35 +130,19 @@ class DeviceArray:
             pass
 
 
+def _null_stream_sync():
+    # The engine runs on non-blocking streams, which are not ordered after
+    # null-stream copies: finish the copy before the engine may read the data.
+    _chk(_load().hipStreamSynchronize(None), "hipStreamSynchronize")
+
+
 def memcpy_htod(dptr, arr):
     a = np.ascontiguousarray(arr)
     _chk(_load().hipMemcpy(dptr, a.ctypes.data, a.nbytes, hipMemcpyHostToDevice), "hipMemcpy H2D")
+    _null_stream_sync()
 
 
 def memcpy_dtoh(arr, dptr):
     assert arr.flags.c_contiguous or arr.flags.f_contiguous
     _chk(_load().hipMemcpy(arr.ctypes.data, dptr, arr.nbytes, hipMemcpyDeviceToHost), "hipMemcpy D2H")
     return arr
-
-
-class Timer:
-    """hipEvent pair on the null stream (the engine's stream is blocking, so the
-    events bracket the engine's work)."""
-
-    def __init__(self):
-        h = _load()
-        self.a = ctypes.c_void_p()
-        self.b = ctypes.c_void_p()
-        _chk(h.hipEventCreate(ctypes.byref(self.a)), "hipEventCreate")
-        _chk(h.hipEventCreate(ctypes.byref(self.b)), "hipEventCreate")
-
-    def start(self):
-        _chk(_load().hipEventRecord(self.a, None), "hipEventRecord")
-
-    def stop(self):
-        h = _load()
-        _chk(h.hipEventRecord(self.b, None), "hipEventRecord")
-        _chk(h.hipEventSynchronize(self.b), "hipEventSynchronize")
-        ms = ctypes.c_float()
-        _chk(h.hipEventElapsedTime(ctypes.byref(ms), self.a, self.b), "hipEventElapsedTime")
-        return ms.value

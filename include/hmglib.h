@@ -181,7 +181,10 @@ hmg_status hmg_get_profile(const struct h_matrix_data* data, int kernel_class, d
 hmg_status hmg_copy_probe(const double* src, double* dst, int64_t n);
 /* last error message of this thread */
 const char* hmg_last_error(void);
-/* sizeof / field offsets of the public structs, for FFI layout checks */
+/* sizeof / field offsets of the public structs, for FFI layout checks: which =
+ * 0 h_matrix_data, 1 work_item, 2 point_set, 3 morton_code, 4 system_assembler,
+ * 5 mat_vec_data_info, 6 hmg_server_config, 7 hmg_build_request, 8
+ * hmg_request_info; field < 0 gives sizeof */
 int64_t hmg_struct_layout(int which, int field);
 /* seeded host generator shared by tests and benchmarks: SplitMix64 seeding
  * xoshiro256**; uniform = (next>>11)*2^-53; normal = Box-Muller. */

@@ -94,3 +94,48 @@ def test_python_front_end_raises_when_library_is_missing(monkeypatch, tmp_path):
     monkeypatch.setattr(_abi, "_lib", None)
     with pytest.raises(RuntimeError, match="not built"):
         _abi.lib()
+
+
+def test_server_struct_layouts_match():
+    L = hmglib_amd.lib()
+    assert L.hmg_struct_layout(6, -1) == ctypes.sizeof(_abi.ServerConfig)
+    assert L.hmg_struct_layout(7, -1) == ctypes.sizeof(_abi.BuildRequest)
+    for i, (name, _) in enumerate(_abi.BuildRequest._fields_):
+        assert getattr(_abi.BuildRequest, name).offset == L.hmg_struct_layout(7, i), name
+    assert L.hmg_struct_layout(8, -1) == ctypes.sizeof(_abi.RequestInfo)
+    for i, (name, _) in enumerate(_abi.RequestInfo._fields_):
+        assert getattr(_abi.RequestInfo, name).offset == L.hmg_struct_layout(8, i), name
+
+
+def test_server_host_logic_without_the_gpu():
+    """Creating, querying and destroying a server with no requests touches no GPU;
+    invalid configurations and requests are refused up front."""
+    L = hmglib_amd.lib()
+    h = ctypes.c_void_p()
+    bad = _abi.ServerConfig(0, 4, 32, _abi.PREC_MIXED)
+    assert L.hmg_server_create(ctypes.byref(bad), ctypes.byref(h)) == _abi.HMG_ERR_INVALID
+    bad = _abi.ServerConfig(1 << 30, 0, 32, _abi.PREC_MIXED)
+    assert L.hmg_server_create(ctypes.byref(bad), ctypes.byref(h)) == _abi.HMG_ERR_INVALID
+    bad = _abi.ServerConfig(1 << 30, 2, 32, 7)
+    assert L.hmg_server_create(ctypes.byref(bad), ctypes.byref(h)) == _abi.HMG_ERR_INVALID
+    cfg = _abi.ServerConfig(1 << 30, 3, 32, _abi.PREC_FP64)
+    assert L.hmg_server_create(ctypes.byref(cfg), ctypes.byref(h)) == _abi.HMG_OK
+    r = ctypes.c_int64()
+    dummy = ctypes.c_void_p(16)
+    # unknown H-matrix: refused at submit
+    assert L.hmg_server_submit_apply(h, 12345, dummy, dummy, 1, ctypes.byref(r)) == _abi.HMG_ERR_NOT_READY
+    assert L.hmg_server_submit_apply(h, 1, dummy, dummy, 0, ctypes.byref(r)) == _abi.HMG_ERR_INVALID
+    b = _abi.BuildRequest()
+    b.dim = 9
+    m = ctypes.c_int64()
+    assert L.hmg_server_submit_build(h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(m)) == \
+        _abi.HMG_ERR_INVALID
+    info = _abi.RequestInfo()
+    assert L.hmg_server_query(h, 999, ctypes.byref(info)) == _abi.HMG_ERR_INVALID
+    out = (ctypes.c_int64 * 8)()
+    assert L.hmg_server_stats(h, out) == _abi.HMG_OK
+    assert list(out) == [0] * 8
+    ids = (ctypes.c_int64 * 4)()
+    assert L.hmg_server_resident(h, ids, 4) == 0
+    assert L.hmg_server_drain(h) == _abi.HMG_OK
+    assert L.hmg_server_destroy(h) == _abi.HMG_OK
