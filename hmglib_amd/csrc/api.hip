@@ -911,6 +911,10 @@ hmg_status hmg_destroy_h_matrix_data(struct h_matrix_data* data)
         e->perm32[i].release();
     }
     e->prof.clear();
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipStreamSynchronize(e->stream2);
+    release_scratch(e->stream);
+    release_scratch(e->stream2);
     (void)hipEventDestroy(e->ev_fork);
     (void)hipEventDestroy(e->ev_join);
     (void)hipStreamDestroy(e->stream2);

@@ -316,6 +316,9 @@ struct Engine {
 Engine* engine_of(const h_matrix_data* d);
 
 // ---- primitives (prims.hip) ------------------------------------------------
+// The primitives keep grow-only scratch buffers per (stream, slot); an engine
+// releases the scratch of its streams when it is destroyed.
+void release_scratch(hipStream_t s);
 // exclusive or inclusive scan of int32, n elements, in -> out (may alias);
 // optional device pointer `total` receives the sum.
 hmg_status scan_i32(const int* in, int* out, int64_t n, bool inclusive, int* total,

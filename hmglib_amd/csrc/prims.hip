@@ -9,8 +9,9 @@
 //   * minmax_f64     : exact min/max of a double array (two-level reduction).
 // These replace the thrust calls of the reference (helper.cu:165 sort_by_key,
 // tree.cu:1495-1500 reduce/exclusive_scan, helper.cu:111 minmax_element).
-#include <unordered_map>
+#include <map>
 #include <mutex>
+#include <utility>
 
 #include "hmg_internal.h"
 
@@ -22,26 +23,28 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-// grow-only scratch buffers keyed by slot (primitives run on one stream at a time)
+// Grow-only scratch buffers keyed by (stream, slot).  Calls on one stream are
+// issued by one thread at a time and run in order, so a slot can be reused and
+// regrown between them (hipFree waits for the device).  Every H-matrix engine
+// owns its streams, so engines driven concurrently from different threads never
+// share a buffer; release_scratch drops a stream's buffers when its engine goes.
 struct ScratchCache {
     std::mutex mu;
-    std::unordered_map<int, DevBuf<unsigned char>> bufs;
-    ~ScratchCache() {
-        // device memory is reclaimed when the process ends; freeing here can run
-        // after the HIP runtime has shut down
-    }
+    std::map<std::pair<hipStream_t, int>, DevBuf<unsigned char>> bufs;
 };
 ScratchCache& scratch_cache()
 {
+    // never destroyed: freeing device memory at exit can run after the HIP
+    // runtime has shut down; the memory is reclaimed with the process
     static ScratchCache* c = new ScratchCache();
     return *c;
 }
 
-hmg_status scratch(int slot, size_t bytes, void** out)
+hmg_status scratch(hipStream_t s, int slot, size_t bytes, void** out)
 {
     ScratchCache& c = scratch_cache();
     std::lock_guard<std::mutex> g(c.mu);
-    DevBuf<unsigned char>& b = c.bufs[slot];
+    DevBuf<unsigned char>& b = c.bufs[std::make_pair(s, slot)];
     if (b.n < bytes) {
         b.release();
         HMG_TRY(b.alloc(bytes + bytes / 4));
@@ -130,7 +133,7 @@ hmg_status scan_rec(const int* in, int* out, int64_t n, bool inclusive, int* tot
     }
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
     void* p;
-    HMG_TRY(scratch(100 + depth, sizeof(int) * (size_t)nb, &p));
+    HMG_TRY(scratch(s, 100 + depth, sizeof(int) * (size_t)nb, &p));
     int* bsums = (int*)p;
     scan_tile_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, out, n, bsums, inclusive ? 1 : 0);
     HMG_HIP(hipGetLastError());
@@ -294,6 +297,20 @@ differ_kernel(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b, in
 
 }  // namespace
 
+void release_scratch(hipStream_t s)
+{
+    ScratchCache& c = scratch_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    for (auto it = c.bufs.begin(); it != c.bufs.end();) {
+        if (it->first.first == s) {
+            it->second.release();
+            it = c.bufs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
 hmg_status scan_i32(const int* in, int* out, int64_t n, bool inclusive, int* total, hipStream_t s)
 {
     return scan_rec(in, out, n, inclusive, total, s, 0);
@@ -305,9 +322,9 @@ hmg_status radix_sort_u64(uint64_t* keys, uint32_t* vals, int64_t n, int key_bit
     const int passes = (key_bits + 7) / 8;
     const int nblocks = (int)((n + kSortTile - 1) / kSortTile);
     void *pk, *pv, *pc;
-    HMG_TRY(scratch(1, sizeof(uint64_t) * (size_t)n, &pk));
-    HMG_TRY(scratch(2, sizeof(uint32_t) * (size_t)n, &pv));
-    HMG_TRY(scratch(3, sizeof(int) * (size_t)nblocks * 256, &pc));
+    HMG_TRY(scratch(s, 1, sizeof(uint64_t) * (size_t)n, &pk));
+    HMG_TRY(scratch(s, 2, sizeof(uint32_t) * (size_t)n, &pv));
+    HMG_TRY(scratch(s, 3, sizeof(int) * (size_t)nblocks * 256, &pc));
     uint64_t* k_alt = (uint64_t*)pk;
     uint32_t* v_alt = (uint32_t*)pv;
     int* counts = (int*)pc;
@@ -338,7 +355,7 @@ hmg_status minmax_f64(const double* x, int64_t n, double* dmin, double* dmax, hi
     if (n <= 0) return HMG_ERR_INVALID;
     int nb = (int)std::min<int64_t>(1024, (n + kRedThreads - 1) / kRedThreads);
     void* p;
-    HMG_TRY(scratch(4, sizeof(double) * 2 * (size_t)nb, &p));
+    HMG_TRY(scratch(s, 4, sizeof(double) * 2 * (size_t)nb, &p));
     double* pmin = (double*)p;
     double* pmax = pmin + nb;
     minmax_partial_kernel<<<nb, kRedThreads, 0, s>>>(x, n, pmin, pmax);
