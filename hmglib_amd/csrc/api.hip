@@ -221,6 +221,9 @@ void release_build(Engine& e, h_matrix_data* d)
     e.T32.release();
     e.Tp32.release();
     e.P32.release();
+    e.Ys32.release();
+    e.inv1.release();
+    e.inv1_ready = false;
     e.Xp.release();
     e.cpan[0].release();
     e.cpan[1].release();
@@ -740,6 +743,7 @@ int64_t engine_bytes(const Engine& e)
     t += bytes_of(e.lv_slices) + bytes_of(e.tmap) + bytes_of(e.Tp) + bytes_of(e.P) + bytes_of(e.xs);
     t += bytes_of(e.T) + bytes_of(e.U32) + bytes_of(e.V32) + bytes_of(e.mx_items) + bytes_of(e.mx_refs);
     t += bytes_of(e.Xs32) + bytes_of(e.T32) + bytes_of(e.Tp32) + bytes_of(e.P32) + bytes_of(e.Xp);
+    t += bytes_of(e.Ys32) + bytes_of(e.inv1);
     t += bytes_of(e.cpan[0]) + bytes_of(e.cpan[1]);
     for (const Combine* c : {&e.comb_stored, &e.comb_fly, &e.comb_mx})
         t += bytes_of(c->ptr) + bytes_of(c->idx) + bytes_of(c->groups);
@@ -776,7 +780,7 @@ hmg_status hmg_memory(const struct h_matrix_data* data, int64_t out[2])
     if (mixed && e->layout_prec != HMG_PREC_MIXED) {
         const int64_t n1 = e->n[0], n2 = e->n[1];
         const int64_t g1 = (n1 + 7) / 8 + 2, g2 = (n2 + 7) / 8 + 2;
-        // near-field items: 64-row slices x 4096-column chunks per dense row cluster
+        // near-field items: 128-row slices x 4096-column chunks per dense row cluster
         int64_t mx_items = 0;
         const int ng = (int)e->h_dgrp_ptr.size() - 1;
         for (int g = 0; g < ng; g++) {
@@ -787,7 +791,7 @@ hmg_status hmg_memory(const struct h_matrix_data* data, int64_t out[2])
                 cols += b.w - b.z + 1;
                 m1 = b.y - b.x + 1;
             }
-            mx_items += (int64_t)((m1 + 63) / 64) * ((cols + 4095) / 4096 + 1);
+            mx_items += (int64_t)((m1 + 127) / 128) * ((cols + 4095) / 4096 + 1);
         }
         // partial V^T X slots of column clusters longer than one 4096-point chunk
         int64_t ntp = 0;
@@ -799,10 +803,11 @@ hmg_status hmg_memory(const struct h_matrix_data* data, int64_t out[2])
             const int64_t nch = (b.w - b.z + 1 + 4095) / 4096;
             if (nch > 1) ntp += nch * ((K + 127) / 128);
         }
-        const int64_t slots = (int64_t)e->n_lr_items * 64 + mx_items * 64;
+        const int64_t slots = (int64_t)e->n_lr_items * 64 + mx_items * 128;
         add += g2 * 8 * kMxRhs * 4 + g2 * 2 * kMxRhs * 8 * 2;     // Xs32, Xp
         add += e->tsize * kMxRhs * 4 + ntp * kMxVCols * kMxRhs * 4;  // T32, Tp32
         add += slots * kMxRhs * 4 + slots * 4 + n1 * 4;              // P32, combine list
+        add += n1 * kMxRhs * 4 + n1 * 4;                             // Ys32, inv1
         add += (g1 + g2) * e->dim * 8 * 4 + mx_items * (16 + 16);    // panels, items
     }
     out[1] = add + add / 10 + (64 << 20);

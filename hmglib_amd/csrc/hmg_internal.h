@@ -167,7 +167,7 @@ struct ColSlots {
 // Mixed-precision near field (many right-hand sides).  Points are handled in
 // aligned groups of 8 (the 8 columns one lane feeds to a BF16 MFMA).  A ref is
 // a run of groups [g0, g1) (g1 - g0 even: two groups per 16-column step) whose
-// columns outside [lo, hi) are masked; an item is up to 64 rows of one row
+// columns outside [lo, hi) are masked; an item is up to 128 rows of one row
 // cluster against the refs [ref0, ref1).
 struct MxRef {
     int g0, g1;
@@ -187,6 +187,10 @@ constexpr int kMxRhs = 32;
 // U T work item (up to four interleaved 32-wide MFMA tiles).
 constexpr int kMxVCols = 128;
 constexpr int kMxLrRows = 128;
+// The mixed-precision layout pads each row cluster's U to a multiple of this many
+// (zero) columns and each column cluster's V to a multiple of this many (zero)
+// rows: the low-rank kernels then reduce over whole operand batches.
+constexpr int kMxLayoutPadK = 8;
 
 // per-point partial lists of one MVP schedule (CSR over sorted rows): an entry
 // v >= 0 is the P slot v, an entry v < 0 the column-slot group -(v+1)
@@ -307,7 +311,9 @@ struct Engine {
     Combine comb_mx;
     // per pass: X as FP32 rows of kMxRhs values, as BF16 hi/lo panels (per group of
     // 8 points: [hi|lo][rhs][8]), and coordinate panels per group ([dim][8] FP32)
-    DevBuf<float> Xs32, T32, Tp32, P32;
+    DevBuf<float> Xs32, T32, Tp32, P32, Ys32;   // Ys32: sorted result rows of a pass
+    DevBuf<int> inv1;                           // sorted position of each original row
+    bool inv1_ready = false;
     DevBuf<__bf16> Xp;
     DevBuf<float> cpan[2];
     bool cpan_ready = false;

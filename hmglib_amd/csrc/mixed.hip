@@ -31,7 +31,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;
-constexpr int kMxRowsItem = 64;  // rows per near-field and low-rank item (two 32-row tiles)
+constexpr int kMxRowsItem = 128; // rows per near-field item (four 32-row tiles)
+constexpr int kMxRowTiles = kMxRowsItem / 32;
 constexpr int kGroup = 8;        // points per panel group
 inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -134,12 +135,13 @@ __device__ __forceinline__ f32x16 near_tile(const KernelParamsF& kp, int dim, co
     return acc;
 }
 
-// One wavefront per item: up to 64 rows (two 32-row tiles) of one row cluster
+// One wavefront per item: up to 128 rows (four 32-row tiles) of one row cluster
 // against the item's column runs, 16 columns (two 8-point groups) per step.
 // Lane l evaluates row (l % 32) of each tile against the 8 columns of group
 // g + l/32, which is exactly its A operand; its B operand is the same group's
-// X panel for right-hand side l % 32.  Columns outside a run are masked by
-// zeroing their B values.  P[(q*64 + r)*32 + c] = row r's partial for rhs c.
+// X panel for right-hand side l % 32, shared by the four tiles.  Columns outside
+// a run are masked by zeroing their B values.  P[(q*128 + r)*32 + c] = row r's
+// partial for rhs c.
 template <int KIND, int DIM, bool REG>
 __global__ void __launch_bounds__(kThreads)
 near_mx_kernel(const float* __restrict__ cp1, const float* __restrict__ cp2, KernelParamsF kp,
@@ -153,23 +155,22 @@ near_mx_kernel(const float* __restrict__ cp1, const float* __restrict__ cp2, Ker
     const MxItem it = items[q];
     const int dim = DIM > 0 ? DIM : kp.dim;
     const int c32 = lane & 31, h = lane >> 5;
-    const bool two = it.nrows > 32;
-    float xr0[DM], xr1[DM];
-    const int rr0 = c32 < it.nrows ? c32 : 0;
-    const int rr1 = 32 + c32 < it.nrows ? 32 + c32 : 0;
-    const int row0 = it.row0 + rr0, row1 = it.row0 + rr1;
+    const int ntile = (it.nrows + 31) >> 5;   // wave-uniform
+    float xr[kMxRowTiles][DM];
+    int row[kMxRowTiles];
 #pragma unroll
-    for (int d = 0; d < DM; d++) {
-        if (DIM > 0 || d < dim) {
-            xr0[d] = cp1[((int64_t)(row0 >> 3) * dim + d) * kGroup + (row0 & 7)];
-            xr1[d] = cp1[((int64_t)(row1 >> 3) * dim + d) * kGroup + (row1 & 7)];
-        } else {
-            xr0[d] = xr1[d] = 0.f;
-        }
+    for (int t = 0; t < kMxRowTiles; t++) {
+        const int rr = t * 32 + c32 < it.nrows ? t * 32 + c32 : 0;  // rows past the item: its first row
+        row[t] = it.row0 + rr;
+#pragma unroll
+        for (int d = 0; d < DM; d++)
+            xr[t][d] = (DIM > 0 || d < dim) ? cp1[((int64_t)(row[t] >> 3) * dim + d) * kGroup + (row[t] & 7)] : 0.f;
     }
-    f32x16 acc0, acc1;
+    f32x16 acc[kMxRowTiles];
 #pragma unroll
-    for (int r = 0; r < 16; r++) acc0[r] = acc1[r] = 0.f;
+    for (int t = 0; t < kMxRowTiles; t++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[t][r] = 0.f;
     for (int rf = it.ref0; rf < it.ref1; rf++) {
         const MxRef f = refs[rf];
         for (int g = f.g0; g < f.g1; g += 2) {
@@ -194,16 +195,19 @@ near_mx_kernel(const float* __restrict__ cp1, const float* __restrict__ cp2, Ker
                 else
                     xc[d] = f32x8{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             }
-            acc0 = near_tile<KIND, DIM, REG>(kp, dim, xr0, xc, row0, col0, bh, bl, acc0);
-            if (two) acc1 = near_tile<KIND, DIM, REG>(kp, dim, xr1, xc, row1, col0, bh, bl, acc1);
+#pragma unroll
+            for (int t = 0; t < kMxRowTiles; t++)
+                if (t < ntile) acc[t] = near_tile<KIND, DIM, REG>(kp, dim, xr[t], xc, row[t], col0, bh, bl, acc[t]);
         }
     }
     float* __restrict__ out = P + (int64_t)q * kMxRowsItem * kMxRhs + c32;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int rr = (r >> 2) * 8 + h * 4 + (r & 3);
-        if (rr < it.nrows) out[(int64_t)rr * kMxRhs] = acc0[r];
-        if (32 + rr < it.nrows) out[(int64_t)(32 + rr) * kMxRhs] = acc1[r];
+    for (int t = 0; t < kMxRowTiles; t++) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int rr = t * 32 + (r >> 2) * 8 + h * 4 + (r & 3);
+            if (rr < it.nrows) out[(int64_t)rr * kMxRhs] = acc[t][r];
+        }
     }
 }
 
@@ -241,22 +245,24 @@ __device__ __forceinline__ float vget(const typename VecT<NT>::type& v, int e)
         return v[e];
 }
 
-// operands of 4 k-steps: 4 B values and 4 NT-wide A vectors
+// operands of kSteps MFMA k-steps (2 k each): B values and NT-wide A vectors
+constexpr int kSteps = 4;  // kBatchK must divide the layout's reduction padding (8)
+constexpr int kBatchK = 2 * kSteps;
 template <int NT>
 struct Batch {
-    float b[4];
-    typename VecT<NT>::type a[4];
+    float b[kSteps];
+    typename VecT<NT>::type a[kSteps];
 };
 
 // a: this lane's A stream (advances by lda per k), b: its B stream (32 floats
-// per k); k-steps k0, k0+2, k0+4, k0+6
+// per k); k-steps k0, k0+2, k0+4, k0+6 (the lane's h is folded into a and b)
 template <int NT>
 __device__ __forceinline__ void lr_load(Batch<NT>& st, const float* __restrict__ a, int64_t lda,
                                         const float* __restrict__ b, int k0)
 {
     typedef typename VecT<NT>::type V;
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
+    for (int s = 0; s < kSteps; s++) {
         st.b[s] = b[(int64_t)(k0 + 2 * s) * kMxRhs];
         st.a[s] = *(const V*)(a + (int64_t)(k0 + 2 * s) * lda);
     }
@@ -266,35 +272,39 @@ template <int NT>
 __device__ __forceinline__ void lr_mfma(f32x16 (&acc)[NT], const Batch<NT>& st)
 {
 #pragma unroll
-    for (int s = 0; s < 4; s++)
+    for (int s = 0; s < kSteps; s++)
 #pragma unroll
         for (int e = 0; e < NT; e++)
             acc[e] = __builtin_amdgcn_mfma_f32_32x32x2f32(vget<NT>(st.a[s], e), st.b[s], acc[e], 0, 0, 0);
 }
 
-// acc[e] += sum over k < K of A-stream x B-stream; the next batch of operands is
-// loaded while the current batch's MFMAs run (two register sets).  The next batch
-// is always loaded (on the last step from a clamped, valid position) so wait
-// counts stay path-independent; sched_barrier keeps the loads ahead of the MFMAs.
+// acc[e] += sum over k < K of A-stream x B-stream.  The mixed-precision factor
+// layout pads every reduction length (U columns of a row cluster, V rows of a
+// column cluster) to a multiple of kBatchK with zeros, so all batches are full.
+// The next batch is loaded while the current batch's MFMAs run (two register
+// sets); the next load is never skipped (on the last step it repeats a valid
+// position) so wait counts stay path-independent, and sched_barrier keeps the
+// loads ahead of the MFMAs.
 template <int NT>
 __device__ __forceinline__ void lr_body(f32x16 (&acc)[NT], const float* __restrict__ a, int64_t lda,
                                         const float* __restrict__ b, int K, int h)
 {
     typedef typename VecT<NT>::type V;
-    const int kfull = K & ~7;
+    const int kfull = K / kBatchK * kBatchK;
     Batch<NT> A, B;
     if (kfull > 0) lr_load<NT>(A, a, lda, b, 0);
-    for (int k = 0; k < kfull; k += 16) {
-        lr_load<NT>(B, a, lda, b, min(k + 8, kfull - 8));
+    for (int k = 0; k < kfull; k += 2 * kBatchK) {
+        lr_load<NT>(B, a, lda, b, min(k + kBatchK, kfull - kBatchK));
         __builtin_amdgcn_sched_barrier(0);
         lr_mfma<NT>(acc, A);
         __builtin_amdgcn_sched_barrier(0);
-        if (k + 8 >= kfull) break;
-        lr_load<NT>(A, a, lda, b, min(k + 16, kfull - 8));
+        if (k + kBatchK >= kfull) break;
+        lr_load<NT>(A, a, lda, b, min(k + 2 * kBatchK, kfull - kBatchK));
         __builtin_amdgcn_sched_barrier(0);
         lr_mfma<NT>(acc, B);
         __builtin_amdgcn_sched_barrier(0);
     }
+    // lengths that are not a multiple of kBatchK (not produced by the layout)
     for (int k = kfull; k < K; k += 2) {
         const bool ok = k + h < K;
         const float bv = ok ? b[(int64_t)k * kMxRhs] : 0.f;
@@ -390,17 +400,50 @@ lr_u32_kernel(const LrItem* __restrict__ items, int nitems, const float* __restr
     }
 }
 
-// Y[order1[i] + (c0 + c)*n1] = sum of row i's partials (list order), c < nr
+// Ys[i*32 + c] = sum of sorted row i's partials, in list order (deterministic)
 __global__ void __launch_bounds__(kThreads)
 combine_mx_kernel(const int* __restrict__ ptr, const int* __restrict__ idx, const float* __restrict__ P,
-                  const uint64_t* __restrict__ order1, double* __restrict__ Y, int n1, int c0, int nr)
+                  float* __restrict__ Ys, int n1)
 {
     const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     const int i = (int)(t / kMxRhs), c = (int)(t % kMxRhs);
     if (i >= n1) return;
     float s = 0.f;
     for (int q = ptr[i]; q < ptr[i + 1]; q++) s += P[(int64_t)idx[q] * kMxRhs + c];
-    if (c < nr) Y[order1[i] + (int64_t)(c0 + c) * n1] = (double)s;
+    Ys[t] = s;
+}
+
+// Y[p + (c0 + c)*n1] = Ys[inv1[p]*32 + c] for c < nr: 64 caller rows per block,
+// read as whole 128-byte sorted rows, transposed through LDS, written as 64
+// consecutive doubles per right-hand side
+constexpr int kPermRows = 64;
+__global__ void __launch_bounds__(kThreads)
+permute_out_kernel(const float* __restrict__ Ys, const int* __restrict__ inv1, double* __restrict__ Y,
+                   int n1, int c0, int nr)
+{
+    __shared__ float tile[kMxRhs][kPermRows + 1];
+    const int p0 = blockIdx.x * kPermRows;
+#pragma unroll
+    for (int k = 0; k < kPermRows * kMxRhs / kThreads; k++) {
+        const int e = threadIdx.x + kThreads * k;
+        const int pl = e / kMxRhs, c = e % kMxRhs;
+        const int p = p0 + pl;
+        tile[c][pl] = p < n1 ? Ys[(int64_t)inv1[p] * kMxRhs + c] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPermRows * kMxRhs / kThreads; k++) {
+        const int e = threadIdx.x + kThreads * k;
+        const int c = e / kPermRows, pl = e % kPermRows;
+        const int p = p0 + pl;
+        if (c < nr && p < n1) Y[p + (int64_t)(c0 + c) * n1] = (double)tile[c][pl];
+    }
+}
+
+__global__ void inv32_kernel(const uint64_t* __restrict__ order, int* __restrict__ inv, int n)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) inv[order[i]] = i;
 }
 
 template <int KIND, int DIM>
@@ -458,9 +501,19 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
     const int64_t slots = near0 + (int64_t)eng.n_mx_items * kMxRowsItem;
     HMG_TRY(eng.Xs32.alloc((size_t)n2p * kMxRhs));
     HMG_TRY(eng.Xp.alloc((size_t)g2 * 2 * kMxRhs * kGroup));
-    HMG_TRY(eng.T32.alloc((size_t)std::max<int64_t>(eng.tsize, 1) * kMxRhs));
+    // T rows past tsize: read (times zero U columns) by the last row cluster's
+    // padded columns; they must hold finite values
+    HMG_TRY(eng.T32.alloc((size_t)(eng.tsize + kMxLayoutPadK) * kMxRhs));
+    HMG_HIP(hipMemsetAsync(eng.T32.p + eng.tsize * kMxRhs, 0, sizeof(float) * kMxLayoutPadK * kMxRhs, s));
     HMG_TRY(eng.Tp32.alloc((size_t)std::max(eng.n_tp, 1) * kMxVCols * kMxRhs));
     HMG_TRY(eng.P32.alloc((size_t)std::max<int64_t>(slots, 1) * kMxRhs));
+    HMG_TRY(eng.Ys32.alloc((size_t)n1 * kMxRhs));
+    if (!eng.inv1_ready) {
+        HMG_TRY(eng.inv1.alloc(n1));
+        inv32_kernel<<<grid_for(n1, kThreads), kThreads, 0, s>>>(order1, eng.inv1.p, n1);
+        HMG_HIP(hipGetLastError());
+        eng.inv1_ready = true;
+    }
     const KernelParamsF kpf = make_params_f(kp);
     const bool reg = kp.reg != 0.0;
     const float* U = eng.U32.p;
@@ -539,7 +592,9 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
         HMG_HIP(hipStreamWaitEvent(s, eng.ev_join, 0));
         eng.prof.begin(PROF_COMBINE, s);
         combine_mx_kernel<<<grid_for((int64_t)n1 * kMxRhs, kThreads), kThreads, 0, s>>>(
-            eng.comb_mx.ptr.p, eng.comb_mx.idx.p, eng.P32.p, order1, Y, n1, c0, nr);
+            eng.comb_mx.ptr.p, eng.comb_mx.idx.p, eng.P32.p, eng.Ys32.p, n1);
+        HMG_HIP(hipGetLastError());
+        permute_out_kernel<<<grid_for(n1, kPermRows), kThreads, 0, s>>>(eng.Ys32.p, eng.inv1.p, Y, n1, c0, nr);
         HMG_HIP(hipGetLastError());
         eng.prof.end(PROF_COMBINE, s);
     }

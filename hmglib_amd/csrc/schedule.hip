@@ -31,7 +31,7 @@ constexpr int kNearSlice = 2 * kTileRows;  // rows per near-field work item (two
 constexpr int kNearChunkCols = 4096;       // near-field columns per work item
 constexpr int kVChunkRows = 1024;          // points per V^T x work item (FP64)
 constexpr int kMxVChunkRows = 4096;        // points per V^T x work item (mixed precision)
-constexpr int kMxRows = 64;                // rows per mixed-precision near-field item
+constexpr int kMxRows = 128;               // rows per mixed-precision near-field item
 constexpr int kMxChunkCols = 4096;         // columns per mixed-precision near-field item
 
 inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
@@ -149,7 +149,7 @@ hmg_status build_combine(Combine& cb, int n1, const std::vector<int4>& near_rows
 // Mixed-precision near field: per dense row cluster, the column ranges of its
 // DENSE blocks sorted by start and merged where contiguous, cut into pieces of at
 // most kMxChunkCols columns; each piece becomes a run of 8-point groups (an even
-// number of groups, masked outside the piece).  Items are 64-row slices of the
+// number of groups, masked outside the piece).  Items are 128-row slices of the
 // row cluster against consecutive pieces of at most kMxChunkCols columns.
 void build_mixed_near(Engine& eng)
 {
@@ -199,8 +199,9 @@ void build_mixed_near(Engine& eng)
             eng.h_mx_refs.insert(eng.h_mx_refs.end(), pieces.begin() + p0, pieces.begin() + p1);
             const int ref1 = (int)eng.h_mx_refs.size();
             for (int r0 = 0; r0 < m1; r0 += kMxRows) {
-                eng.h_mx_items.push_back(MxItem{l1 + r0, std::min(kMxRows, m1 - r0), ref0, ref1});
-                eng.mx_evals += (int64_t)kMxRows * cols;
+                const int nr = std::min(kMxRows, m1 - r0);
+                eng.h_mx_items.push_back(MxItem{l1 + r0, nr, ref0, ref1});
+                eng.mx_evals += round_up(nr, 32) * cols;  // whole 32-row tiles are evaluated
             }
             p0 = p1;
         }
@@ -373,10 +374,13 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
             trun += eng.h_kb[a];
             K += eng.h_kb[a];
         }
-        urun += (int64_t)ldu * K;
+        // mixed precision: K padded with zero columns (the T rows they meet hold
+        // the next cluster's values, or zeros past the end)
+        const int Kp = mixed ? (int)round_up(K, kMxLayoutPadK) : K;
+        urun += (int64_t)ldu * Kp;
         if (K == 0) continue;
         for (int r0 = 0; r0 < m1; r0 += lrows)
-            items.push_back(LrItem{l1 + r0, std::min(lrows, m1 - r0), ldu, K, uC + r0, tC});
+            items.push_back(LrItem{l1 + r0, std::min(lrows, m1 - r0), ldu, Kp, uC + r0, tC});
     }
     eng.h_uoff[na] = urun;
     eng.h_toff[na] = trun;
@@ -427,8 +431,11 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
             for (int l = 0; l < eng.h_kb[a]; l++) tmap[col + pos + l] = (int)(eng.h_toff[a] + l);
             pos += eng.h_kb[a];
         }
-        vrun += (int64_t)m2 * ld;
-        const int nch = (m2 + vchunk - 1) / vchunk;
+        // mixed precision: m2 padded with zero rows (they meet X rows of the next
+        // points, or the zero rows past the end of X)
+        const int m2p = mixed ? (int)round_up(m2, kMxLayoutPadK) : m2;
+        vrun += (int64_t)m2p * ld;
+        const int nch = (m2p + vchunk - 1) / vchunk;
         for (int s0 = 0; s0 < K; s0 += vslice) {
             const int nc = std::min(vslice, K - s0);
             // mixed precision: a single-chunk slice writes T directly
@@ -436,7 +443,7 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
             const int tp0 = direct ? -1 : (mixed ? ntp : (int)vitems.size());
             for (int c = 0; c < nch; c++) {
                 const int j0 = c * vchunk;
-                vitems.push_back(LvItem{l2 + j0, std::min(vchunk, m2 - j0), ld, nc,
+                vitems.push_back(LvItem{l2 + j0, std::min(vchunk, m2p - j0), ld, nc,
                                         vD + s0 + (int64_t)j0 * ld, (int)(col + s0),
                                         direct ? -1 : tp0 + c});
             }
