@@ -211,8 +211,9 @@ def cloud(rng, n, dim, blobs):
     return np.clip(centres[lab].T + 0.05 * rng.standard_normal((dim, n)), 0.0, 1.0)
 
 
-def serve_session(reqs, precision, budget, slots, log_tag):
-    """Replays `reqs` open-loop against a native server; returns latency stats."""
+def serve_session(reqs, precision, budget, slots, log_tag, trace_path=None):
+    """Replays `reqs` open-loop against a native server; returns latency stats.
+    With trace_path, writes one JSON record per request (kind, size, times)."""
     import hmglib_amd
     from hmglib_amd import hip
     from hmglib_amd.server import Server
@@ -237,23 +238,29 @@ def serve_session(reqs, precision, budget, slots, log_tag):
             bits, leaf, eta, k = (32, 64, 1.5, 32) if r["dim"] == 2 else (20, 256, 1.0, 24)
             rid, mid = srv.submit_build(X, X, bits, eta, 50, leaf, k, r["kind"], 0.0, 0.1)
             mat_n[mid] = r["n"]
-            ids.append((rid, "build", r["n"]))
+            ids.append((rid, "build", r))
         else:
             m = resident[int(r["pick"] * len(resident))]
             y = ring[k_ring % len(ring)]
             k_ring += 1
             rid = srv.submit_apply_ptr(m, Xpool.ptr, y.ptr, r["nrhs"])
-            ids.append((rid, "apply", mat_n[m]))
+            ids.append((rid, "apply", dict(r, n=mat_n[m])))
     srv.drain()
     wall = time.perf_counter() - t0
     lat = {"build": [], "apply": []}
+    svc = {"build": [], "apply": []}
     fails = 0
-    for rid, kind, n in ids:
+    trace = []
+    for rid, kind, r in ids:
         info = srv.query(rid)
+        trace.append({"kind": kind, "n": r["n"], "dim": r["dim"], "nrhs": r["nrhs"], "status": info["status"],
+                      "t_submit": info["t_submit"], "t_start": info["t_start"], "t_end": info["t_end"],
+                      "merged": info["merged"]})
         if info["status"] != 0:
             fails += 1
             continue
         lat[kind].append((info["t_end"] - info["t_submit"]) * 1e3)
+        svc[kind].append((info["t_end"] - info["t_start"]) * 1e3)
     st = srv.stats()
     srv.destroy()
     for b in ring:
@@ -266,9 +273,14 @@ def serve_session(reqs, precision, budget, slots, log_tag):
            "apply_p99_ms": float(np.percentile(lat["apply"], 99)) if lat["apply"] else None,
            "build_p50_ms": float(np.percentile(lat["build"], 50)) if lat["build"] else None,
            "build_p99_ms": float(np.percentile(lat["build"], 99)) if lat["build"] else None,
+           "build_service_p50_ms": float(np.percentile(svc["build"], 50)) if svc["build"] else None,
+           "apply_service_p50_ms": float(np.percentile(svc["apply"], 50)) if svc["apply"] else None,
            "builds": len(lat["build"]), "applies": len(lat["apply"]), "evictions": st["evictions"],
            "passes": st["passes"], "shared_requests": st["shared"]}
     log(f"[bench] serving {log_tag}: {json.dumps(res)}")
+    if trace_path:
+        with open(trace_path, "w") as f:
+            json.dump(trace, f)
     return res
 
 
@@ -306,6 +318,7 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=10.0)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-trace", default="", help="prefix of per-request JSON traces of the serving leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -421,7 +434,8 @@ def main():
                   "workload": "Poisson arrivals; N=2^(12+6u); uniform or 8-blob; 2D/3D; exp or Gaussian "
                               "l=0.1; 1-16 RHS; 70% applies to cached H-matrices, 30% builds"}
         for name, prec in (("fp64", hmglib_amd.PREC_FP64), ("mixed", hmglib_amd.PREC_MIXED)):
-            served[name] = serve_session(reqs, prec, 200e9, args.serve_slots, name)
+            served[name] = serve_session(reqs, prec, 200e9, args.serve_slots, name,
+                                         f"{args.serve_trace}_{name}.json" if args.serve_trace else None)
 
     cpu = None
     if not args.no_cpu:

@@ -22,6 +22,8 @@ OPT_NEAR_SYMMETRIC = 1
 OPT_NEAR_LDS_BYTES = 2
 OPT_SERIALIZE = 3
 OPT_PRECISION = 4
+OPT_ACA_ROW_SCAN = 5
+OPT_ACA_BIG_BLOCK = 6
 
 PREC_FP64 = 0
 PREC_MIXED = 1

@@ -28,30 +28,230 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// val - sum_{l < r} p[l*sp] * q[l*sq], accumulated as fma(-p, q, val) in l order
+// (the reference's contraction order, linear_algebra.cu:955/:1143).  The loads of
+// eight consecutive l are issued before their FMAs, so a step waits for one
+// memory latency per eight terms instead of one per term.
+template <class T>
+__device__ __forceinline__ double residual_update(double val, int r, const T* __restrict__ p, int64_t sp,
+                                                  const T* __restrict__ q, int64_t sq)
+{
+    int l = 0;
+    for (; l + 8 <= r; l += 8) {
+        T pv[8], qv[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            pv[e] = p[(int64_t)(l + e) * sp];
+            qv[e] = q[(int64_t)(l + e) * sq];
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) val = __fma_rn(-(double)pv[e], (double)qv[e], val);
+    }
+    for (; l < r; l++) val = __fma_rn(-(double)p[(int64_t)l * sp], (double)q[(int64_t)l * sq], val);
+    return val;
+}
+
+// A team runs the ACA of one block: one wavefront (NW = 1) for most blocks, or a
+// whole workgroup of NW wavefronts for the few large blocks, whose rank loop is
+// otherwise the critical path of the launch.  Team reductions (row norm,
+// column-pivot argmax) go through LDS when NW > 1; every thread ends with the
+// same value.
+template <int NW>
+struct Team {
+    double* s_v;
+    int* s_i;
+    double* s_x;
+    __device__ int tid() const { return NW == 1 ? (int)(threadIdx.x % kWave) : (int)threadIdx.x; }
+    static constexpr int size = NW * kWave;
+    __device__ void sync() const
+    {
+        if (NW == 1)
+            __threadfence_block();
+        else
+            __syncthreads();
+    }
+    __device__ double sum(double v) const
+    {
+        v = wave_sum(v);
+        if (NW == 1) return v;
+        const int w = threadIdx.x / kWave;
+        if (threadIdx.x % kWave == 0) s_v[w] = v;
+        __syncthreads();
+        double t = 0.0;
+        for (int q = 0; q < NW; q++) t += s_v[q];
+        __syncthreads();
+        return t;
+    }
+    // R sums at once, each equal to sum() of its value
+    template <int R>
+    __device__ void sum_n(double (&v)[R]) const
+    {
+#pragma unroll
+        for (int c = 0; c < R; c++) v[c] = wave_sum(v[c]);
+        if (NW == 1) return;
+        const int w = threadIdx.x / kWave;
+        if (threadIdx.x % kWave == 0) {
+#pragma unroll
+            for (int c = 0; c < R; c++) s_v[w * R + c] = v[c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            double t = 0.0;
+            for (int q = 0; q < NW; q++) t += s_v[q * R + c];
+            v[c] = t;
+        }
+        __syncthreads();
+    }
+    // (|v|, index, v) maximum with the tie rule: last index (tie_last) or first
+    __device__ static bool better(double ob, int oj, double best, int bj, int tie_last)
+    {
+        return (ob > best) || (ob == best && (tie_last ? (oj > bj) : (oj < bj && oj >= 0)));
+    }
+    __device__ void argmax(double& best, int& bj, double& bval, int tie_last) const
+    {
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off, kWave);
+            const int oj = __shfl_xor(bj, off, kWave);
+            const double ov = __shfl_xor(bval, off, kWave);
+            if (better(ob, oj, best, bj, tie_last)) {
+                best = ob;
+                bj = oj;
+                bval = ov;
+            }
+        }
+        if (NW == 1) return;
+        const int w = threadIdx.x / kWave;
+        if (threadIdx.x % kWave == 0) {
+            s_v[w] = best;
+            s_i[w] = bj;
+            s_x[w] = bval;
+        }
+        __syncthreads();
+        best = s_v[0];
+        bj = s_i[0];
+        bval = s_x[0];
+        for (int q = 1; q < NW; q++) {
+            if (better(s_v[q], s_i[q], best, bj, tie_last)) {
+                best = s_v[q];
+                bj = s_i[q];
+                bval = s_x[q];
+            }
+        }
+        __syncthreads();
+    }
+};
+
+// Rows scanned at once when the residual-row search rejects a row, and the
+// largest rank step that scans (its U terms are staged per wavefront in LDS;
+// later steps keep the one-row search).
+constexpr int kScanRows = 4;
+constexpr int kScanL = 64;
+
+// Squared 2-norms of the residual rows i0 .. i0+kScanRows-1 at rank step r (rows
+// past m1 - 1 give values that the caller ignores), without storing the rows.
+// Each entry and each per-thread partial sum is formed exactly as the one-row
+// search forms it (same l order of the fma chain, same j order of the squares),
+// and the team sums are the same, so a row's norm here equals its norm there
+// bitwise.  The kScanRows rows are independent chains sharing one load of each
+// V entry, and their U terms (the same for every lane) are read from the
+// wavefront's LDS slice sU[l][c], so a scan of the many rejected rows of a
+// numerically rank-deficient block runs at arithmetic rate rather than one
+// memory latency per term.
+template <class T, int NW, int KIND>
+__device__ void scan_row_norms(const Team<NW>& team, Coords c1, Coords c2, const KernelParams& kp, int l1,
+                               int m1, int l2, int m2, int i0, int r, int64_t ld, int64_t lu,
+                               const T* __restrict__ Ub, const T* __restrict__ Vb, T* __restrict__ sU,
+                               double* __restrict__ sX, double (&nrm)[kScanRows])
+{
+    constexpr int R = kScanRows;
+    constexpr int TS = Team<NW>::size;
+    const int lane = threadIdx.x % kWave;
+    int ic[R];
+#pragma unroll
+    for (int c = 0; c < R; c++) {
+        ic[c] = min(i0 + c, m1 - 1);
+        nrm[c] = 0.0;
+    }
+    // the rows' coordinates, sX[c * MAX_POINT_DIMENSION + d] (read back from LDS
+    // instead of keeping R x dim addresses live in registers), and their U terms
+    // U[ic[c], l] at sU[l * R + c], l < r <= kScanL
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < R * MAX_POINT_DIMENSION) {
+        const int c = lane / MAX_POINT_DIMENSION, d = lane % MAX_POINT_DIMENSION;
+        sX[lane] = d < kp.dim ? c1.x[d][l1 + min(i0 + c, m1 - 1)] : 0.0;
+    }
+    for (int q = lane; q < R * r; q += kWave) {
+        const int c = q % R, l = q / R;
+        sU[q] = Ub[(int64_t)l * lu + min(i0 + c, m1 - 1)];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int j = team.tid(); j < m2; j += TS) {
+        // K(i, j) exactly as kernel_entry_k forms it (same r^2 order), row
+        // coordinates from sX
+        double xj[MAX_POINT_DIMENSION];
+#pragma unroll
+        for (int d = 0; d < MAX_POINT_DIMENSION; d++) xj[d] = d < kp.dim ? c2.x[d][l2 + j] : 0.0;
+        double val[R];
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < MAX_POINT_DIMENSION; d++) {
+                if (d < kp.dim) {
+                    const double t = sX[c * MAX_POINT_DIMENSION + d] - xj[d];
+                    r2 = __dadd_rn(r2, __dmul_rn(t, t));
+                }
+            }
+            double v = kernel_of_r2_k<KIND>(kp, r2);
+            if (l1 + ic[c] == l2 + j) v += kp.reg;
+            val[c] = v;
+        }
+        const T* __restrict__ vrow = Vb + (int64_t)j * ld;
+        int l = 0;
+        for (; l + 8 <= r; l += 8) {
+            T pv[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) pv[e] = vrow[l + e];
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const T* u = sU + (l + e) * R;
+#pragma unroll
+                for (int c = 0; c < R; c++) val[c] = __fma_rn(-(double)pv[e], (double)u[c], val[c]);
+            }
+        }
+        for (; l < r; l++) {
+            const double pv = (double)vrow[l];
+            const T* u = sU + l * R;
+#pragma unroll
+            for (int c = 0; c < R; c++) val[c] = __fma_rn(-pv, (double)u[c], val[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < R; c++) nrm[c] = __fma_rn(val[c], val[c], nrm[c]);
+    }
+    team.template sum_n<R>(nrm);
+}
+
 // Factor layout: U_b[i, l] at U + uoff[a] + l * ldu[a] + i (column-major, ldu >= m1:
 // the row cluster's padded height); V_b is stored transposed, V_b[j, l] at
 // V + voff[a] + j * ldv[a] + l, i.e. row j of V_b is contiguous (ldv >= total rank
-// of the column cluster's blocks).  T is the storage type of the factors (double,
-// or float for mixed precision): every residual, norm and pivot division is
-// computed in FP64 from the stored values and rounded to T once when stored.
-template <class T>
-__global__ void __launch_bounds__(kAcaThreads)
-aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, int naca,
-           const int* __restrict__ kb, const int64_t* __restrict__ uoff, const int* __restrict__ ldu,
-           const int64_t* __restrict__ voff, const int* __restrict__ ldv, T* __restrict__ U,
-           T* __restrict__ V, int* __restrict__ keff, int tie_last)
+// of the column cluster's blocks).  T is the storage type of the factors (the
+// engine instantiates double only); every residual, norm and pivot division is
+// computed in FP64 from the stored values and rounded to T when stored.
+// Entry j of a row (i of a column) is always handled by the same thread, so each
+// entry is computed exactly as in a single-wavefront team.
+template <class T, int NW, int KIND>
+__device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const KernelParams& kp, int4 b, int k,
+                          int64_t ld, int64_t lu, T* __restrict__ Ub, T* __restrict__ Vb, int* keff_out,
+                          int tie_last, int scan, T* __restrict__ sU, double* __restrict__ sX)
 {
-    const int lane = threadIdx.x % kWave;
-    const int a = blockIdx.x * kAcaWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    if (a >= naca) return;
-    const int4 b = blk[a];
+    const int tid = team.tid();
+    constexpr int TS = Team<NW>::size;
     const int l1 = b.x, m1 = b.y - b.x + 1;
     const int l2 = b.z, m2 = b.w - b.z + 1;
-    const int k = kb[a];
-    const int64_t ld = ldv[a];
-    const int64_t lu = ldu[a];
-    T* __restrict__ Ub = U + uoff[a];
-    T* __restrict__ Vb = V + voff[a];
     int i_r = -1;
     int ke = 0;
     for (int r = 0; r < k; r++) {
@@ -64,27 +264,43 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
             }
             i_r++;
             double nrm = 0.0;
-            for (int j = lane; j < m2; j += kWave) {
-                double val = kernel_entry(kp, c1, l1 + i_r, c2, l2 + j);
+            for (int j = tid; j < m2; j += TS) {
+                double val = kernel_entry_k<KIND>(kp, c1, l1 + i_r, c2, l2 + j);
                 const T* __restrict__ vrow = Vb + (int64_t)j * ld;
-                for (int l = 0; l < r; l++)
-                    val = __fma_rn(-(double)Ub[l * lu + i_r], (double)vrow[l], val);
+                val = residual_update<T>(val, r, vrow, 1, Ub + i_r, lu);
                 Vb[(int64_t)j * ld + r] = (T)val;
                 nrm = __fma_rn(val, val, nrm);
             }
-            nrm = wave_sum(nrm);
+            nrm = team.sum(nrm);
             if (sqrt(nrm) >= 1.0e-13) break;
+            // row i_r rejected: look for the next acceptable row kScanRows rows at a
+            // time; i_r is left just before it (the one-row search above then
+            // recomputes and stores it), or at m1 - 1 when every row is rejected
+            if (!scan || r > kScanL) continue;
+            int found = m1;
+            for (int i0 = i_r + 1; i0 < m1; i0 += kScanRows) {
+                double rn[kScanRows];
+                scan_row_norms<T, NW, KIND>(team, c1, c2, kp, l1, m1, l2, m2, i0, r, ld, lu, Ub, Vb, sU, sX, rn);
+                for (int c = 0; c < kScanRows && i0 + c < m1; c++) {
+                    if (sqrt(rn[c]) >= 1.0e-13) {
+                        found = i0 + c;
+                        break;
+                    }
+                }
+                if (found < m1) break;
+            }
+            i_r = found - 1;
         }
         if (stop) {
             // the rank stops here: column r of V (last rejected row) is cleared, column r
             // of U was never written; both stay zero in the product
-            for (int j = lane; j < m2; j += kWave) Vb[(int64_t)j * ld + r] = (T)0;
+            for (int j = tid; j < m2; j += TS) Vb[(int64_t)j * ld + r] = (T)0;
             break;
         }
         // column pivot (tuple_absolute_maximum, :991-1018)
         double best = -1.0, bval = 0.0;
         int bj = -1;
-        for (int j = lane; j < m2; j += kWave) {
+        for (int j = tid; j < m2; j += TS) {
             const double x = (double)Vb[(int64_t)j * ld + r];
             const double ax = fabs(x);
             if (tie_last ? (ax >= best) : (ax > best)) {
@@ -93,59 +309,140 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
                 bval = x;
             }
         }
-#pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) {
-            const double ob = __shfl_xor(best, off, kWave);
-            const int oj = __shfl_xor(bj, off, kWave);
-            const double ov = __shfl_xor(bval, off, kWave);
-            const bool take = (ob > best) || (ob == best && (tie_last ? (oj > bj) : (oj < bj && oj >= 0)));
-            if (take) {
-                best = ob;
-                bj = oj;
-                bval = ov;
-            }
-        }
+        team.argmax(best, bj, bval, tie_last);
         const int jr = bj;
         const double p = bval;
-        for (int j = lane; j < m2; j += kWave)
+        for (int j = tid; j < m2; j += TS)
             Vb[(int64_t)j * ld + r] = (T)((double)Vb[(int64_t)j * ld + r] / p);
-        __threadfence_block();
+        team.sync();
         // residual column (batched_fill_kernel_vector_and_scaled_substraction_for_u_r)
         const T* __restrict__ vpiv = Vb + (int64_t)jr * ld;
         T* __restrict__ u = Ub + r * lu;
-        for (int i = lane; i < m1; i += kWave) {
-            double val = kernel_entry(kp, c1, l1 + i, c2, l2 + jr);
-            for (int l = 0; l < r; l++) val = __fma_rn(-(double)vpiv[l], (double)Ub[l * lu + i], val);
+        for (int i = tid; i < m1; i += TS) {
+            double val = kernel_entry_k<KIND>(kp, c1, l1 + i, c2, l2 + jr);
+            val = residual_update<T>(val, r, Ub + i, lu, vpiv, 1);
             u[i] = (T)val;
         }
-        __threadfence_block();
+        team.sync();
         ke = r + 1;
     }
-    if (lane == 0) keff[a] = ke;
+    if (tid == 0) *keff_out = ke;
+}
+
+// one wavefront per block of list[] (indices into the ACA blocks)
+// four wavefronts per SIMD: the rank loop is latency-bound, so occupancy pays
+template <class T, int KIND>
+__global__ void __launch_bounds__(kAcaThreads) __attribute__((amdgpu_waves_per_eu(4)))
+aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, const int* __restrict__ list,
+           int nlist, const int* __restrict__ kb, const int64_t* __restrict__ uoff, const int* __restrict__ ldu,
+           const int64_t* __restrict__ voff, const int* __restrict__ ldv, T* __restrict__ U,
+           T* __restrict__ V, int* __restrict__ keff, int tie_last, int scan)
+{
+    const int q = blockIdx.x * kAcaWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if (q >= nlist) return;
+    const int a = list[q];
+    __shared__ T sU[kAcaWaves][kScanRows * kScanL];
+    __shared__ double sX[kAcaWaves][kScanRows * MAX_POINT_DIMENSION];
+    const Team<1> team{nullptr, nullptr, nullptr};
+    aca_block<T, 1, KIND>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a], keff + a,
+                          tie_last, scan, sU[threadIdx.x / kWave], sX[threadIdx.x / kWave]);
+}
+
+// one workgroup of kAcaBigWaves wavefronts per block of list[]
+constexpr int kAcaBigWaves = 8;
+template <class T, int KIND>
+__global__ void __launch_bounds__(kAcaBigWaves * kWave)
+aca_big_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, const int* __restrict__ list,
+               const int* __restrict__ kb, const int64_t* __restrict__ uoff, const int* __restrict__ ldu,
+               const int64_t* __restrict__ voff, const int* __restrict__ ldv, T* __restrict__ U,
+               T* __restrict__ V, int* __restrict__ keff, int tie_last, int scan)
+{
+    __shared__ double s_v[kAcaBigWaves * kScanRows];
+    __shared__ int s_i[kAcaBigWaves];
+    __shared__ double s_x[kAcaBigWaves];
+    const int a = list[blockIdx.x];
+    __shared__ T sU[kAcaBigWaves][kScanRows * kScanL];
+    __shared__ double sX[kAcaBigWaves][kScanRows * MAX_POINT_DIMENSION];
+    const Team<kAcaBigWaves> team{s_v, s_i, s_x};
+    aca_block<T, kAcaBigWaves, KIND>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
+                                     keff + a, tie_last, scan, sU[threadIdx.x / kWave], sX[threadIdx.x / kWave]);
 }
 
 }  // namespace
 
 template <class T>
-hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp, int k,
-                     int tie_last, T* U, T* V, int* keff)
+static hmg_status aca_build_t(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp, int k,
+                       int tie_last, T* U, T* V, int* keff)
 {
     (void)k;
     const int na = eng.naca;
     if (na == 0) return HMG_OK;
-    hipStream_t s = eng.stream;
-    const unsigned grid = (unsigned)((na + kAcaWaves - 1) / kAcaWaves);
+    hipStream_t s = eng.stream, s2 = eng.stream2;
+    // large blocks (a workgroup each, on the second stream) and the rest (a
+    // wavefront each); the large ones first in the list
+    std::vector<int>& order = eng.h_aca_list;
+    order.clear();
+    order.reserve(na);
+    int nbig = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        for (int a = 0; a < na; a++) {
+            const int4 b = eng.h_blk[eng.ndense + a];
+            const bool big = (b.y - b.x + 1) + (b.w - b.z + 1) > eng.aca_big_entries;
+            if (big == (pass == 0)) order.push_back(a);
+        }
+        if (pass == 0) nbig = (int)order.size();
+    }
+    HMG_TRY(eng.aca_list.alloc(na));
+    HMG_HIP(hipMemcpyAsync(eng.aca_list.p, order.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
+    const int nsmall = na - nbig;
+    const int scan = eng.aca_row_scan ? 1 : 0;
+    const int4* blk = eng.blk.p + eng.ndense;
     eng.prof.begin(PROF_ACA, s);
-    aca_kernel<T><<<grid, kAcaThreads, 0, s>>>(c1, c2, kp, eng.blk.p + eng.ndense, na, eng.kb.p,
-                                            eng.uoff.p, eng.ldu.p, eng.voff.p, eng.ldv.p, U, V, keff, tie_last);
-    HMG_HIP(hipGetLastError());
+    HMG_HIP(hipEventRecord(eng.ev_fork, s));
+    if (nbig > 0) {
+        HMG_HIP(hipStreamWaitEvent(s2, eng.ev_fork, 0));
+        HMG_DISPATCH_KIND(kp.kind, aca_big_kernel<T, KIND><<<nbig, kAcaBigWaves * kWave, 0, s2>>>(
+                                       c1, c2, kp, blk, eng.aca_list.p, eng.kb.p, eng.uoff.p, eng.ldu.p,
+                                       eng.voff.p, eng.ldv.p, U, V, keff, tie_last, scan));
+        HMG_HIP(hipGetLastError());
+        HMG_HIP(hipEventRecord(eng.ev_join, s2));
+    }
+    if (nsmall > 0) {
+        const unsigned grid = (unsigned)((nsmall + kAcaWaves - 1) / kAcaWaves);
+        HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND><<<grid, kAcaThreads, 0, s>>>(
+                                       c1, c2, kp, blk, eng.aca_list.p + nbig, nsmall, eng.kb.p, eng.uoff.p,
+                                       eng.ldu.p, eng.voff.p, eng.ldv.p, U, V, keff, tie_last, scan));
+        HMG_HIP(hipGetLastError());
+    }
+    if (nbig > 0) HMG_HIP(hipStreamWaitEvent(s, eng.ev_join, 0));
     eng.prof.end(PROF_ACA, s);
     return HMG_OK;
 }
 
-template hmg_status aca_build<double>(Engine&, const Coords&, const Coords&, const KernelParams&, int,
-                                      int, double*, double*, int*);
-template hmg_status aca_build<float>(Engine&, const Coords&, const Coords&, const KernelParams&, int,
-                                     int, float*, float*, int*);
+hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp, int k,
+                     int tie_last, double* U, double* V, int* keff)
+{
+    return aca_build_t<double>(eng, c1, c2, kp, k, tie_last, U, V, keff);
+}
+
+namespace {
+constexpr int kRoundThreads = 256;
+__global__ void __launch_bounds__(kRoundThreads) round_f32_kernel(const double* __restrict__ src,
+                                                                  float* __restrict__ dst, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * kRoundThreads;
+    for (int64_t i = (int64_t)blockIdx.x * kRoundThreads + threadIdx.x; i < n; i += stride)
+        dst[i] = (float)src[i];
+}
+}  // namespace
+
+hmg_status round_to_f32(Engine& eng, const double* src, float* dst, int64_t n)
+{
+    if (n <= 0) return HMG_OK;
+    const unsigned grid = (unsigned)std::min<int64_t>((n + kRoundThreads - 1) / kRoundThreads, 8192);
+    round_f32_kernel<<<grid, kRoundThreads, 0, eng.stream>>>(src, dst, n);
+    HMG_HIP(hipGetLastError());
+    return HMG_OK;
+}
 
 }  // namespace hmg

@@ -18,6 +18,8 @@ namespace {
 thread_local std::string g_error;
 }
 
+thread_local hipStream_t g_alloc_stream = nullptr;
+
 void set_error(const std::string& msg) { g_error = msg; }
 
 Engine* engine_of(const h_matrix_data* d) { return d ? (Engine*)d->engine : nullptr; }
@@ -161,6 +163,7 @@ void release_build(Engine& e, h_matrix_data* d)
     e.h_blk.clear();
     e.kb.release();
     e.keff.release();
+    e.aca_list.release();
     e.uoff.release();
     e.voff.release();
     e.toff.release();
@@ -250,11 +253,31 @@ hmg_status build_factors(Engine& e, h_matrix_data* d, const KernelParams& kp, in
     const int na = e.naca;
     // 256 elements of zero padding past the end: the mixed-precision products
     // read whole 128-element rows or columns, up to 32 elements beyond the last
-    HMG_TRY(U.alloc(e.h_uoff[na] + 256));
-    HMG_TRY(V.alloc(e.h_voff[na] + 256));
-    HMG_HIP(hipMemsetAsync(U.p, 0, sizeof(T) * U.n, e.stream));
-    HMG_HIP(hipMemsetAsync(V.p, 0, sizeof(T) * V.n, e.stream));
-    HMG_TRY(aca_build<T>(e, coords_of(d, 0), coords_of(d, 1), kp, k, tie_last, U.p, V.p, e.keff.p));
+    const int64_t nu = e.h_uoff[na] + 256, nv = e.h_voff[na] + 256;
+    if constexpr (sizeof(T) == sizeof(double)) {
+        HMG_TRY(U.alloc(nu));
+        HMG_TRY(V.alloc(nv));
+        HMG_HIP(hipMemsetAsync(U.p, 0, sizeof(double) * U.n, e.stream));
+        HMG_HIP(hipMemsetAsync(V.p, 0, sizeof(double) * V.n, e.stream));
+        HMG_TRY(aca_build(e, coords_of(d, 0), coords_of(d, 1), kp, k, tie_last, U.p, V.p, e.keff.p));
+    } else {
+        // the ACA runs in FP64 into temporaries of the FP32 layout (same pivots,
+        // ranks and stopping as in FP64 precision: FP32 storage during the ACA
+        // would turn the small residuals of late rank steps into rounding noise);
+        // the factors are rounded to FP32 once at the end
+        DevBuf<double> U64, V64;
+        HMG_TRY(U64.alloc(nu));
+        HMG_TRY(V64.alloc(nv));
+        HMG_HIP(hipMemsetAsync(U64.p, 0, sizeof(double) * U64.n, e.stream));
+        HMG_HIP(hipMemsetAsync(V64.p, 0, sizeof(double) * V64.n, e.stream));
+        HMG_TRY(aca_build(e, coords_of(d, 0), coords_of(d, 1), kp, k, tie_last, U64.p, V64.p, e.keff.p));
+        HMG_TRY(U.alloc(nu));
+        HMG_TRY(V.alloc(nv));
+        HMG_TRY(round_to_f32(e, U64.p, U.p, nu));
+        HMG_TRY(round_to_f32(e, V64.p, V.p, nv));
+        U64.release();
+        V64.release();
+    }
     HMG_HIP(hipStreamSynchronize(e.stream));
     return HMG_OK;
 }
@@ -296,6 +319,16 @@ hmg_status hmg_init_h_matrix_data(struct h_matrix_data* data, int point_count[2]
     HMG_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
     e->dim = dim;
     e->bits = bits;
+    // keep freed pool memory for reuse (temporaries of the tree build and the
+    // products come and go), up to a bound
+    {
+        int dev = 0;
+        hipMemPool_t pool = nullptr;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t thr = (uint64_t)8 << 30;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+    }
     if (const char* v = getenv("HMG_NEAR_LDS_KB")) e->near_lds_pad = std::max(0, atoi(v)) * 1024;
     if (const char* v = getenv("HMG_SERIALIZE")) e->serialize = atoi(v) != 0;
     data->engine = e;
@@ -318,21 +351,21 @@ hmg_status hmg_init_h_matrix_data(struct h_matrix_data* data, int point_count[2]
         data->point_count[i] = n;
         data->coords_d[i] = (double**)calloc(dim, sizeof(double*));
         for (int d = 0; d < dim; d++)
-            HMG_HIP(hipMalloc((void**)&data->coords_d[i][d], sizeof(double) * n));
-        HMG_HIP(hipMalloc((void**)&data->point_ids_d[i], sizeof(unsigned) * n));
-        HMG_HIP(hipMalloc((void**)&data->max_per_dim_d[i], sizeof(double) * dim));
-        HMG_HIP(hipMalloc((void**)&data->min_per_dim_d[i], sizeof(double) * dim));
-        HMG_HIP(hipMalloc((void**)&data->coords_device[i], sizeof(double*) * dim));
+            HMG_HIP(hipMallocAsync((void**)&data->coords_d[i][d], sizeof(double) * n, e->stream));
+        HMG_HIP(hipMallocAsync((void**)&data->point_ids_d[i], sizeof(unsigned) * n, e->stream));
+        HMG_HIP(hipMallocAsync((void**)&data->max_per_dim_d[i], sizeof(double) * dim, e->stream));
+        HMG_HIP(hipMallocAsync((void**)&data->min_per_dim_d[i], sizeof(double) * dim, e->stream));
+        HMG_HIP(hipMallocAsync((void**)&data->coords_device[i], sizeof(double*) * dim, e->stream));
         HMG_HIP(hipMemcpyAsync(data->coords_device[i], data->coords_d[i], sizeof(double*) * dim,
                                hipMemcpyHostToDevice, e->stream));
-        HMG_HIP(hipMalloc((void**)&data->code_d[i], sizeof(uint64_t) * n));
-        HMG_HIP(hipMalloc((void**)&data->order[i], sizeof(uint64_t) * n));
+        HMG_HIP(hipMallocAsync((void**)&data->code_d[i], sizeof(uint64_t) * n, e->stream));
+        HMG_HIP(hipMallocAsync((void**)&data->order[i], sizeof(uint64_t) * n, e->stream));
         point_set ps{data->coords_device[i], dim, n, data->max_per_dim_d[i], data->min_per_dim_d[i],
                      data->point_ids_d[i]};
-        HMG_HIP(hipMalloc((void**)&data->points_d[i], sizeof(point_set)));
+        HMG_HIP(hipMallocAsync((void**)&data->points_d[i], sizeof(point_set), e->stream));
         HMG_HIP(hipMemcpyAsync(data->points_d[i], &ps, sizeof(ps), hipMemcpyHostToDevice, e->stream));
         morton_code mc{data->code_d[i], n, dim, bits, nullptr, nullptr};
-        HMG_HIP(hipMalloc((void**)&data->morton_d[i], sizeof(morton_code)));
+        HMG_HIP(hipMallocAsync((void**)&data->morton_d[i], sizeof(morton_code), e->stream));
         HMG_HIP(hipMemcpyAsync(data->morton_d[i], &mc, sizeof(mc), hipMemcpyHostToDevice, e->stream));
         HMG_HIP(hipStreamSynchronize(e->stream));
     }
@@ -351,6 +384,7 @@ hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
         set_error("max_level and c_leaf must be non-negative");
         return HMG_ERR_INVALID;
     }
+    AllocScope scope(e->stream);
     release_build(*e, data);
     hipStream_t s = e->stream;
     const int dim = data->dim;
@@ -472,6 +506,7 @@ hmg_status hmg_precompute_aca(struct h_matrix_data* data)
 {
     Engine* e;
     HMG_TRY(check_ready(data, &e));
+    AllocScope scope(e->stream);
     KernelParams kp;
     HMG_TRY(read_kernel(data, &kp));
     release_factors(*e, data);
@@ -495,6 +530,7 @@ hmg_status hmg_precompute_dense(struct h_matrix_data* data)
 {
     Engine* e;
     HMG_TRY(check_ready(data, &e));
+    AllocScope scope(e->stream);
     KernelParams kp;
     HMG_TRY(read_kernel(data, &kp));
     int64_t entries = 0;
@@ -522,6 +558,7 @@ hmg_status hmg_apply_h_matrix_mvp_multi(const double* X, double* Y, int nrhs, st
 {
     Engine* e;
     HMG_TRY(check_ready(data, &e));
+    AllocScope scope(e->stream);
     if (!X || !Y || nrhs < 1) {
         set_error("invalid x / y / nrhs");
         return HMG_ERR_INVALID;
@@ -575,6 +612,7 @@ hmg_status hmg_apply_h_matrix_mvp_without_batching(double* x, double* y, struct 
 {
     Engine* e;
     HMG_TRY(check_ready(data, &e));
+    AllocScope scope(e->stream);
     if (!x || !y) {
         set_error("invalid x / y");
         return HMG_ERR_INVALID;
@@ -617,6 +655,7 @@ hmg_status hmg_apply_full_mvp(double* x, double* y, struct h_matrix_data* data)
 {
     Engine* e;
     HMG_TRY(check_ready(data, &e));
+    AllocScope scope(e->stream);
     KernelParams kp;
     HMG_TRY(read_kernel(data, &kp));
     HMG_TRY(full_mvp(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[0], data->order[1],
@@ -630,6 +669,7 @@ hmg_status hmg_dense_rows(const double* X, int nrhs, const int64_t* rows, int nr
 {
     Engine* e;
     HMG_TRY(check_ready(data, &e));
+    AllocScope scope(e->stream);
     if (nrows <= 0) return HMG_OK;
     KernelParams kp;
     HMG_TRY(read_kernel(data, &kp));
@@ -647,6 +687,7 @@ hmg_status hmg_set_gaussian_kernel_rhs(double* b, struct h_matrix_data* data)
 {
     Engine* e;
     HMG_TRY(check_ready(data, &e));
+    AllocScope scope(e->stream);
     HMG_TRY(gaussian_rhs(*e, coords_of(data, 0), data->order[0], b));
     HMG_HIP(hipStreamSynchronize(e->stream));
     return HMG_OK;
@@ -736,7 +777,7 @@ int64_t engine_bytes(const Engine& e)
 {
     int64_t t = 0;
     t += bytes_of(e.perm32[0]) + bytes_of(e.perm32[1]) + bytes_of(e.blocks) + bytes_of(e.blk);
-    t += bytes_of(e.kb) + bytes_of(e.keff) + bytes_of(e.ldv) + bytes_of(e.ldu);
+    t += bytes_of(e.kb) + bytes_of(e.keff) + bytes_of(e.ldv) + bytes_of(e.ldu) + bytes_of(e.aca_list);
     t += bytes_of(e.uoff) + bytes_of(e.voff) + bytes_of(e.toff) + bytes_of(e.U) + bytes_of(e.V);
     t += bytes_of(e.aoff) + bytes_of(e.A) + bytes_of(e.near_items) + bytes_of(e.near_blk);
     t += bytes_of(e.fly_items) + bytes_of(e.fly_refs) + bytes_of(e.lr_items) + bytes_of(e.lv_items);
@@ -775,7 +816,8 @@ hmg_status hmg_memory(const struct h_matrix_data* data, int64_t out[2])
         // factor elements of the current layout; the mixed layout pads to 32 floats
         // instead of 16 doubles
         const double elems = (double)(e->h_uoff[na] + e->h_voff[na]);
-        add += (int64_t)(elems * (mixed ? 4.0 * 1.03 : 8.0));
+        // mixed: FP32 factors plus the FP64 factors the ACA computes them from
+        add += (int64_t)(elems * (mixed ? 12.0 * 1.03 : 8.0));
     }
     if (mixed && e->layout_prec != HMG_PREC_MIXED) {
         const int64_t n1 = e->n[0], n2 = e->n[1];
@@ -842,6 +884,16 @@ hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value)
         }
         e->precision = (int)value;
         return HMG_OK;
+    case HMG_OPT_ACA_ROW_SCAN:
+        e->aca_row_scan = value != 0;
+        return HMG_OK;
+    case HMG_OPT_ACA_BIG_BLOCK:
+        if (value < 0 || value > (int64_t)1 << 30) {
+            set_error("ACA workgroup threshold must be in [0, 2^30]");
+            return HMG_ERR_INVALID;
+        }
+        e->aca_big_entries = (int)value;
+        return HMG_OK;
     default:
         set_error("unknown option " + std::to_string(option));
         return HMG_ERR_INVALID;
@@ -893,33 +945,37 @@ hmg_status hmg_destroy_h_matrix_data(struct h_matrix_data* data)
 {
     Engine* e = engine_of(data);
     if (!e) return HMG_OK;
-    release_build(*e, data);
-    for (int i = 0; i < 2; i++) {
-        for (int d = 0; d < data->dim; d++) (void)hipFree(data->coords_d[i][d]);
-        free(data->coords_d[i]);
-        data->coords_d[i] = nullptr;
-        (void)hipFree(data->point_ids_d[i]);
-        (void)hipFree(data->max_per_dim_d[i]);
-        (void)hipFree(data->min_per_dim_d[i]);
-        (void)hipFree(data->coords_device[i]);
-        (void)hipFree(data->code_d[i]);
-        (void)hipFree(data->order[i]);
-        (void)hipFree(data->points_d[i]);
-        (void)hipFree(data->morton_d[i]);
-        data->point_ids_d[i] = nullptr;
-        data->max_per_dim_d[i] = data->min_per_dim_d[i] = nullptr;
-        data->coords_device[i] = nullptr;
-        data->code_d[i] = nullptr;
-        data->order[i] = nullptr;
-        data->points_d[i] = nullptr;
-        data->morton_d[i] = nullptr;
-        e->perm32[i].release();
+    // finish the engine's work, then return every buffer on its own stream
+    // (stream-ordered frees do not wait for other H-matrices' kernels)
+    (void)hipStreamSynchronize(e->stream2);
+    (void)hipStreamSynchronize(e->stream);
+    hipStream_t s = e->stream;
+    {
+        AllocScope scope(s);
+        release_build(*e, data);
+        for (int i = 0; i < 2; i++) {
+            for (int d = 0; d < data->dim; d++) (void)hipFreeAsync(data->coords_d[i][d], s);
+            free(data->coords_d[i]);
+            data->coords_d[i] = nullptr;
+            for (void* p : {(void*)data->point_ids_d[i], (void*)data->max_per_dim_d[i],
+                            (void*)data->min_per_dim_d[i], (void*)data->coords_device[i], (void*)data->code_d[i],
+                            (void*)data->order[i], (void*)data->points_d[i], (void*)data->morton_d[i]})
+                (void)hipFreeAsync(p, s);
+            data->point_ids_d[i] = nullptr;
+            data->max_per_dim_d[i] = data->min_per_dim_d[i] = nullptr;
+            data->coords_device[i] = nullptr;
+            data->code_d[i] = nullptr;
+            data->order[i] = nullptr;
+            data->points_d[i] = nullptr;
+            data->morton_d[i] = nullptr;
+            e->perm32[i].release();
+        }
+        release_scratch(e->stream);
+        release_scratch(e->stream2);
     }
     e->prof.clear();
-    (void)hipStreamSynchronize(e->stream);
     (void)hipStreamSynchronize(e->stream2);
-    release_scratch(e->stream);
-    release_scratch(e->stream2);
+    (void)hipStreamSynchronize(e->stream);
     (void)hipEventDestroy(e->ev_fork);
     (void)hipEventDestroy(e->ev_join);
     (void)hipStreamDestroy(e->stream2);

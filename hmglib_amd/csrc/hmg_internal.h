@@ -63,29 +63,53 @@ struct Coords {
 };
 
 // ---- device buffer helper --------------------------------------------------
+// Stream-ordered allocation: inside an AllocScope, DevBuf memory comes from the
+// device's memory pool ordered on the scope's stream (hipMallocAsync) and is
+// returned on the stream it was allocated on (hipFreeAsync).  hipFree implies a
+// device-wide synchronization, which would make every H-matrix served
+// concurrently wait for the kernels of all others whenever one frees a buffer.
+// Outside a scope DevBuf uses plain hipMalloc / hipFree.
+extern thread_local hipStream_t g_alloc_stream;
+struct AllocScope {
+    hipStream_t prev;
+    explicit AllocScope(hipStream_t s) : prev(g_alloc_stream) { g_alloc_stream = s; }
+    ~AllocScope() { g_alloc_stream = prev; }
+    AllocScope(const AllocScope&) = delete;
+    AllocScope& operator=(const AllocScope&) = delete;
+};
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    hipStream_t s = nullptr;  // allocation stream (nullptr: hipMalloc)
     hmg_status alloc(size_t count) {
         if (count <= n && p) return HMG_OK;
         release();
         if (count == 0) count = 1;
-        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        hipStream_t st = g_alloc_stream;
+        hipError_t e = st ? hipMallocAsync((void**)&p, count * sizeof(T), st) : hipMalloc(&p, count * sizeof(T));
         if (e != hipSuccess) {
             p = nullptr;
             n = 0;
-            set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+            set_error(std::string("device allocation of ") + std::to_string(count * sizeof(T)) +
                       " bytes failed: " + hipGetErrorString(e));
             return HMG_ERR_OOM;
         }
         n = count;
+        s = st;
         return HMG_OK;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (s)
+                (void)hipFreeAsync(p, s);
+            else
+                (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
+        s = nullptr;
     }
 };
 
@@ -231,6 +255,8 @@ struct Engine {
     std::vector<int> h_kb, h_ldv, h_ldu;
     std::vector<int64_t> h_uoff, h_voff, h_toff; // per ACA block (+ total at [naca])
     DevBuf<int> kb, keff, ldv, ldu;
+    DevBuf<int> aca_list;                       // ACA blocks: large (workgroup each) first
+    std::vector<int> h_aca_list;
     DevBuf<int64_t> uoff, voff, toff;
     DevBuf<double> U, V;
     int kmax = 0;
@@ -290,6 +316,10 @@ struct Engine {
     // product vs 46.2 ms with 3 workgroups per CU and 57.9 ms without overlap)
     int near_lds_pad = 64 * 1024;
     bool serialize = false;
+    // ACA: scan rejected rows of the residual-row search several at a time;
+    // blocks with more rows plus columns than aca_big_entries get a workgroup
+    bool aca_row_scan = true;
+    int aca_big_entries = 4096;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     // ---- mixed precision ------------------------------------------------------
@@ -360,11 +390,11 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
 
 // ---- ACA (aca.hip) -------------------------------------------------------------
 // tie_last: 1 = last argmax (batched path), 0 = first argmax (unbatched path).
-// The factors are stored as T (double, or float for mixed precision); residuals,
-// norms and pivot scaling are computed in FP64 either way.
-template <class T>
+// Factors in FP64; mixed precision rounds them to FP32 afterwards (round_to_f32).
 hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
-                     int k, int tie_last, T* U, T* V, int* keff);
+                     int k, int tie_last, double* U, double* V, int* keff);
+// dst[i] = (float)src[i], i < n, on the engine stream
+hmg_status round_to_f32(Engine& eng, const double* src, float* dst, int64_t n);
 
 // ---- schedule (schedule.hip, host) ---------------------------------------------------
 // row-cluster groups and near-field work items (once per tree); the matrix-free

@@ -83,6 +83,9 @@ struct hmg_server {
     int64_t next_req = 1, next_mat = 1;
     int64_t n_evict = 0, n_done = 0, n_failed = 0, n_passes = 0, n_shared = 0, n_builds = 0;
     int64_t outstanding = 0;          // submitted, not completed
+    // kernel descriptors of evicted H-matrices, reused by later builds (hipFree
+    // would synchronize the whole device while other requests run)
+    std::vector<system_assembler*> free_kernels;
 
     double now() const { return std::chrono::duration<double>(hmg::Clock::now() - t0).count(); }
 };
@@ -102,16 +105,51 @@ void finish(hmg_server* s, Request* r, int status, double t_end)
     s->outstanding--;
 }
 
-// free a matrix's device memory; caller holds no lock, the matrix is no longer
-// reachable for new work
-void destroy_matrix(Matrix* m)
+// free a matrix's device memory (stream-ordered on its engine's stream); the
+// caller holds no lock and the matrix is no longer reachable for new work; its
+// kernel descriptor goes back to the server's free list
+void destroy_matrix(hmg_server* s, Matrix* m)
 {
-    if (m->inited) (void)hmg_destroy_h_matrix_data(&m->data);
-    m->inited = false;
-    if (m->kern) hmg_kernel_free(m->kern);
-    m->kern = nullptr;
     m->Xb.release();
     m->Yb.release();
+    if (m->inited) (void)hmg_destroy_h_matrix_data(&m->data);
+    m->inited = false;
+    if (m->kern) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->free_kernels.push_back(m->kern);
+    }
+    m->kern = nullptr;
+}
+
+// a kernel descriptor holding (kind, regularization, length scale), reusing a
+// freed one when available; written on `st`
+system_assembler* kernel_for(hmg_server* s, const hmg_build_request& b, hipStream_t st)
+{
+    system_assembler* k = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!s->free_kernels.empty()) {
+            k = s->free_kernels.back();
+            s->free_kernels.pop_back();
+        }
+    }
+    if (!k) return hmg_kernel_new(b.kernel_kind, b.regularization, b.length_scale);
+    if (b.kernel_kind < HMG_KERNEL_GAUSSIAN_REF || b.kernel_kind > HMG_KERNEL_EXPONENTIAL) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->free_kernels.push_back(k);
+        set_error("unknown kernel kind");
+        return nullptr;
+    }
+    system_assembler h{};
+    h.kind = b.kernel_kind;
+    h.regularization = b.regularization;
+    h.length_scale = b.length_scale;
+    if (hipMemcpyAsync(k, &h, sizeof(h), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("kernel descriptor upload failed");
+        return nullptr;
+    }
+    return k;
 }
 
 // the job a free slot takes: the matrix whose oldest waiting request is oldest
@@ -202,7 +240,7 @@ hmg_status run_build(hmg_server* s, Matrix* m, Request* r, std::unique_lock<std:
     }
     if (st == HMG_OK) st = hmg_setup_h_matrix(&d);
     if (st == HMG_OK) {
-        m->kern = hmg_kernel_new(b.kernel_kind, b.regularization, b.length_scale);
+        m->kern = kernel_for(s, b, engine_of(&d)->stream);
         if (!m->kern) st = HMG_ERR_INVALID;
         d.assem = m->kern;
     }
@@ -216,7 +254,7 @@ hmg_status run_build(hmg_server* s, Matrix* m, Request* r, std::unique_lock<std:
         std::vector<std::unique_ptr<Matrix>> victims = make_room(s, need);
         if (!victims.empty()) {
             lk.unlock();
-            for (auto& v : victims) destroy_matrix(v.get());
+            for (auto& v : victims) destroy_matrix(s, v.get());
             victims.clear();
             lk.lock();
         }
@@ -245,6 +283,7 @@ hmg_status run_pass(hmg_server* s, Matrix* m, std::vector<Request*>& batch, int 
     lk.unlock();
     hmg_status st = HMG_OK;
     Engine* e = engine_of(&m->data);
+    AllocScope scope(e->stream);  // merged pass buffers are stream-ordered
     const int64_t n1 = m->data.point_count[0], n2 = m->data.point_count[1];
     const double* X = batch[0]->X;
     double* Y = batch[0]->Y;
@@ -350,7 +389,7 @@ void worker(hmg_server* s)
         s->cv_work.notify_all();
         if (drop) {
             lk.unlock();
-            destroy_matrix(drop.get());
+            destroy_matrix(s, drop.get());
             lk.lock();
         }
     }
@@ -402,7 +441,8 @@ hmg_status hmg_server_destroy(hmg_server* s)
     }
     s->cv_work.notify_all();
     for (auto& t : s->workers) t.join();
-    for (auto& kv : s->mats) destroy_matrix(kv.second.get());
+    for (auto& kv : s->mats) destroy_matrix(s, kv.second.get());
+    for (system_assembler* k : s->free_kernels) hmg_kernel_free(k);
     delete s;
     return HMG_OK;
 }
@@ -511,7 +551,7 @@ hmg_status hmg_server_release(hmg_server* s, int64_t matrix)
             s->mats.erase(it);
         }
     }
-    if (drop) destroy_matrix(drop.get());
+    if (drop) destroy_matrix(s, drop.get());
     s->cv_done.notify_all();
     return HMG_OK;
 }

@@ -155,16 +155,26 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
  *   caps its occupancy so the low-rank streams keep wave slots beside it.
  * HMG_OPT_SERIALIZE [0]: run the near field on the main stream (no overlap).
  * HMG_OPT_PRECISION [HMG_PREC_FP64]: arithmetic of the next precompute_aca and
- *   of every product.  HMG_PREC_MIXED stores the ACA factors in FP32 (residuals,
- *   norms and pivot scaling in FP64), runs the low-rank products as FP32 MFMA
- *   GEMMs and evaluates the near field in FP32, multiplied as BF16 hi+lo split
- *   MFMA GEMMs accumulated in FP32, for up to 32 right-hand sides per pass.
+ *   of every product.  HMG_PREC_MIXED computes the ACA in FP64 exactly as
+ *   HMG_PREC_FP64 does and stores its factors rounded to FP32, runs the low-rank
+ *   products as FP32 MFMA GEMMs and evaluates the near field in FP32, multiplied
+ *   as BF16 hi+lo split MFMA GEMMs accumulated in FP32, for up to 32 right-hand
+ *   sides per pass.
  *   X and Y stay FP64 at the boundary.  In this mode data->U and data->V point at
- *   the FP32 factors (float arrays) and a stored FP64 near field is not used. */
+ *   the FP32 factors (float arrays) and a stored FP64 near field is not used.
+ * HMG_OPT_ACA_ROW_SCAN [1]: when the residual-row search of the ACA rejects a
+ *   row, compute the norms of the following rows four at a time (rank steps up
+ *   to 64) instead of one at a time.  The factors are bitwise the same.
+ * HMG_OPT_ACA_BIG_BLOCK [4096]: admissible blocks with more rows plus columns
+ *   than this run their ACA on a workgroup of 8 wavefronts instead of one
+ *   wavefront (row norms summed in another order; same factors up to the
+ *   rounding of those norms). */
 #define HMG_OPT_NEAR_SYMMETRIC 1
 #define HMG_OPT_NEAR_LDS_BYTES 2
 #define HMG_OPT_SERIALIZE 3
 #define HMG_OPT_PRECISION 4
+#define HMG_OPT_ACA_ROW_SCAN 5
+#define HMG_OPT_ACA_BIG_BLOCK 6
 #define HMG_PREC_FP64 0
 #define HMG_PREC_MIXED 1
 hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value);

@@ -265,3 +265,79 @@ def test_same_sets_detection(gpu):
     h2 = gpu.HMatrix(X1, X1.copy(), 32, 1.5, 50, 64, 8)
     h2.setup()
     assert h2.schedule()["same_sets"]
+
+
+@pytest.mark.parametrize("k,ls,prec", [(32, 1.0, "fp64"), (100, 0.3, "fp64"), (32, 0.05, "fp64"),
+                                       (32, 0.05, "mixed")])
+def test_aca_row_scan_same_factors(gpu, k, ls, prec):
+    """Gaussian kernels where many admissible blocks reach numerical rank zero
+    before step k (smooth ones), or have entries below 1e-13 altogether (narrow
+    ones, which also holds with FP32 factors), so the residual-row search rejects
+    long runs of rows.  Those are scanned four rows at a time for rank steps up
+    to 64 and one at a time beyond (with k = 100 some blocks stop between steps
+    64 and 100); the factors must be bitwise those of the one-row search, so
+    ranks and products are equal.  Against the oracle (glibc exp, another
+    summation order of the row norms) FP64 rank decisions at the 1e-13 threshold
+    may differ in a few blocks only."""
+    X = _points(4101, 2, 3000)
+    om = O.HMatrix(X, X, 32, 1.5, 50, 64, 0, 0)
+    ok = O.kernel(O.K_GAUSSIAN, 2, 0.0, ls)
+    om.aca(ok, k)
+    f = om.aca_factors()
+    keff = f["keff"]
+    assert keff.sum() < 0.8 * f["kb"].sum()
+    if k > 64:
+        assert ((keff > 64) & (keff < f["kb"])).any()
+    x = O.normal(4103, 3000)
+    res = []
+    for scan in (0, 1):
+        hm, _, _, _ = _pair(gpu, X, X, 32, 1.5, 64, k, O.K_GAUSSIAN, ls, precompute=False)
+        hm.set_option(gpu.OPT_ACA_ROW_SCAN, scan)
+        hm.set_precision(gpu.PREC_MIXED if prec == "mixed" else gpu.PREC_FP64)
+        hm.precompute_aca()
+        res.append((hm.work(), hm.apply(x)))
+    (w0, y0), (w1, y1) = res
+    assert w0["rank_sum"] == w1["rank_sum"] and w0["E_lr"] == w1["E_lr"]
+    np.testing.assert_array_equal(y0, y1)
+    if prec == "fp64":
+        assert abs(w1["rank_sum"] - int(keff.sum())) <= 1e-3 * keff.sum()
+        assert _rel(y1, om.hmvp(ok, x)) < TOL_FP64
+    else:
+        assert w1["rank_sum"] < 0.95 * f["kb"].sum()
+        assert _rel(y1, om.hmvp(ok, x)) < 1e-4
+
+
+@pytest.mark.parametrize("kind,ls", [(O.K_EXPONENTIAL, 0.1), (O.K_GAUSSIAN, 0.2)])
+def test_aca_large_blocks_workgroup_path(gpu, kind, ls):
+    """N = 40000 in 2D gives admissible blocks with more than 4096 rows plus
+    columns, which run on a whole workgroup each.  Ranks and product match the
+    oracle and the all-wavefront run (threshold raised past every block), in FP64
+    and in mixed precision."""
+    n = 40000
+    X = _points(4201, 2, n)
+    hm, _, om, ok = _pair(gpu, X, X, 32, 1.5, 64, 16, kind, ls)
+    om.aca(ok, 16)
+    f = om.aca_factors()
+    blocks = hm.blocks(O.WORK_ITEM_DTYPE)
+    aca = blocks[blocks["work_type"] == O.WT_ACA]
+    sizes = (aca["set1_u"] - aca["set1_l"] + 1) + (aca["set2_u"] - aca["set2_l"] + 1)
+    assert (sizes > 4096).sum() >= 16
+    w = hm.work()
+    assert w["rank_sum"] == int(f["keff"].sum()) and w["E_lr"] == om.work()[1]
+    x = O.normal(4203, n)
+    y = hm.apply(x)
+    assert _rel(y, om.hmvp(ok, x)) < TOL_FP64
+    hm.set_option(gpu.OPT_ACA_BIG_BLOCK, 1 << 30)
+    hm.precompute_aca()
+    assert _rel(hm.apply(x), y) < TOL_FP64
+    hm.set_precision(gpu.PREC_MIXED)
+    hm.precompute_aca()
+    ym_wave = hm.apply(x)
+    hm.set_option(gpu.OPT_ACA_BIG_BLOCK, 4096)
+    hm.precompute_aca()
+    ym = hm.apply(x)
+    assert _rel(ym, ym_wave) < 1e-6
+    # mixed precision: FP32 factors change late pivots, so the product is held to
+    # the FP64 H-matrix's own approximation error against the dense product
+    y_dense = hm.apply_full(x)
+    assert _rel(ym, y_dense) < _rel(y, y_dense) + 2e-5
