@@ -52,13 +52,22 @@ def kernel_roofline(hm, prof, nrhs_pass, mixed, traffic_name):
     sch = hm.schedule()
     R = nrhs_pass
     s_f = 4 if mixed else 8          # bytes per factor / vector element
-    # algorithmic bytes per launch (R right-hand sides per launch)
-    #   lr_vtx: V once (s_f x sum m2*rank) + X~_J once per block (s_f x R x sum m2)
-    #           + T written (s_f x R x sum rank)
-    #   lr_u:   U once (s_f x sum m1*rank) + T once per 64-row item (s_f x R x
-    #           sum K over items) + partial row sums written (s_f x R x rows)
-    bytes_vtx = s_f * int(w["lr_m2_rank"] + R * (sch["lr_m2"] + w["rank_sum"]))
-    bytes_lru = s_f * int(w["lr_m1_rank"] + R * (sch["lr_t_reads"] + sch["lr_rows"]))
+    # distinct row and column clusters of the admissible blocks
+    import hmglib_amd
+    blk = hm.blocks(np.dtype(hmglib_amd.WorkItem))
+    aca = blk[blk["work_type"] == hmglib_amd.WT_ACA]
+    rows_c = np.unique(np.stack([aca["set1_l"], aca["set1_u"]], 1), axis=0)
+    cols_c = np.unique(np.stack([aca["set2_l"], aca["set2_u"]], 1), axis=0)
+    sum_m1_c = int((rows_c[:, 1] - rows_c[:, 0] + 1).sum())
+    sum_m2_c = int((cols_c[:, 1] - cols_c[:, 0] + 1).sum())
+    # algorithmic bytes per launch (R right-hand sides per launch): every factor
+    # element once, every vector entry a cluster needs once per cluster
+    #   lr_vtx: V (s_f x sum m2*rank) + X of each column cluster (s_f x R x
+    #           sum_D m2(D)) + T written (s_f x R x sum rank)
+    #   lr_u:   U (s_f x sum m1*rank) + T (s_f x R x sum rank) + the partial row
+    #           sums of each row cluster written (s_f x R x sum_C m1(C))
+    bytes_vtx = s_f * int(w["lr_m2_rank"] + R * (sum_m2_c + w["rank_sum"]))
+    bytes_lru = s_f * int(w["lr_m1_rank"] + R * (w["rank_sum"] + sum_m1_c))
     flops_lr_v = 2 * R * int(w["lr_m2_rank"])
     flops_lr_u = 2 * R * int(w["lr_m1_rank"])
     evals = int(sch["near_evals"]) or e_near
@@ -90,6 +99,7 @@ def kernel_roofline(hm, prof, nrhs_pass, mixed, traffic_name):
                  "TFLOPs": flops_near / (t["near"] * 1e-3) / 1e12,
                  "Geval_s": evals / (t["near"] * 1e-3) / 1e9},
         "gather_ms": t["gather"], "combine_ms": t["combine"],
+        "sum_m1_row_clusters": sum_m1_c, "sum_m2_col_clusters": sum_m2_c,
     }
     dominant = max(("lr_vtx", "lr_u", "near"), key=lambda k: t[k])
     if mixed:
@@ -448,11 +458,11 @@ def main():
     config = {"workload": cfg["desc"], "n": n, "nrhs": R, "build_s": {"setup": t_setup},
               "blocks": {"dense": w["dense"], "aca": w["aca"]}, "E_near": w["E_near"], "E_lr": w["E_lr"]}
     if mixed:
-        config.update({"precision": "mixed: FP32 factors and low-rank MFMA, BF16x3 near-field MFMA, "
-                                    "FP64 ACA residuals",
+        config.update({"precision": "mixed: FP64 ACA, factors rounded to FP32, FP32 MFMA low rank, "
+                                    "BF16x3 MFMA near field",
                        "rel_err_fp64_rows_per_rhs_max": max(result["errs"]),
                        "rel_err_fp64_rows_per_rhs": result["errs"]})
-        config["build_s"]["aca_fp32"] = result["aca_s"]
+        config["build_s"]["aca"] = result["aca_s"]
         if plain is not None:
             config["plain_fp64"] = {"H-MVP/s": plain["value"], "ms_per_step": plain["ms_per_step"],
                                     "steps": plain["steps"], "nrhs": 1,

@@ -18,8 +18,6 @@ namespace hmg {
 
 namespace {
 
-constexpr int kAcaThreads = 256;
-constexpr int kAcaWaves = kAcaThreads / kWave;
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -330,42 +328,43 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
 }
 
 // one wavefront per block of list[] (indices into the ACA blocks)
-// four wavefronts per SIMD: the rank loop is latency-bound, so occupancy pays
+// One launch for every admissible block: nbig workgroups each run the ACA of one
+// large block (list[0 .. nbig)) with all kAcaTeam wavefronts; every other
+// workgroup runs kAcaTeam small blocks (list[nbig ..)), one per wavefront.  Four
+// wavefronts per SIMD: the rank loop is latency-bound, so occupancy pays.
+constexpr int kAcaTeam = 4;
 template <class T, int KIND>
-__global__ void __launch_bounds__(kAcaThreads) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kAcaTeam * kWave) __attribute__((amdgpu_waves_per_eu(4)))
 aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, const int* __restrict__ list,
-           int nlist, const int* __restrict__ kb, const int64_t* __restrict__ uoff, const int* __restrict__ ldu,
-           const int64_t* __restrict__ voff, const int* __restrict__ ldv, T* __restrict__ U,
-           T* __restrict__ V, int* __restrict__ keff, int tie_last, int scan)
+           int nbig, int nsmall, const int* __restrict__ kb, const int64_t* __restrict__ uoff,
+           const int* __restrict__ ldu, const int64_t* __restrict__ voff, const int* __restrict__ ldv,
+           T* __restrict__ U, T* __restrict__ V, int* __restrict__ keff, int tie_last, int scan)
 {
-    const int q = blockIdx.x * kAcaWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    if (q >= nlist) return;
-    const int a = list[q];
-    __shared__ T sU[kAcaWaves][kScanRows * kScanL];
-    __shared__ double sX[kAcaWaves][kScanRows * MAX_POINT_DIMENSION];
-    const Team<1> team{nullptr, nullptr, nullptr};
-    aca_block<T, 1, KIND>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a], keff + a,
-                          tie_last, scan, sU[threadIdx.x / kWave], sX[threadIdx.x / kWave]);
-}
-
-// one workgroup of kAcaBigWaves wavefronts per block of list[]
-constexpr int kAcaBigWaves = 8;
-template <class T, int KIND>
-__global__ void __launch_bounds__(kAcaBigWaves * kWave)
-aca_big_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, const int* __restrict__ list,
-               const int* __restrict__ kb, const int64_t* __restrict__ uoff, const int* __restrict__ ldu,
-               const int64_t* __restrict__ voff, const int* __restrict__ ldv, T* __restrict__ U,
-               T* __restrict__ V, int* __restrict__ keff, int tie_last, int scan)
-{
-    __shared__ double s_v[kAcaBigWaves * kScanRows];
-    __shared__ int s_i[kAcaBigWaves];
-    __shared__ double s_x[kAcaBigWaves];
-    const int a = list[blockIdx.x];
-    __shared__ T sU[kAcaBigWaves][kScanRows * kScanL];
-    __shared__ double sX[kAcaBigWaves][kScanRows * MAX_POINT_DIMENSION];
-    const Team<kAcaBigWaves> team{s_v, s_i, s_x};
-    aca_block<T, kAcaBigWaves, KIND>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
-                                     keff + a, tie_last, scan, sU[threadIdx.x / kWave], sX[threadIdx.x / kWave]);
+    __shared__ double s_v[kAcaTeam * kScanRows];
+    __shared__ int s_i[kAcaTeam];
+    __shared__ double s_x[kAcaTeam];
+    __shared__ T sU[kAcaTeam][kScanRows * kScanL];
+    __shared__ double sX[kAcaTeam][kScanRows * MAX_POINT_DIMENSION];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    // the nbig large-block workgroups are spread evenly over the grid, so large
+    // and small blocks run side by side: workgroup i is the large one number
+    // floor(i nbig / G) when that count steps at i, else small one number
+    // i - floor(i nbig / G)
+    const int64_t G = (int64_t)gridDim.x, i = blockIdx.x;
+    const int bi = (int)(i * nbig / G), bi1 = (int)((i + 1) * nbig / G);
+    if (bi1 != bi) {
+        const int a = list[bi];
+        const Team<kAcaTeam> team{s_v, s_i, s_x};
+        aca_block<T, kAcaTeam, KIND>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
+                                     keff + a, tie_last, scan, sU[w], sX[w]);
+    } else {
+        const int q = ((int)i - bi) * kAcaTeam + w;
+        if (q >= nsmall) return;
+        const int a = list[nbig + q];
+        const Team<1> team{nullptr, nullptr, nullptr};
+        aca_block<T, 1, KIND>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
+                              keff + a, tie_last, scan, sU[w], sX[w]);
+    }
 }
 
 }  // namespace
@@ -377,9 +376,8 @@ static hmg_status aca_build_t(Engine& eng, const Coords& c1, const Coords& c2, c
     (void)k;
     const int na = eng.naca;
     if (na == 0) return HMG_OK;
-    hipStream_t s = eng.stream, s2 = eng.stream2;
-    // large blocks (a workgroup each, on the second stream) and the rest (a
-    // wavefront each); the large ones first in the list
+    hipStream_t s = eng.stream;
+    // large blocks (a workgroup each) first, then the rest (a wavefront each)
     std::vector<int>& order = eng.h_aca_list;
     order.clear();
     order.reserve(na);
@@ -398,23 +396,11 @@ static hmg_status aca_build_t(Engine& eng, const Coords& c1, const Coords& c2, c
     const int scan = eng.aca_row_scan ? 1 : 0;
     const int4* blk = eng.blk.p + eng.ndense;
     eng.prof.begin(PROF_ACA, s);
-    HMG_HIP(hipEventRecord(eng.ev_fork, s));
-    if (nbig > 0) {
-        HMG_HIP(hipStreamWaitEvent(s2, eng.ev_fork, 0));
-        HMG_DISPATCH_KIND(kp.kind, aca_big_kernel<T, KIND><<<nbig, kAcaBigWaves * kWave, 0, s2>>>(
-                                       c1, c2, kp, blk, eng.aca_list.p, eng.kb.p, eng.uoff.p, eng.ldu.p,
-                                       eng.voff.p, eng.ldv.p, U, V, keff, tie_last, scan));
-        HMG_HIP(hipGetLastError());
-        HMG_HIP(hipEventRecord(eng.ev_join, s2));
-    }
-    if (nsmall > 0) {
-        const unsigned grid = (unsigned)((nsmall + kAcaWaves - 1) / kAcaWaves);
-        HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND><<<grid, kAcaThreads, 0, s>>>(
-                                       c1, c2, kp, blk, eng.aca_list.p + nbig, nsmall, eng.kb.p, eng.uoff.p,
-                                       eng.ldu.p, eng.voff.p, eng.ldv.p, U, V, keff, tie_last, scan));
-        HMG_HIP(hipGetLastError());
-    }
-    if (nbig > 0) HMG_HIP(hipStreamWaitEvent(s, eng.ev_join, 0));
+    const unsigned grid = (unsigned)(nbig + (nsmall + kAcaTeam - 1) / kAcaTeam);
+    HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND><<<grid, kAcaTeam * kWave, 0, s>>>(
+                                   c1, c2, kp, blk, eng.aca_list.p, nbig, nsmall, eng.kb.p, eng.uoff.p,
+                                   eng.ldu.p, eng.voff.p, eng.ldv.p, U, V, keff, tie_last, scan));
+    HMG_HIP(hipGetLastError());
     eng.prof.end(PROF_ACA, s);
     return HMG_OK;
 }

@@ -294,6 +294,21 @@ const char* hmg_last_error(void) { return g_error.c_str(); }
 // ------------------------------------------------------------------ init
 hmg_status hmg_init_h_matrix_data(struct h_matrix_data* data, int point_count[2], int dim, int bits)
 {
+    return engine_init(data, point_count, dim, bits, nullptr);
+}
+
+}  // extern "C"
+
+namespace hmg {
+
+void engine_bind_stream(Engine* e, hipStream_t s, h_matrix_data* data)
+{
+    e->stream = e->stream2 = s;
+    data->magma_queue = (void*)s;
+}
+
+hmg_status engine_init(h_matrix_data* data, int point_count[2], int dim, int bits, hipStream_t ext)
+{
     if (!data || !point_count) {
         set_error("null argument");
         return HMG_ERR_INVALID;
@@ -311,10 +326,16 @@ hmg_status hmg_init_h_matrix_data(struct h_matrix_data* data, int point_count[2]
         return HMG_ERR_INVALID;
     }
     Engine* e = new Engine();
-    // non-blocking streams: no implicit ordering with the null stream, so the work
-    // of independent H-matrices driven from different host threads overlaps
-    HMG_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-    HMG_HIP(hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking));
+    if (ext) {
+        // work goes to a stream the caller owns (and may rebind between jobs)
+        e->own_streams = false;
+        e->stream = e->stream2 = ext;
+    } else {
+        // non-blocking streams: no implicit ordering with the null stream, so the
+        // work of independent H-matrices driven from different host threads overlaps
+        HMG_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+        HMG_HIP(hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking));
+    }
     HMG_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
     HMG_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
     e->dim = dim;
@@ -371,6 +392,10 @@ hmg_status hmg_init_h_matrix_data(struct h_matrix_data* data, int point_count[2]
     }
     return HMG_OK;
 }
+
+}  // namespace hmg
+
+extern "C" {
 
 // ------------------------------------------------------------------ setup
 hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
@@ -946,9 +971,12 @@ hmg_status hmg_destroy_h_matrix_data(struct h_matrix_data* data)
     Engine* e = engine_of(data);
     if (!e) return HMG_OK;
     // finish the engine's work, then return every buffer on its own stream
-    // (stream-ordered frees do not wait for other H-matrices' kernels)
-    (void)hipStreamSynchronize(e->stream2);
-    (void)hipStreamSynchronize(e->stream);
+    // (stream-ordered frees do not wait for other H-matrices' kernels).  With a
+    // caller-owned stream the caller has finished this H-matrix's work already.
+    if (e->own_streams) {
+        (void)hipStreamSynchronize(e->stream2);
+        (void)hipStreamSynchronize(e->stream);
+    }
     hipStream_t s = e->stream;
     {
         AllocScope scope(s);
@@ -970,16 +998,24 @@ hmg_status hmg_destroy_h_matrix_data(struct h_matrix_data* data)
             data->morton_d[i] = nullptr;
             e->perm32[i].release();
         }
-        release_scratch(e->stream);
-        release_scratch(e->stream2);
+        // scratch of caller-owned streams stays with their owner (another job may
+        // be using it right now)
+        if (e->own_streams) {
+            release_scratch(e->stream);
+            release_scratch(e->stream2);
+        }
     }
     e->prof.clear();
-    (void)hipStreamSynchronize(e->stream2);
-    (void)hipStreamSynchronize(e->stream);
+    if (e->own_streams) {
+        (void)hipStreamSynchronize(e->stream2);
+        (void)hipStreamSynchronize(e->stream);
+    }
     (void)hipEventDestroy(e->ev_fork);
     (void)hipEventDestroy(e->ev_join);
-    (void)hipStreamDestroy(e->stream2);
-    (void)hipStreamDestroy(e->stream);
+    if (e->own_streams) {
+        (void)hipStreamDestroy(e->stream2);
+        (void)hipStreamDestroy(e->stream);
+    }
     delete e;
     data->engine = nullptr;
     data->magma_queue = nullptr;

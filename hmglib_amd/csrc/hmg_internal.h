@@ -318,6 +318,9 @@ struct Engine {
     bool serialize = false;
     // ACA: scan rejected rows of the residual-row search several at a time;
     // blocks with more rows plus columns than aca_big_entries get a workgroup
+    // false: stream == stream2 belongs to the caller (the serving loop's slot
+    // streams), which rebinds it per job and keeps its scratch buffers
+    bool own_streams = true;
     bool aca_row_scan = true;
     int aca_big_entries = 4096;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -387,6 +390,14 @@ struct TreeParams {
 // Builds the DENSE-first block list into eng.blocks / eng.blk / eng.h_blk.
 hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uint64_t* codes1,
                       const uint64_t* codes2, const TreeParams& tp);
+
+// ---- engine set-up (api.hip) ---------------------------------------------------
+// init_h_matrix_data; ext != nullptr runs all of the H-matrix's work on that
+// caller-owned stream (no near-field / low-rank overlap inside one product)
+hmg_status engine_init(h_matrix_data* data, int point_count[2], int dim, int bits, hipStream_t ext);
+// moves a caller-owned-stream engine to another stream of the caller (between
+// jobs, with no work of this H-matrix in flight)
+void engine_bind_stream(Engine* e, hipStream_t s, h_matrix_data* data);
 
 // ---- ACA (aca.hip) -------------------------------------------------------------
 // tie_last: 1 = last argmax (batched path), 0 = first argmax (unbatched path).

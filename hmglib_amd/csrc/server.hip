@@ -76,6 +76,10 @@ struct hmg_server {
     std::unordered_map<int64_t, std::unique_ptr<hmg::Matrix>> mats;
     std::deque<int64_t> builds;       // matrices whose build has not started
     std::vector<std::thread> workers;
+    // one stream per slot, created together with the first build: with one
+    // hardware queue per stream (HIP hands queues out round-robin) a long build
+    // on one slot never holds back the kernels of another slot's job
+    std::vector<hipStream_t> slot_streams;
     bool stop = false;
     int running = 0;                  // jobs on slots
     int waiting = 0;                  // builds waiting for memory to free up
@@ -201,13 +205,13 @@ std::vector<std::unique_ptr<Matrix>> make_room(hmg_server* s, int64_t need)
     return out;
 }
 
-hmg_status run_build(hmg_server* s, Matrix* m, Request* r, std::unique_lock<std::mutex>& lk)
+hmg_status run_build(hmg_server* s, Matrix* m, Request* r, hipStream_t slot, std::unique_lock<std::mutex>& lk)
 {
     const hmg_build_request& b = r->bp;
     lk.unlock();
     h_matrix_data& d = m->data;
     int pc[2] = {b.n[0], b.n[1]};
-    hmg_status st = hmg_init_h_matrix_data(&d, pc, b.dim, b.bits);
+    hmg_status st = engine_init(&d, pc, b.dim, b.bits, slot);
     if (st == HMG_OK) {
         m->inited = true;
         Engine* e = engine_of(&d);
@@ -277,12 +281,13 @@ hmg_status run_build(hmg_server* s, Matrix* m, Request* r, std::unique_lock<std:
     return st;
 }
 
-hmg_status run_pass(hmg_server* s, Matrix* m, std::vector<Request*>& batch, int total,
+hmg_status run_pass(hmg_server* s, Matrix* m, std::vector<Request*>& batch, int total, hipStream_t slot,
                     std::unique_lock<std::mutex>& lk)
 {
     lk.unlock();
     hmg_status st = HMG_OK;
     Engine* e = engine_of(&m->data);
+    engine_bind_stream(e, slot, &m->data);
     AllocScope scope(e->stream);  // merged pass buffers are stream-ordered
     const int64_t n1 = m->data.point_count[0], n2 = m->data.point_count[1];
     const double* X = batch[0]->X;
@@ -319,7 +324,7 @@ hmg_status run_pass(hmg_server* s, Matrix* m, std::vector<Request*>& batch, int 
     return st;
 }
 
-void worker(hmg_server* s)
+void worker(hmg_server* s, int slot_index)
 {
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
@@ -327,6 +332,7 @@ void worker(hmg_server* s)
         Matrix* m = nullptr;
         s->cv_work.wait(lk, [&] { return s->stop || (m = pick_job(s, &is_build)) != nullptr; });
         if (!m) return;  // stopping with nothing to do
+        const hipStream_t slot = s->slot_streams[slot_index];
         m->busy = true;
         s->running++;
         const double t_start = s->now();
@@ -335,12 +341,14 @@ void worker(hmg_server* s)
             Request* r = m->build_req;
             m->build_req = nullptr;
             r->info.t_start = t_start;
-            const hmg_status st = run_build(s, m, r, lk);
+            const hmg_status st = run_build(s, m, r, slot, lk);
             const double t_end = s->now();
             if (st == HMG_OK) {
                 m->ready = true;
+                // admitted at the build's peak (the mixed-precision ACA works in FP64
+                // temporaries); held from now on: what it owns plus its first product's buffers
                 int64_t mem[2] = {0, 0};
-                if (hmg_memory(&m->data, mem) == HMG_OK && mem[0] + mem[1] > m->bytes) {
+                if (hmg_memory(&m->data, mem) == HMG_OK) {
                     s->used += mem[0] + mem[1] - m->bytes;
                     m->bytes = mem[0] + mem[1];
                 }
@@ -368,7 +376,7 @@ void worker(hmg_server* s)
                 r->info.t_start = t_start;
                 r->info.merged = (int)batch.size();
             }
-            const hmg_status st = run_pass(s, m, batch, total, lk);
+            const hmg_status st = run_pass(s, m, batch, total, slot, lk);
             const double t_end = s->now();
             s->n_passes++;
             if (batch.size() > 1) s->n_shared += (int64_t)batch.size();
@@ -418,7 +426,7 @@ hmg_status hmg_server_create(const hmg_server_config* cfg, hmg_server** out)
     hmg_server* s = new hmg_server();
     s->cfg = c;
     s->t0 = Clock::now();
-    for (int i = 0; i < c.slots; i++) s->workers.emplace_back(worker, s);
+    for (int i = 0; i < c.slots; i++) s->workers.emplace_back(worker, s, i);
     *out = s;
     return HMG_OK;
 }
@@ -443,6 +451,12 @@ hmg_status hmg_server_destroy(hmg_server* s)
     for (auto& t : s->workers) t.join();
     for (auto& kv : s->mats) destroy_matrix(s, kv.second.get());
     for (system_assembler* k : s->free_kernels) hmg_kernel_free(k);
+    for (hipStream_t st : s->slot_streams) {
+        (void)hipStreamSynchronize(st);
+        release_scratch(st);
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
     delete s;
     return HMG_OK;
 }
@@ -462,6 +476,15 @@ hmg_status hmg_server_submit_build(hmg_server* s, const hmg_build_request* b, in
     if (!same) r->coords[1].assign(b->coords[1], b->coords[1] + (size_t)b->dim * b->n[1]);
     auto m = std::make_unique<Matrix>();
     std::lock_guard<std::mutex> lk(s->mu);
+    // the slot streams, all at once with the first build (every job follows one)
+    while ((int)s->slot_streams.size() < s->cfg.slots) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            set_error("stream creation failed");
+            return HMG_ERR_HIP;
+        }
+        s->slot_streams.push_back(st);
+    }
     r->id = s->next_req++;
     m->id = s->next_mat++;
     r->matrix = m->id;

@@ -166,7 +166,7 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
  *   row, compute the norms of the following rows four at a time (rank steps up
  *   to 64) instead of one at a time.  The factors are bitwise the same.
  * HMG_OPT_ACA_BIG_BLOCK [4096]: admissible blocks with more rows plus columns
- *   than this run their ACA on a workgroup of 8 wavefronts instead of one
+ *   than this run their ACA on a workgroup of 4 wavefronts instead of one
  *   wavefront (row norms summed in another order; same factors up to the
  *   rounding of those norms). */
 #define HMG_OPT_NEAR_SYMMETRIC 1

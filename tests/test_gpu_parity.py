@@ -341,3 +341,4 @@ def test_aca_large_blocks_workgroup_path(gpu, kind, ls):
     # the FP64 H-matrix's own approximation error against the dense product
     y_dense = hm.apply_full(x)
     assert _rel(ym, y_dense) < _rel(y, y_dense) + 2e-5
+
