@@ -4,9 +4,10 @@
 Default workload (mixed vs plain): N = 2^20 points i.i.d. uniform in [0,1]^3
 (seed 2001), source = target, exponential kernel exp(-r/0.1), leaf 256, eta 1.0,
 ACA rank 24 (the reference's fixed-rank ACA).
-  value:  mixed precision with 32 right-hand sides N(0,1) (seed 3001): FP32 ACA
-          factors, FP32 MFMA low-rank products, near field evaluated in FP32 and
-          multiplied as BF16 hi+lo split MFMA GEMMs.  A step is one call of the
+  value:  mixed precision with 32 right-hand sides N(0,1) (seed 3001): the FP64
+          ACA with its factors stored as BF16 hi + lo panels, low-rank products
+          and the near field (evaluated in FP32) as BF16 hi+lo split MFMA GEMMs
+          accumulated in FP32.  A step is one call of the
           product on 32 device-resident right-hand sides; value = RHS x steps /
           time.  The relative error against exact FP64 rows is measured per RHS on
           2048 sampled target rows (seed 2003).
@@ -51,7 +52,7 @@ def kernel_roofline(hm, prof, nrhs_pass, mixed, traffic_name):
     e_near = int(w["E_near"])
     sch = hm.schedule()
     R = nrhs_pass
-    s_f = 4 if mixed else 8          # bytes per factor / vector element
+    s_f = 4 if mixed else 8          # bytes per factor / vector element (mixed: BF16 hi + lo, or FP32)
     # distinct row and column clusters of the admissible blocks
     import hmglib_amd
     blk = hm.blocks(np.dtype(hmglib_amd.WorkItem))
@@ -103,7 +104,7 @@ def kernel_roofline(hm, prof, nrhs_pass, mixed, traffic_name):
     }
     dominant = max(("lr_vtx", "lr_u", "near"), key=lambda k: t[k])
     if mixed:
-        kname = {"lr_vtx": "lr_v32_kernel", "lr_u": "lr_u32_kernel", "near": "near_mx_kernel"}[dominant]
+        kname = {"lr_vtx": "lr_vb_kernel", "lr_u": "lr_ub_kernel", "near": "near_mx_kernel"}[dominant]
     else:
         kname = {"lr_vtx": "lr_v_kernel", "lr_u": "lr_u_kernel", "near": "near_fly_kernel"}[dominant]
     traffic = measured_traffic(hm, kname, traffic_name)
@@ -177,7 +178,7 @@ WORKLOADS = {
     "c3": dict(dim=3, bits=20, n=1 << 20, kind=4, ls=0.1, c_leaf=256, eta=1.0, k=24, seed_pts=2001,
                seed_x=2002, seed_X=3001, seed_rows=2003, nrhs=32, mixed=True,
                desc="N=2^20 U[0,1]^3, exp(-r/0.1), leaf 256, eta 1.0, ACA rank 24; mixed precision "
-                    "(FP32 factors, FP32 MFMA low rank, BF16 hi+lo MFMA near field), 32 RHS"),
+                    "(FP64 ACA, BF16 hi+lo factor panels, BF16x3 MFMA low rank and near field), 32 RHS"),
     # the plain method at scale
     "c2": dict(dim=3, bits=20, n=1 << 20, kind=4, ls=0.1, c_leaf=256, eta=1.0, k=24, seed_pts=2001,
                seed_x=2002, seed_X=3001, seed_rows=2003, nrhs=1, mixed=False,
@@ -458,8 +459,8 @@ def main():
     config = {"workload": cfg["desc"], "n": n, "nrhs": R, "build_s": {"setup": t_setup},
               "blocks": {"dense": w["dense"], "aca": w["aca"]}, "E_near": w["E_near"], "E_lr": w["E_lr"]}
     if mixed:
-        config.update({"precision": "mixed: FP64 ACA, factors rounded to FP32, FP32 MFMA low rank, "
-                                    "BF16x3 MFMA near field",
+        config.update({"precision": "mixed: FP64 ACA, factors as BF16 hi+lo panels, BF16x3 MFMA low "
+                                    "rank and near field (FP32 accumulation, FP32 near-field evaluation)",
                        "rel_err_fp64_rows_per_rhs_max": max(result["errs"]),
                        "rel_err_fp64_rows_per_rhs": result["errs"]})
         config["build_s"]["aca"] = result["aca_s"]
@@ -471,7 +472,7 @@ def main():
             config["build_s"]["aca_fp64"] = plain["aca_s"]
         if served is not None:
             config["served"] = served
-        value, ms, steps, dtype = result["value"], result["ms"], args.steps, "f32/bf16x3 (mixed)"
+        value, ms, steps, dtype = result["value"], result["ms"], args.steps, "bf16x3/f32 (mixed)"
         metric = ("H-MVP/s at N=2^20 3D, 32 RHS, mixed precision (rel.err<=1e-4 vs FP64), "
                   "exp(-r/0.1), rank-24 ACA")
     else:

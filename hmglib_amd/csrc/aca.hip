@@ -101,6 +101,24 @@ struct Team {
         }
         __syncthreads();
     }
+    // team minimum and maximum (every thread ends with the same value)
+    template <bool MAX>
+    __device__ double extreme(double v) const
+    {
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            const double o = __shfl_xor(v, off, kWave);
+            v = MAX ? fmax(v, o) : fmin(v, o);
+        }
+        if (NW == 1) return v;
+        const int w = threadIdx.x / kWave;
+        if (threadIdx.x % kWave == 0) s_v[w] = v;
+        __syncthreads();
+        double t = s_v[0];
+        for (int q = 1; q < NW; q++) t = MAX ? fmax(t, s_v[q]) : fmin(t, s_v[q]);
+        __syncthreads();
+        return t;
+    }
     // (|v|, index, v) maximum with the tie rule: last index (tie_last) or first
     __device__ static bool better(double ob, int oj, double best, int bj, int tie_last)
     {
@@ -233,6 +251,58 @@ __device__ void scan_row_norms(const Team<NW>& team, Coords c1, Coords c2, const
     team.template sum_n<R>(nrm);
 }
 
+// True when every entry of the block is so small that every row of the first
+// rank step has residual norm below the 1e-13 row threshold, i.e. the search would
+// reject all m1 rows and the block gets rank 0 after evaluating all m1 x m2
+// entries.  Only for kernels that decrease with the distance and when no entry
+// carries the regularization (it is added where sorted indices coincide).  The
+// tree's boxes only pre-screen (a cluster of one point can carry another
+// cluster's box); the decision uses the bounding boxes of the block's points:
+// |K(i,j)| <= k(d_min), and sqrt(m2) k(d_min) < 1e-13 / 2 leaves a factor two for
+// the rounding of the evaluated norms.
+template <int NW, int KIND>
+__device__ bool block_negligible(const Team<NW>& team, Coords c1, Coords c2, const KernelParams& kp, int4 b,
+                                 const work_item& wi)
+{
+    if (KIND != HMG_KERNEL_GAUSSIAN && KIND != HMG_KERNEL_EXPONENTIAL && KIND != HMG_KERNEL_GAUSSIAN_REF)
+        return false;
+    if (kp.reg != 0.0 && b.x <= b.w && b.z <= b.y) return false;
+    const double m2 = (double)(b.w - b.z + 1);
+    const double thr = 0.5e-13;
+    // pre-screen with the tree's boxes, loosely
+    double r2 = 0.0;
+    for (int d = 0; d < kp.dim; d++) {
+        const double gap = fmax(0.0, fmax(wi.min1[d] - wi.max2[d], wi.min2[d] - wi.max1[d]));
+        r2 += gap * gap;
+    }
+    if (sqrt(m2) * kernel_of_r2_k<KIND>(kp, 0.5 * r2) >= thr) return false;
+    // exact boxes of the points
+    r2 = 0.0;
+    const int tid = team.tid();
+    constexpr int TS = Team<NW>::size;
+    for (int d = 0; d < kp.dim; d++) {
+        double lo1 = 1e300, hi1 = -1e300, lo2 = 1e300, hi2 = -1e300;
+        for (int i = b.x + tid; i <= b.y; i += TS) {
+            const double x = c1.x[d][i];
+            lo1 = fmin(lo1, x);
+            hi1 = fmax(hi1, x);
+        }
+        for (int j = b.z + tid; j <= b.w; j += TS) {
+            const double x = c2.x[d][j];
+            lo2 = fmin(lo2, x);
+            hi2 = fmax(hi2, x);
+        }
+        lo1 = team.template extreme<false>(lo1);
+        hi1 = team.template extreme<true>(hi1);
+        lo2 = team.template extreme<false>(lo2);
+        hi2 = team.template extreme<true>(hi2);
+        const double gap = fmax(0.0, fmax(lo1 - hi2, lo2 - hi1));
+        r2 += gap * gap;
+    }
+    // r2 shrunk by far more than its rounding: the bound stays an upper bound
+    return sqrt(m2) * kernel_of_r2_k<KIND>(kp, r2 * (1.0 - 1e-9)) < thr;
+}
+
 // Factor layout: U_b[i, l] at U + uoff[a] + l * ldu[a] + i (column-major, ldu >= m1:
 // the row cluster's padded height); V_b is stored transposed, V_b[j, l] at
 // V + voff[a] + j * ldv[a] + l, i.e. row j of V_b is contiguous (ldv >= total rank
@@ -244,12 +314,20 @@ __device__ void scan_row_norms(const Team<NW>& team, Coords c1, Coords c2, const
 template <class T, int NW, int KIND>
 __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const KernelParams& kp, int4 b, int k,
                           int64_t ld, int64_t lu, T* __restrict__ Ub, T* __restrict__ Vb, int* keff_out,
-                          int tie_last, int scan, T* __restrict__ sU, double* __restrict__ sX)
+                          int tie_last, int scan, T* __restrict__ sU, double* __restrict__ sX,
+                          const work_item& wi)
 {
     const int tid = team.tid();
     constexpr int TS = Team<NW>::size;
     const int l1 = b.x, m1 = b.y - b.x + 1;
     const int l2 = b.z, m2 = b.w - b.z + 1;
+    if (scan && k > 0 && block_negligible<NW, KIND>(team, c1, c2, kp, b, wi)) {
+        // what the search ends with after rejecting every row: rank 0, column 0 of
+        // V cleared
+        for (int j = tid; j < m2; j += TS) Vb[(int64_t)j * ld] = (T)0;
+        if (tid == 0) *keff_out = 0;
+        return;
+    }
     int i_r = -1;
     int ke = 0;
     for (int r = 0; r < k; r++) {
@@ -335,7 +413,8 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
 constexpr int kAcaTeam = 4;
 template <class T, int KIND>
 __global__ void __launch_bounds__(kAcaTeam * kWave) __attribute__((amdgpu_waves_per_eu(4)))
-aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, const int* __restrict__ list,
+aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk,
+           const work_item* __restrict__ items, const int* __restrict__ list,
            int nbig, int nsmall, const int* __restrict__ kb, const int64_t* __restrict__ uoff,
            const int* __restrict__ ldu, const int64_t* __restrict__ voff, const int* __restrict__ ldv,
            T* __restrict__ U, T* __restrict__ V, int* __restrict__ keff, int tie_last, int scan)
@@ -346,32 +425,44 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk, 
     __shared__ T sU[kAcaTeam][kScanRows * kScanL];
     __shared__ double sX[kAcaTeam][kScanRows * MAX_POINT_DIMENSION];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    // the nbig large-block workgroups are spread evenly over the grid, so large
-    // and small blocks run side by side: workgroup i is the large one number
-    // floor(i nbig / G) when that count steps at i, else small one number
-    // i - floor(i nbig / G)
-    const int64_t G = (int64_t)gridDim.x, i = blockIdx.x;
-    const int bi = (int)(i * nbig / G), bi1 = (int)((i + 1) * nbig / G);
-    if (bi1 != bi) {
-        const int a = list[bi];
+    // large blocks first: their long rank loops start at once
+    if ((int)blockIdx.x < nbig) {
+        const int a = list[blockIdx.x];
         const Team<kAcaTeam> team{s_v, s_i, s_x};
         aca_block<T, kAcaTeam, KIND>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
-                                     keff + a, tie_last, scan, sU[w], sX[w]);
+                                     keff + a, tie_last, scan, sU[w], sX[w], items[a]);
     } else {
-        const int q = ((int)i - bi) * kAcaTeam + w;
+        const int q = ((int)blockIdx.x - nbig) * kAcaTeam + w;
         if (q >= nsmall) return;
         const int a = list[nbig + q];
         const Team<1> team{nullptr, nullptr, nullptr};
         aca_block<T, 1, KIND>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
-                              keff + a, tie_last, scan, sU[w], sX[w]);
+                              keff + a, tie_last, scan, sU[w], sX[w], items[a]);
     }
 }
 
 }  // namespace
 
+// one launch over list[0 .. nbig + nsmall): nbig large blocks first; factor
+// offsets uoff / voff (leading dims eng.ldu / eng.ldv) into U / V
 template <class T>
-static hmg_status aca_build_t(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp, int k,
-                       int tie_last, T* U, T* V, int* keff)
+static hmg_status aca_launch(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
+                             int tie_last, const int* list, int nbig, int nsmall, const int64_t* uoff,
+                             const int64_t* voff, T* U, T* V, int* keff)
+{
+    const int scan = eng.aca_row_scan ? 1 : 0;
+    const int4* blk = eng.blk.p + eng.ndense;
+    const unsigned grid = (unsigned)(nbig + (nsmall + kAcaTeam - 1) / kAcaTeam);
+    if (grid == 0) return HMG_OK;
+    HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND><<<grid, kAcaTeam * kWave, 0, eng.stream>>>(
+                                   c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, nbig, nsmall, eng.kb.p,
+                                   uoff, eng.ldu.p, voff, eng.ldv.p, U, V, keff, tie_last, scan));
+    HMG_HIP(hipGetLastError());
+    return HMG_OK;
+}
+
+hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp, int k,
+                     int tie_last, double* U, double* V, int* keff)
 {
     (void)k;
     const int na = eng.naca;
@@ -392,42 +483,156 @@ static hmg_status aca_build_t(Engine& eng, const Coords& c1, const Coords& c2, c
     }
     HMG_TRY(eng.aca_list.alloc(na));
     HMG_HIP(hipMemcpyAsync(eng.aca_list.p, order.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
-    const int nsmall = na - nbig;
-    const int scan = eng.aca_row_scan ? 1 : 0;
-    const int4* blk = eng.blk.p + eng.ndense;
     eng.prof.begin(PROF_ACA, s);
-    const unsigned grid = (unsigned)(nbig + (nsmall + kAcaTeam - 1) / kAcaTeam);
-    HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND><<<grid, kAcaTeam * kWave, 0, s>>>(
-                                   c1, c2, kp, blk, eng.aca_list.p, nbig, nsmall, eng.kb.p, eng.uoff.p,
-                                   eng.ldu.p, eng.voff.p, eng.ldv.p, U, V, keff, tie_last, scan));
-    HMG_HIP(hipGetLastError());
+    HMG_TRY(aca_launch<double>(eng, c1, c2, kp, tie_last, eng.aca_list.p, nbig, na - nbig, eng.uoff.p,
+                               eng.voff.p, U, V, keff));
     eng.prof.end(PROF_ACA, s);
     return HMG_OK;
 }
 
-hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp, int k,
-                     int tie_last, double* U, double* V, int* keff)
+namespace {
+
+constexpr int kConvThreads = 256;
+
+// BF16 hi + lo of an FP64 value (the value rounded to FP32 first)
+__device__ __forceinline__ void split_bf16(double v, __bf16& hi, __bf16& lo)
 {
-    return aca_build_t<double>(eng, c1, c2, kp, k, tie_last, U, V, keff);
+    const float f = (float)v;
+    hi = (__bf16)f;
+    lo = (__bf16)(f - (float)hi);
 }
 
-namespace {
-constexpr int kRoundThreads = 256;
-__global__ void __launch_bounds__(kRoundThreads) round_f32_kernel(const double* __restrict__ src,
-                                                                  float* __restrict__ dst, int64_t n)
+// One workgroup per block of list[]: its FP64 factors in the scratch (U_b
+// column-major m1 x kb at suoff, V_b row-major m2 x kb at svoff) into the panels.
+// Rank columns at or past the block's effective rank are written as zeros (the
+// ACA leaves them unwritten).
+__global__ void __launch_bounds__(kConvThreads)
+mx_panel_kernel(const int* __restrict__ list, const int4* __restrict__ blk, const int* __restrict__ kb,
+                const int* __restrict__ keff, const int64_t* __restrict__ suoff, const int64_t* __restrict__ svoff,
+                const MxPanel* __restrict__ mxp, const double* __restrict__ S, __bf16* __restrict__ Up,
+                __bf16* __restrict__ Vp)
 {
-    const int64_t stride = (int64_t)gridDim.x * kRoundThreads;
-    for (int64_t i = (int64_t)blockIdx.x * kRoundThreads + threadIdx.x; i < n; i += stride)
-        dst[i] = (float)src[i];
+    const int a = list[blockIdx.x];
+    const int4 b = blk[a];
+    const int m1 = b.y - b.x + 1, m2 = b.w - b.z + 1;
+    const int k = kb[a], ke = keff[a];
+    const MxPanel mp = mxp[a];
+    const double* __restrict__ Su = S + suoff[a];
+    const double* __restrict__ Sv = S + svoff[a];
+    const int64_t n_u = (int64_t)m1 * k;
+    for (int64_t q = threadIdx.x; q < n_u; q += kConvThreads) {
+        const int i = (int)(q % m1), l = (int)(q / m1);
+        const double v = l < ke ? Su[q] : 0.0;
+        const int kk = mp.koff + l;
+        const int64_t at = mp.ubase + (int64_t)(kk >> 3) * 16 * mp.m1p + (int64_t)i * 8 + (kk & 7);
+        __bf16 hi, lo;
+        split_bf16(v, hi, lo);
+        Up[at] = hi;
+        Up[at + 8 * (int64_t)mp.m1p] = lo;
+    }
+    const int64_t n_v = (int64_t)m2 * k;
+    for (int64_t q = threadIdx.x; q < n_v; q += kConvThreads) {
+        const int j = (int)(q / k), l = (int)(q % k);
+        const double v = l < ke ? Sv[q] : 0.0;
+        const int p = b.z + j;
+        const int64_t at = mp.vbase + (int64_t)((p >> 3) - mp.glo) * 16 * mp.kp + (int64_t)(mp.pos + l) * 8 + (p & 7);
+        __bf16 hi, lo;
+        split_bf16(v, hi, lo);
+        Vp[at] = hi;
+        Vp[at + 8 * (int64_t)mp.kp] = lo;
+    }
 }
+
+// FP64 elements of one block's factors in the scratch
+int64_t block_scratch(const Engine& eng, int a)
+{
+    const int4 b = eng.h_blk[eng.ndense + a];
+    return (int64_t)((b.y - b.x + 1) + (b.w - b.z + 1)) * eng.h_kb[a];
+}
+
 }  // namespace
 
-hmg_status round_to_f32(Engine& eng, const double* src, float* dst, int64_t n)
+int64_t aca_mixed_scratch(const Engine& eng)
 {
-    if (n <= 0) return HMG_OK;
-    const unsigned grid = (unsigned)std::min<int64_t>((n + kRoundThreads - 1) / kRoundThreads, 8192);
-    round_f32_kernel<<<grid, kRoundThreads, 0, eng.stream>>>(src, dst, n);
-    HMG_HIP(hipGetLastError());
+    int64_t total = 0;
+    for (int a = 0; a < eng.naca; a++) total += block_scratch(eng, a);
+    // all at once up to 2^31 elements (16 GiB), else a quarter of the factors at a
+    // time (each batch ends with its slowest block)
+    const int64_t cap = std::max<int64_t>(total / 4, (int64_t)1 << 31);
+    return std::min(total, cap);
+}
+
+hmg_status aca_build_mixed(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp, int tie_last,
+                           __bf16* Up, __bf16* Vp)
+{
+    const int na = eng.naca;
+    if (na == 0) return HMG_OK;
+    hipStream_t s = eng.stream;
+    const int64_t cap = std::max<int64_t>(aca_mixed_scratch(eng), 1);
+    // batches: blocks dealt round-robin (each batch mixes large and small blocks);
+    // more batches until every batch's factors fit the scratch
+    int64_t total = 0, largest = 0;
+    for (int a = 0; a < na; a++) {
+        total += block_scratch(eng, a);
+        largest = std::max(largest, block_scratch(eng, a));
+    }
+    const int64_t scratch = std::max(cap, largest);
+    int nb = (int)std::max<int64_t>(1, (total + scratch - 1) / scratch);
+    std::vector<int64_t> suoff(na), svoff(na), used;
+    std::vector<int> ldu(na), ldv(na);
+    for (;; nb++) {
+        used.assign(nb, 0);
+        bool fits = true;
+        for (int a = 0; a < na && fits; a++) {
+            const int4 b = eng.h_blk[eng.ndense + a];
+            const int m1 = b.y - b.x + 1, m2 = b.w - b.z + 1, k = eng.h_kb[a];
+            int64_t& u = used[a % nb];
+            suoff[a] = u;
+            svoff[a] = u + (int64_t)m1 * k;
+            ldu[a] = m1;
+            ldv[a] = std::max(k, 1);
+            u += (int64_t)(m1 + m2) * k;
+            fits = u <= scratch;
+        }
+        if (fits) break;
+    }
+    DevBuf<double> S;
+    HMG_TRY(S.alloc((size_t)scratch));
+    DevBuf<int64_t> d_suoff, d_svoff;
+    DevBuf<int> d_list;
+    HMG_TRY(upload(d_suoff, suoff, s));
+    HMG_TRY(upload(d_svoff, svoff, s));
+    HMG_TRY(upload(eng.ldu, ldu, s));
+    HMG_TRY(upload(eng.ldv, ldv, s));
+    // per batch: its large blocks first, then the rest (the ACA kernel's order)
+    std::vector<int> list;
+    list.reserve(na);
+    std::vector<int> bptr(nb + 1, 0), bbig(nb, 0);
+    for (int q = 0; q < nb; q++) {
+        bptr[q] = (int)list.size();
+        for (int pass = 0; pass < 2; pass++) {
+            for (int a = q; a < na; a += nb) {
+                const int4 b = eng.h_blk[eng.ndense + a];
+                const bool big = (b.y - b.x + 1) + (b.w - b.z + 1) > eng.aca_big_entries;
+                if (big == (pass == 0)) list.push_back(a);
+            }
+            if (pass == 0) bbig[q] = (int)list.size() - bptr[q];
+        }
+    }
+    bptr[nb] = (int)list.size();
+    HMG_TRY(upload(d_list, list, s));
+    eng.prof.begin(PROF_ACA, s);
+    for (int q = 0; q < nb; q++) {
+        const int n = bptr[q + 1] - bptr[q];
+        if (n == 0) continue;
+        HMG_TRY(aca_launch<double>(eng, c1, c2, kp, tie_last, d_list.p + bptr[q], bbig[q], n - bbig[q],
+                                   d_suoff.p, d_svoff.p, S.p, S.p, eng.keff.p));
+        mx_panel_kernel<<<n, kConvThreads, 0, s>>>(d_list.p + bptr[q], eng.blk.p + eng.ndense, eng.kb.p,
+                                                  eng.keff.p, d_suoff.p, d_svoff.p, eng.mxp.p, S.p, Up, Vp);
+        HMG_HIP(hipGetLastError());
+    }
+    eng.prof.end(PROF_ACA, s);
+    HMG_HIP(hipStreamSynchronize(s));
     return HMG_OK;
 }
 

@@ -144,8 +144,10 @@ void release_factors(Engine& e, h_matrix_data* d)
 {
     e.U.release();
     e.V.release();
-    e.U32.release();
-    e.V32.release();
+    e.Up.release();
+    e.Vp.release();
+    e.Tpan.release();
+    e.tpan_ready = false;
     e.aca_k = -1;
     d->U = nullptr;
     d->V = nullptr;
@@ -220,9 +222,8 @@ void release_build(Engine& e, h_matrix_data* d)
     e.comb_mx.idx.release();
     e.comb_mx.groups.release();
     e.comb_mx.total = 0;
-    e.Xs32.release();
-    e.T32.release();
     e.Tp32.release();
+    e.mxp.release();
     e.P32.release();
     e.Ys32.release();
     e.inv1.release();
@@ -243,42 +244,31 @@ void release_build(Engine& e, h_matrix_data* d)
     d->aca_work_size = nullptr;
 }
 
-// ACA factors into (U, V, keff) for the current block list and rank k; T = double
-// (FP64 layout) or float (mixed-precision layout)
-template <class T>
+// FP64 ACA factors into (U, V, keff) for the current block list and rank k
 hmg_status build_factors(Engine& e, h_matrix_data* d, const KernelParams& kp, int k, int tie_last,
-                         DevBuf<T>& U, DevBuf<T>& V)
+                         DevBuf<double>& U, DevBuf<double>& V)
 {
-    HMG_TRY(aca_layout(e, k, sizeof(T) == sizeof(float) ? HMG_PREC_MIXED : HMG_PREC_FP64));
+    HMG_TRY(aca_layout(e, k, HMG_PREC_FP64));
     const int na = e.naca;
-    // 256 elements of zero padding past the end: the mixed-precision products
-    // read whole 128-element rows or columns, up to 32 elements beyond the last
-    const int64_t nu = e.h_uoff[na] + 256, nv = e.h_voff[na] + 256;
-    if constexpr (sizeof(T) == sizeof(double)) {
-        HMG_TRY(U.alloc(nu));
-        HMG_TRY(V.alloc(nv));
-        HMG_HIP(hipMemsetAsync(U.p, 0, sizeof(double) * U.n, e.stream));
-        HMG_HIP(hipMemsetAsync(V.p, 0, sizeof(double) * V.n, e.stream));
-        HMG_TRY(aca_build(e, coords_of(d, 0), coords_of(d, 1), kp, k, tie_last, U.p, V.p, e.keff.p));
-    } else {
-        // the ACA runs in FP64 into temporaries of the FP32 layout (same pivots,
-        // ranks and stopping as in FP64 precision: FP32 storage during the ACA
-        // would turn the small residuals of late rank steps into rounding noise);
-        // the factors are rounded to FP32 once at the end
-        DevBuf<double> U64, V64;
-        HMG_TRY(U64.alloc(nu));
-        HMG_TRY(V64.alloc(nv));
-        HMG_HIP(hipMemsetAsync(U64.p, 0, sizeof(double) * U64.n, e.stream));
-        HMG_HIP(hipMemsetAsync(V64.p, 0, sizeof(double) * V64.n, e.stream));
-        HMG_TRY(aca_build(e, coords_of(d, 0), coords_of(d, 1), kp, k, tie_last, U64.p, V64.p, e.keff.p));
-        HMG_TRY(U.alloc(nu));
-        HMG_TRY(V.alloc(nv));
-        HMG_TRY(round_to_f32(e, U64.p, U.p, nu));
-        HMG_TRY(round_to_f32(e, V64.p, V.p, nv));
-        U64.release();
-        V64.release();
-    }
+    HMG_TRY(U.alloc(e.h_uoff[na] + 256));
+    HMG_TRY(V.alloc(e.h_voff[na] + 256));
+    HMG_HIP(hipMemsetAsync(U.p, 0, sizeof(double) * U.n, e.stream));
+    HMG_HIP(hipMemsetAsync(V.p, 0, sizeof(double) * V.n, e.stream));
+    HMG_TRY(aca_build(e, coords_of(d, 0), coords_of(d, 1), kp, k, tie_last, U.p, V.p, e.keff.p));
     HMG_HIP(hipStreamSynchronize(e.stream));
+    return HMG_OK;
+}
+
+// mixed precision: the same FP64 ACA, its factors split into the BF16 hi + lo
+// panels e.Up / e.Vp
+hmg_status build_factors_mixed(Engine& e, h_matrix_data* d, const KernelParams& kp, int k, int tie_last)
+{
+    HMG_TRY(aca_layout(e, k, HMG_PREC_MIXED));
+    HMG_TRY(e.Up.alloc((size_t)e.up_size));
+    HMG_TRY(e.Vp.alloc((size_t)e.vp_size));
+    HMG_HIP(hipMemsetAsync(e.Up.p, 0, sizeof(__bf16) * e.Up.n, e.stream));
+    HMG_HIP(hipMemsetAsync(e.Vp.p, 0, sizeof(__bf16) * e.Vp.n, e.stream));
+    HMG_TRY(aca_build_mixed(e, coords_of(d, 0), coords_of(d, 1), kp, tie_last, e.Up.p, e.Vp.p));
     return HMG_OK;
 }
 
@@ -539,9 +529,9 @@ hmg_status hmg_precompute_aca(struct h_matrix_data* data)
     // max_batched_aca_size (hmglib.cu:232, linear_algebra.cu:4163-4171); the
     // engine has no batch cap and always builds every ACA block.
     if (e->precision == HMG_PREC_MIXED) {
-        HMG_TRY(build_factors(*e, data, kp, std::max(data->k, 0), 1, e->U32, e->V32));
-        data->U = (double*)e->U32.p;  // FP32 factors in mixed precision
-        data->V = (double*)e->V32.p;
+        HMG_TRY(build_factors_mixed(*e, data, kp, std::max(data->k, 0), 1));
+        data->U = (double*)e->Up.p;  // BF16 hi + lo panels in mixed precision
+        data->V = (double*)e->Vp.p;
     } else {
         HMG_TRY(build_factors(*e, data, kp, std::max(data->k, 0), 1, e->U, e->V));
         data->U = e->U.p;
@@ -595,15 +585,15 @@ hmg_status hmg_apply_h_matrix_mvp_multi(const double* X, double* Y, int nrhs, st
     const int prec = pre_aca ? e->fact_prec : e->precision;
     if (prec == HMG_PREC_MIXED) {
         if (!pre_aca) {
-            HMG_TRY(build_factors(*e, data, kp, std::max(data->k, 0), 1, e->U32, e->V32));
+            HMG_TRY(build_factors_mixed(*e, data, kp, std::max(data->k, 0), 1));
             e->fact_prec = HMG_PREC_MIXED;
         }
         HMG_TRY(hmvp_mixed(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[0], data->order[1],
                            X, Y, nrhs));
         HMG_HIP(hipStreamSynchronize(e->stream));
         if (!pre_aca) {
-            e->U32.release();
-            e->V32.release();
+            e->Up.release();
+            e->Vp.release();
             e->aca_k = -1;
         }
         return HMG_OK;
@@ -807,8 +797,8 @@ int64_t engine_bytes(const Engine& e)
     t += bytes_of(e.aoff) + bytes_of(e.A) + bytes_of(e.near_items) + bytes_of(e.near_blk);
     t += bytes_of(e.fly_items) + bytes_of(e.fly_refs) + bytes_of(e.lr_items) + bytes_of(e.lv_items);
     t += bytes_of(e.lv_slices) + bytes_of(e.tmap) + bytes_of(e.Tp) + bytes_of(e.P) + bytes_of(e.xs);
-    t += bytes_of(e.T) + bytes_of(e.U32) + bytes_of(e.V32) + bytes_of(e.mx_items) + bytes_of(e.mx_refs);
-    t += bytes_of(e.Xs32) + bytes_of(e.T32) + bytes_of(e.Tp32) + bytes_of(e.P32) + bytes_of(e.Xp);
+    t += bytes_of(e.T) + bytes_of(e.Up) + bytes_of(e.Vp) + bytes_of(e.mx_items) + bytes_of(e.mx_refs);
+    t += bytes_of(e.Tpan) + bytes_of(e.Tp32) + bytes_of(e.P32) + bytes_of(e.Xp) + bytes_of(e.mxp);
     t += bytes_of(e.Ys32) + bytes_of(e.inv1);
     t += bytes_of(e.cpan[0]) + bytes_of(e.cpan[1]);
     for (const Combine* c : {&e.comb_stored, &e.comb_fly, &e.comb_mx})
@@ -841,12 +831,13 @@ hmg_status hmg_memory(const struct h_matrix_data* data, int64_t out[2])
         // factor elements of the current layout; the mixed layout pads to 32 floats
         // instead of 16 doubles
         const double elems = (double)(e->h_uoff[na] + e->h_voff[na]);
-        // mixed: FP32 factors plus the FP64 factors the ACA computes them from
-        add += (int64_t)(elems * (mixed ? 12.0 * 1.03 : 8.0));
+        // mixed: BF16 hi + lo panels (4 bytes per element, padded to whole MFMA
+        // tiles) plus the FP64 scratch of the ACA batches
+        add += mixed ? (int64_t)(elems * 4.0 * 1.25) + 8 * aca_mixed_scratch(*e) : (int64_t)(elems * 8.0);
     }
     if (mixed && e->layout_prec != HMG_PREC_MIXED) {
         const int64_t n1 = e->n[0], n2 = e->n[1];
-        const int64_t g1 = (n1 + 7) / 8 + 2, g2 = (n2 + 7) / 8 + 2;
+        const int64_t g1 = (n1 + 7) / 8 + 4, g2 = (n2 + 7) / 8 + 4;
         // near-field items: 128-row slices x 4096-column chunks per dense row cluster
         int64_t mx_items = 0;
         const int ng = (int)e->h_dgrp_ptr.size() - 1;
@@ -871,8 +862,8 @@ hmg_status hmg_memory(const struct h_matrix_data* data, int64_t out[2])
             if (nch > 1) ntp += nch * ((K + 127) / 128);
         }
         const int64_t slots = (int64_t)e->n_lr_items * 64 + mx_items * 128;
-        add += g2 * 8 * kMxRhs * 4 + g2 * 2 * kMxRhs * 8 * 2;     // Xs32, Xp
-        add += e->tsize * kMxRhs * 4 + ntp * kMxVCols * kMxRhs * 4;  // T32, Tp32
+        add += g2 * 2 * kMxRhs * 8 * 2;                                         // Xp
+        add += (e->tsize + 32 * (int64_t)ncg) * kMxRhs * 4 + ntp * kMxVCols * kMxRhs * 4;  // Tpan, Tp32
         add += slots * kMxRhs * 4 + slots * 4 + n1 * 4;              // P32, combine list
         add += n1 * kMxRhs * 4 + n1 * 4;                             // Ys32, inv1
         add += (g1 + g2) * e->dim * 8 * 4 + mx_items * (16 + 16);    // panels, items

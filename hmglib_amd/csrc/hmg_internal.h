@@ -126,7 +126,9 @@ struct Profiler {
 
 // low-rank work item: rows [row0, row0+nrows) of the row cluster's U matrix
 // (K columns, column-major with leading dim ld, U + uoff points at row0), times
-// t at T + toff
+// t at T + toff.  Mixed precision: U is the row cluster's BF16 panel (ld = padded
+// rows, K = columns padded to 16, uoff = the panel element of row row0 in column
+// group 0) and toff the first T panel group (8 slots each).
 struct LrItem {
     int row0;
     int nrows;
@@ -137,7 +139,11 @@ struct LrItem {
 };
 
 // V^T x work item: columns [0, ncols) of a 64-column slice of a column cluster's
-// transposed V (leading dim ld), rows (points) j0 .. j0+nrows-1, V + vbase at row j0
+// transposed V (leading dim ld), rows (points) j0 .. j0+nrows-1, V + vbase at row j0.
+// Mixed precision: a 128-column slice of the column cluster's BF16 panel (ld =
+// columns padded to 32), j0 = first (absolute) point group of 8, nrows = point
+// groups (a multiple of 4), vbase = the panel element of the slice's first column
+// in group j0
 // tcol0: the slice's first column in the tmap order; tp: partial-sum slot of this
 // chunk, or -1 when the slice has a single chunk and the item writes T itself
 // (mixed precision; the FP64 kernel always writes partials at its item index)
@@ -211,10 +217,27 @@ constexpr int kMxRhs = 32;
 // U T work item (up to four interleaved 32-wide MFMA tiles).
 constexpr int kMxVCols = 128;
 constexpr int kMxLrRows = 128;
-// The mixed-precision layout pads each row cluster's U to a multiple of this many
-// (zero) columns and each column cluster's V to a multiple of this many (zero)
-// rows: the low-rank kernels then reduce over whole operand batches.
-constexpr int kMxLayoutPadK = 8;
+// Mixed-precision factor panels (BF16 hi + lo, zero padded): for a row cluster C
+// with m1p = rows padded to 32 and its rank columns padded to 32, column group kg
+// (8 columns) is [hi: m1p x 8][lo: m1p x 8], U_C[i][8kg+e] at
+// 16 m1p kg + 8 i + e (+ 8 m1p for lo).  For a column cluster D with Kp = its rank
+// columns padded to 32, absolute point group g (points 8g .. 8g+7; groups from the
+// cluster's first to its last, padded to a multiple of 4) is [hi: Kp x 8][lo: Kp x 8],
+// V_D[8g+e][l] at 16 Kp (g - g_first) + 8 l + e.  T panels: slot group s/8 is
+// [hi: 32 rhs x 8][lo: 32 rhs x 8].  Elements of slack past each panel array let
+// the widest tiles read past the last cluster.
+constexpr int kMxPanelSlack = 4096;
+// placement of one ACA block in the panels
+struct MxPanel {
+    int64_t ubase;  // U panel of the block's row cluster
+    int64_t vbase;  // V panel of the block's column cluster
+    int m1p;        // padded rows of that U panel
+    int koff;       // the block's first column in the row cluster
+    int kp;         // padded columns of that V panel
+    int pos;        // the block's first column in the column cluster
+    int glo;        // first point group of the column cluster
+    int pad;
+};
 
 // per-point partial lists of one MVP schedule (CSR over sorted rows): an entry
 // v >= 0 is the P slot v, an entry v < 0 the column-slot group -(v+1)
@@ -332,7 +355,11 @@ struct Engine {
     int precision = HMG_PREC_FP64;
     int layout_prec = HMG_PREC_FP64;
     int fact_prec = HMG_PREC_FP64;
-    DevBuf<float> U32, V32;                     // FP32 factors (same grouped layout, pad 32)
+    DevBuf<__bf16> Up, Vp, Tpan;                // factor and T panels (BF16 hi + lo)
+    bool tpan_ready = false;                    // Tpan allocated and zeroed for the layout
+    int64_t up_size = 0, vp_size = 0;           // panel elements (with slack)
+    std::vector<MxPanel> h_mxp;
+    DevBuf<MxPanel> mxp;
     // near field: 64-row items against merged, group-aligned column runs
     std::vector<MxItem> h_mx_items;
     std::vector<MxRef> h_mx_refs;
@@ -344,7 +371,7 @@ struct Engine {
     Combine comb_mx;
     // per pass: X as FP32 rows of kMxRhs values, as BF16 hi/lo panels (per group of
     // 8 points: [hi|lo][rhs][8]), and coordinate panels per group ([dim][8] FP32)
-    DevBuf<float> Xs32, T32, Tp32, P32, Ys32;   // Ys32: sorted result rows of a pass
+    DevBuf<float> Tp32, P32, Ys32;              // Ys32: sorted result rows of a pass
     DevBuf<int> inv1;                           // sorted position of each original row
     bool inv1_ready = false;
     DevBuf<__bf16> Xp;
@@ -391,6 +418,16 @@ struct TreeParams {
 hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uint64_t* codes1,
                       const uint64_t* codes2, const TreeParams& tp);
 
+// host vector -> device buffer (grown as needed), on stream s
+template <class T>
+hmg_status upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s)
+{
+    HMG_TRY(d.alloc(std::max<size_t>(h.size(), 1)));
+    if (!h.empty())
+        HMG_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    return HMG_OK;
+}
+
 // ---- engine set-up (api.hip) ---------------------------------------------------
 // init_h_matrix_data; ext != nullptr runs all of the H-matrix's work on that
 // caller-owned stream (no near-field / low-rank overlap inside one product)
@@ -401,11 +438,16 @@ void engine_bind_stream(Engine* e, hipStream_t s, h_matrix_data* data);
 
 // ---- ACA (aca.hip) -------------------------------------------------------------
 // tie_last: 1 = last argmax (batched path), 0 = first argmax (unbatched path).
-// Factors in FP64; mixed precision rounds them to FP32 afterwards (round_to_f32).
+// FP64 factors straight into the FP64 layout.
 hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
                      int k, int tie_last, double* U, double* V, int* keff);
-// dst[i] = (float)src[i], i < n, on the engine stream
-hmg_status round_to_f32(Engine& eng, const double* src, float* dst, int64_t n);
+// Mixed precision: the same FP64 ACA, batch by batch through a bounded FP64
+// scratch, each batch's factors split into the BF16 hi + lo panels Up / Vp
+// (zeroed by the caller).
+hmg_status aca_build_mixed(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
+                           int tie_last, __bf16* Up, __bf16* Vp);
+// FP64 scratch elements aca_build_mixed uses for the current layout
+int64_t aca_mixed_scratch(const Engine& eng);
 
 // ---- schedule (schedule.hip, host) ---------------------------------------------------
 // row-cluster groups and near-field work items (once per tree); the matrix-free

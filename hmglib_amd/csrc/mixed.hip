@@ -2,14 +2,15 @@
 //
 // Same decomposition as the FP64 product (reference: linear_algebra.cu:4292-4379,
 // 1429-1591, 2817-3137), with up to kMxRhs = 32 right-hand sides per pass:
-//   prep      X (FP64, caller order) -> sorted FP32 rows Xs[p][32] for the
-//             low-rank product and BF16 hi/lo panels Xp for the near field
-//   lr_v32    T_D = V_D^T X_D per (column cluster, 128 columns, <= 1024 points):
-//             v_mfma_f32_32x32x2f32 tiles (M = columns, N = right-hand sides,
-//             K = points), partial sums per point chunk; lr_v32_finalize adds
-//             the chunks in order and scatters T into the U order
-//   lr_u32    Y_C = U_C T_C per 64 rows of a row cluster: two 32x32 FP32 MFMA
-//             tiles over the cluster's K columns
+//   prep      X (FP64, caller order) -> BF16 hi/lo panels Xp of the sorted points
+//             (groups of 8 points: [hi | lo][32 right-hand sides][8 points])
+//   lr_vb     T_D = V_D^T X_D per (column cluster, 128 columns, <= 4096 points):
+//             v_mfma_f32_32x32x16_bf16 tiles (M = columns, N = right-hand sides,
+//             K = points) over the BF16 hi + lo factor panels and Xp (hi*hi +
+//             hi*lo + lo*hi); single-chunk slices write T (split into BF16 hi +
+//             lo panels) directly, longer ones FP32 partials that lr_vb_finalize
+//             adds in chunk order
+//   lr_ub     Y_C = U_C T_C per 128 rows of a row cluster, same MFMA scheme
 //   near_mx   the near field evaluated on the fly in FP32: each lane evaluates
 //             the 8 kernel entries (its row, 8 consecutive columns) that it feeds
 //             to v_mfma_f32_32x32x16_bf16 as A operand, split into BF16 hi + lo;
@@ -74,18 +75,17 @@ __device__ __forceinline__ float kval_f32(const KernelParamsF& kp, float r2)
 }
 
 // ---- per-pass inputs --------------------------------------------------------------
-// Xs[p*32 + c] = X[order2[p] + (c0+c)*n2] (0 past n2 or nr); Xp panels: group
-// g = p/8 holds [hi | lo][32 right-hand sides][8 points] BF16
+// Xp panels: group g = p/8 holds [hi | lo][32 right-hand sides][8 points] BF16 of
+// X[order2[p] + (c0+c)*n2] rounded to FP32 (0 past n2 or nr)
 __global__ void __launch_bounds__(kThreads)
 prep_x_kernel(const double* __restrict__ X, const uint64_t* __restrict__ order2, int n2, int n2p, int c0,
-              int nr, float* __restrict__ Xs, __bf16* __restrict__ Xp)
+              int nr, __bf16* __restrict__ Xp)
 {
     const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (t >= (int64_t)n2p * kMxRhs) return;
     const int p = (int)(t / kMxRhs), c = (int)(t % kMxRhs);
     float v = 0.f;
     if (p < n2 && c < nr) v = (float)X[order2[p] + (int64_t)(c0 + c) * n2];
-    Xs[t] = v;
     const __bf16 h = (__bf16)v;
     const __bf16 l = (__bf16)(v - (float)h);
     const int64_t gbase = (int64_t)(p / kGroup) * 2 * kMxRhs * kGroup + c * kGroup + (p % kGroup);
@@ -217,138 +217,112 @@ near_mx_kernel(const float* __restrict__ cp1, const float* __restrict__ cp2, Ker
 //   U T:   A[row][col]   = U[col][row]     (U column-major: a column's rows contiguous)
 // Lane (c, h) of a wavefront loads NT consecutive A values of the contiguous
 // direction in one vector load: for k-step k, V[j+h][NT*c .. NT*c+NT-1] or
-// U[k+h][NT*c .. NT*c+NT-1].  Element e of that load is the A operand of MFMA
-// tile e, so tile e covers columns (rows) NT*m + e, m < 32, and one 32x32x2 FP32
-// MFMA per tile and k-step consumes the whole load.  With NT = 4 a load
-// instruction reads two contiguous 512-byte segments.
+// ---- low-rank products: BF16 hi + lo panels, v_mfma_f32_32x32x16_bf16 ----------------
+// A wavefront owns NT 32-wide output tiles; per 16-long reduction step each lane
+// loads, for every tile, the 8 hi and 8 lo BF16 values of its A row (16 B each,
+// the 32 lanes of a half-wave reading 512 contiguous bytes) and its B column's 8
+// hi and 8 lo values, and issues hi*hi + hi*lo + lo*hi per tile (lo*lo, below
+// FP32 resolution, is dropped).  The next step's operands load while the current
+// step's MFMAs run; loads are never skipped (past the last step they repeat a
+// valid position), so wait counts do not depend on the path.
 template <int NT>
-struct VecT;
-template <>
-struct VecT<1> {
-    typedef float type;
-};
-template <>
-struct VecT<2> {
-    typedef float type __attribute__((ext_vector_type(2)));
-};
-template <>
-struct VecT<4> {
-    typedef float type __attribute__((ext_vector_type(4)));
+struct BfStage {
+    bf16x8 ah[NT], al[NT];
+    bf16x8 bh, bl;
 };
 
+// a: this lane's A row (tile e at + 256 e, lo at + a_lo); b: its B column (lo at + 256)
 template <int NT>
-__device__ __forceinline__ float vget(const typename VecT<NT>::type& v, int e)
+__device__ __forceinline__ void bf_load(BfStage<NT>& st, const __bf16* __restrict__ a, int64_t a_lo,
+                                        const __bf16* __restrict__ b)
 {
-    if constexpr (NT == 1)
-        return v;
-    else
-        return v[e];
+#pragma unroll
+    for (int e = 0; e < NT; e++) {
+        st.ah[e] = *(const bf16x8*)(a + 256 * e);
+        st.al[e] = *(const bf16x8*)(a + 256 * e + a_lo);
+    }
+    st.bh = *(const bf16x8*)b;
+    st.bl = *(const bf16x8*)(b + 256);
 }
 
-// operands of kSteps MFMA k-steps (2 k each): B values and NT-wide A vectors
-constexpr int kSteps = 4;  // kBatchK must divide the layout's reduction padding (8)
-constexpr int kBatchK = 2 * kSteps;
 template <int NT>
-struct Batch {
-    float b[kSteps];
-    typename VecT<NT>::type a[kSteps];
-};
-
-// a: this lane's A stream (advances by lda per k), b: its B stream (32 floats
-// per k); k-steps k0, k0+2, k0+4, k0+6 (the lane's h is folded into a and b)
-template <int NT>
-__device__ __forceinline__ void lr_load(Batch<NT>& st, const float* __restrict__ a, int64_t lda,
-                                        const float* __restrict__ b, int k0)
+__device__ __forceinline__ void bf_mfma(f32x16 (&acc)[NT], const BfStage<NT>& st)
 {
-    typedef typename VecT<NT>::type V;
 #pragma unroll
-    for (int s = 0; s < kSteps; s++) {
-        st.b[s] = b[(int64_t)(k0 + 2 * s) * kMxRhs];
-        st.a[s] = *(const V*)(a + (int64_t)(k0 + 2 * s) * lda);
+    for (int e = 0; e < NT; e++) {
+        acc[e] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(st.ah[e], st.bh, acc[e], 0, 0, 0);
+        acc[e] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(st.ah[e], st.bl, acc[e], 0, 0, 0);
+        acc[e] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(st.al[e], st.bh, acc[e], 0, 0, 0);
     }
 }
 
+// acc[e] += sum over nsteps reduction steps (A advances a_step, B b_step per
+// step); nsteps is even (the panels pad every reduction to whole step pairs)
 template <int NT>
-__device__ __forceinline__ void lr_mfma(f32x16 (&acc)[NT], const Batch<NT>& st)
+__device__ __forceinline__ void bf_body(f32x16 (&acc)[NT], const __bf16* __restrict__ a, int64_t a_step,
+                                        int64_t a_lo, const __bf16* __restrict__ b, int64_t b_step, int nsteps)
 {
-#pragma unroll
-    for (int s = 0; s < kSteps; s++)
-#pragma unroll
-        for (int e = 0; e < NT; e++)
-            acc[e] = __builtin_amdgcn_mfma_f32_32x32x2f32(vget<NT>(st.a[s], e), st.b[s], acc[e], 0, 0, 0);
-}
-
-// acc[e] += sum over k < K of A-stream x B-stream.  The mixed-precision factor
-// layout pads every reduction length (U columns of a row cluster, V rows of a
-// column cluster) to a multiple of kBatchK with zeros, so all batches are full.
-// The next batch is loaded while the current batch's MFMAs run (two register
-// sets); the next load is never skipped (on the last step it repeats a valid
-// position) so wait counts stay path-independent, and sched_barrier keeps the
-// loads ahead of the MFMAs.
-template <int NT>
-__device__ __forceinline__ void lr_body(f32x16 (&acc)[NT], const float* __restrict__ a, int64_t lda,
-                                        const float* __restrict__ b, int K, int h)
-{
-    typedef typename VecT<NT>::type V;
-    const int kfull = K / kBatchK * kBatchK;
-    Batch<NT> A, B;
-    if (kfull > 0) lr_load<NT>(A, a, lda, b, 0);
-    for (int k = 0; k < kfull; k += 2 * kBatchK) {
-        lr_load<NT>(B, a, lda, b, min(k + kBatchK, kfull - kBatchK));
+    if (nsteps <= 0) return;
+    BfStage<NT> A, B;
+    bf_load<NT>(A, a, a_lo, b);
+    for (int s = 0; s < nsteps; s += 2) {
+        bf_load<NT>(B, a + (s + 1) * a_step, a_lo, b + (s + 1) * b_step);
         __builtin_amdgcn_sched_barrier(0);
-        lr_mfma<NT>(acc, A);
+        bf_mfma<NT>(acc, A);
         __builtin_amdgcn_sched_barrier(0);
-        if (k + kBatchK >= kfull) break;
-        lr_load<NT>(A, a, lda, b, min(k + 2 * kBatchK, kfull - kBatchK));
+        const int s2 = min(s + 2, nsteps - 1);
+        bf_load<NT>(A, a + s2 * a_step, a_lo, b + s2 * b_step);
         __builtin_amdgcn_sched_barrier(0);
-        lr_mfma<NT>(acc, B);
+        bf_mfma<NT>(acc, B);
         __builtin_amdgcn_sched_barrier(0);
-    }
-    // lengths that are not a multiple of kBatchK (not produced by the layout)
-    for (int k = kfull; k < K; k += 2) {
-        const bool ok = k + h < K;
-        const float bv = ok ? b[(int64_t)k * kMxRhs] : 0.f;
-        V av;
-        if (ok)
-            av = *(const V*)(a + (int64_t)k * lda);
-        else
-            av = V{};
-#pragma unroll
-        for (int e = 0; e < NT; e++)
-            acc[e] = __builtin_amdgcn_mfma_f32_32x32x2f32(vget<NT>(av, e), bv, acc[e], 0, 0, 0);
     }
 }
 
-// V^T X for the item's column slice (ncols <= 32*NT) over its points:
-//   tp < 0:  T[tmap[tcol0 + col]*32 + c] = result (single-chunk slice)
+// T panel element of slot t, right-hand side c (hi; lo at + 256)
+__device__ __forceinline__ int64_t tpan_at(int t, int c)
+{
+    return (int64_t)(t >> 3) * (2 * kMxRhs * kGroup) + c * kGroup + (t & 7);
+}
+
+__device__ __forceinline__ void store_split(__bf16* __restrict__ T, int64_t at, float v)
+{
+    const __bf16 h = (__bf16)v;
+    T[at] = h;
+    T[at + kMxRhs * kGroup] = (__bf16)(v - (float)h);
+}
+
+// T_D = V_D^T X for the item's columns [32 NT) of its slice over its point groups
+// (M = columns, N = right-hand sides, K = points):
+//   tp < 0:  T panel slot tmap[tcol0 + col] = result (single-chunk slice)
 //   tp >= 0: Tp[(tp*128 + col)*32 + c] = partial for this chunk of points
 template <int NT>
 __global__ void __launch_bounds__(kThreads)
-lr_v32_kernel(const LvItem* __restrict__ items, int nitems, const float* __restrict__ V,
-              const float* __restrict__ Xs, const int* __restrict__ tmap, float* __restrict__ T,
-              float* __restrict__ Tp)
+lr_vb_kernel(const LvItem* __restrict__ items, int nitems, const __bf16* __restrict__ Vp,
+             const __bf16* __restrict__ Xp, const int* __restrict__ tmap, __bf16* __restrict__ Tpan,
+             float* __restrict__ Tp)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int q = blockIdx.x * kWaves + wave_id();
     if (q >= nitems) return;
     const LvItem it = items[q];
     const int c32 = lane & 31, h = lane >> 5;
-    const int64_t ld = it.ld;
+    const int64_t gstride = 16 * (int64_t)it.ld;
     f32x16 acc[NT];
 #pragma unroll
     for (int e = 0; e < NT; e++)
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[e][r] = 0.f;
-    lr_body<NT>(acc, V + it.vbase + h * ld + NT * c32, ld, Xs + (int64_t)(it.j0 + h) * kMxRhs + c32,
-                it.nrows, h);
+    bf_body<NT>(acc, Vp + it.vbase + h * gstride + c32 * 8, 2 * gstride, 8 * (int64_t)it.ld,
+                Xp + (int64_t)(it.j0 + h) * (2 * kMxRhs * kGroup) + c32 * kGroup, 2 * (2 * kMxRhs * kGroup),
+                it.nrows / 2);
 #pragma unroll
     for (int e = 0; e < NT; e++) {
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const int col = NT * ((r >> 2) * 8 + h * 4 + (r & 3)) + e;
+            const int col = 32 * e + (r >> 2) * 8 + h * 4 + (r & 3);
             if (col < it.ncols) {
                 if (it.tp < 0)
-                    T[(int64_t)tmap[it.tcol0 + col] * kMxRhs + c32] = acc[e][r];
+                    store_split(Tpan, tpan_at(tmap[it.tcol0 + col], c32), acc[e][r]);
                 else
                     Tp[((int64_t)it.tp * kMxVCols + col) * kMxRhs + c32] = acc[e][r];
             }
@@ -356,10 +330,10 @@ lr_v32_kernel(const LvItem* __restrict__ items, int nitems, const float* __restr
     }
 }
 
-// T[tmap[tcol0 + col]*32 + c] = sum of the slice's point chunks, in chunk order
+// T panel slot tmap[tcol0 + col] = sum of the slice's point chunks, in chunk order
 __global__ void __launch_bounds__(kThreads)
-lr_v32_finalize_kernel(const LvSlice* __restrict__ slices, const float* __restrict__ Tp,
-                       const int* __restrict__ tmap, float* __restrict__ T)
+lr_vb_finalize_kernel(const LvSlice* __restrict__ slices, const float* __restrict__ Tp,
+                      const int* __restrict__ tmap, __bf16* __restrict__ Tpan)
 {
     const LvSlice sl = slices[blockIdx.x];
     for (int e = threadIdx.x; e < sl.ncols * kMxRhs; e += kThreads) {
@@ -367,34 +341,37 @@ lr_v32_finalize_kernel(const LvSlice* __restrict__ slices, const float* __restri
         float s = 0.f;
         for (int ch = 0; ch < sl.nchunks; ch++)
             s += Tp[((int64_t)(sl.item0 + ch) * kMxVCols + col) * kMxRhs + c];
-        T[(int64_t)tmap[sl.tcol0 + col] * kMxRhs + c] = s;
+        store_split(Tpan, tpan_at(tmap[sl.tcol0 + col], c), s);
     }
 }
 
 // U T: P[(q*128 + r)*32 + c] = sum_k U_C[row0 + r][k] T_C[k][c], r < nrows <= 32*NT
+// (M = rows, N = right-hand sides, K = the row cluster's rank columns)
 template <int NT>
 __global__ void __launch_bounds__(kThreads)
-lr_u32_kernel(const LrItem* __restrict__ items, int nitems, const float* __restrict__ U,
-              const float* __restrict__ T, float* __restrict__ P)
+lr_ub_kernel(const LrItem* __restrict__ items, int nitems, const __bf16* __restrict__ Up,
+             const __bf16* __restrict__ Tpan, float* __restrict__ P)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int q = blockIdx.x * kWaves + wave_id();
     if (q >= nitems) return;
     const LrItem it = items[q];
     const int c32 = lane & 31, h = lane >> 5;
-    const int64_t ld = it.ld;
+    const int64_t gstride = 16 * (int64_t)it.ld;
     f32x16 acc[NT];
 #pragma unroll
     for (int e = 0; e < NT; e++)
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[e][r] = 0.f;
-    lr_body<NT>(acc, U + it.uoff + h * ld + NT * c32, ld, T + (it.toff + h) * kMxRhs + c32, it.K, h);
+    bf_body<NT>(acc, Up + it.uoff + h * gstride + c32 * 8, 2 * gstride, 8 * (int64_t)it.ld,
+                Tpan + (it.toff + h) * (2 * kMxRhs * kGroup) + c32 * kGroup, 2 * (2 * kMxRhs * kGroup),
+                it.K / 16);
     float* __restrict__ out = P + (int64_t)q * kMxLrRows * kMxRhs + c32;
 #pragma unroll
     for (int e = 0; e < NT; e++) {
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const int rr = NT * ((r >> 2) * 8 + h * 4 + (r & 3)) + e;
+            const int rr = 32 * e + (r >> 2) * 8 + h * 4 + (r & 3);
             if (rr < it.nrows) out[(int64_t)rr * kMxRhs] = acc[e][r];
         }
     }
@@ -451,11 +428,14 @@ void launch_near_mx(bool reg, unsigned grid, hipStream_t s, const Engine& eng, c
                     float* Pn)
 {
     const bf16x8* xp = (const bf16x8*)eng.Xp.p;
+    // the LDS reservation caps near-field workgroups per CU so the low-rank
+    // kernels on the other stream keep register room beside them
+    const size_t lds = eng.serialize ? 0 : (size_t)eng.near_lds_pad;
     if (reg)
-        near_mx_kernel<KIND, DIM, true><<<grid, kThreads, 0, s>>>(
+        near_mx_kernel<KIND, DIM, true><<<grid, kThreads, lds, s>>>(
             eng.cpan[0].p, eng.cpan[1].p, kp, eng.mx_items.p, eng.n_mx_items, eng.mx_refs.p, xp, Pn);
     else
-        near_mx_kernel<KIND, DIM, false><<<grid, kThreads, 0, s>>>(
+        near_mx_kernel<KIND, DIM, false><<<grid, kThreads, lds, s>>>(
             eng.cpan[0].p, eng.cpan[1].p, kp, eng.mx_items.p, eng.n_mx_items, eng.mx_refs.p, xp, Pn);
 }
 
@@ -471,7 +451,9 @@ void launch_near_mx_dim(bool reg, unsigned grid, hipStream_t s, const Engine& en
         launch_near_mx<KIND, 0>(reg, grid, s, eng, kp, Pn);
 }
 
-inline int64_t groups_of(int n) { return (n + kGroup - 1) / kGroup + 2; }
+// point groups of the X and coordinate panels: 4 zero groups past the last point
+// (V panels of the last column cluster reach up to 3 groups past it)
+inline int64_t groups_of(int n) { return (n + kGroup - 1) / kGroup + 4; }
 
 }  // namespace
 
@@ -499,12 +481,14 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
     const int n2p = (int)(g2 * kGroup);
     const int64_t near0 = (int64_t)eng.n_lr_items * kMxLrRows;
     const int64_t slots = near0 + (int64_t)eng.n_mx_items * kMxRowsItem;
-    HMG_TRY(eng.Xs32.alloc((size_t)n2p * kMxRhs));
     HMG_TRY(eng.Xp.alloc((size_t)g2 * 2 * kMxRhs * kGroup));
-    // T rows past tsize: read (times zero U columns) by the last row cluster's
-    // padded columns; they must hold finite values
-    HMG_TRY(eng.T32.alloc((size_t)(eng.tsize + kMxLayoutPadK) * kMxRhs));
-    HMG_HIP(hipMemsetAsync(eng.T32.p + eng.tsize * kMxRhs, 0, sizeof(float) * kMxLayoutPadK * kMxRhs, s));
+    // T panels: slots no V column maps to (each row cluster's padding) stay zero
+    if (!eng.tpan_ready) {
+        const size_t n = (size_t)(eng.tsize / kGroup + 2) * 2 * kMxRhs * kGroup;
+        HMG_TRY(eng.Tpan.alloc(n));
+        HMG_HIP(hipMemsetAsync(eng.Tpan.p, 0, sizeof(__bf16) * n, s));
+        eng.tpan_ready = true;
+    }
     HMG_TRY(eng.Tp32.alloc((size_t)std::max(eng.n_tp, 1) * kMxVCols * kMxRhs));
     HMG_TRY(eng.P32.alloc((size_t)std::max<int64_t>(slots, 1) * kMxRhs));
     HMG_TRY(eng.Ys32.alloc((size_t)n1 * kMxRhs));
@@ -516,13 +500,13 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
     }
     const KernelParamsF kpf = make_params_f(kp);
     const bool reg = kp.reg != 0.0;
-    const float* U = eng.U32.p;
-    const float* V = eng.V32.p;
+    const __bf16* U = eng.Up.p;
+    const __bf16* V = eng.Vp.p;
     for (int c0 = 0; c0 < nrhs; c0 += kMxRhs) {
         const int nr = std::min(kMxRhs, nrhs - c0);
         eng.prof.begin(PROF_GATHER, s);
         prep_x_kernel<<<grid_for((int64_t)n2p * kMxRhs, kThreads), kThreads, 0, s>>>(
-            X, order2, n2, n2p, c0, nr, eng.Xs32.p, eng.Xp.p);
+            X, order2, n2, n2p, c0, nr, eng.Xp.p);
         HMG_HIP(hipGetLastError());
         eng.prof.end(PROF_GATHER, s);
         HMG_HIP(hipEventRecord(eng.ev_fork, s));
@@ -558,16 +542,16 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
                 const LvItem* li = eng.lv_items.p + b;
                 const unsigned gr = grid_for(e - b, kWaves);
                 if (g == 0)
-                    lr_v32_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xs32.p, eng.tmap.p, eng.T32.p, eng.Tp32.p);
+                    lr_vb_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xp.p, eng.tmap.p, eng.Tpan.p, eng.Tp32.p);
                 else if (g == 1)
-                    lr_v32_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xs32.p, eng.tmap.p, eng.T32.p, eng.Tp32.p);
+                    lr_vb_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xp.p, eng.tmap.p, eng.Tpan.p, eng.Tp32.p);
                 else
-                    lr_v32_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xs32.p, eng.tmap.p, eng.T32.p, eng.Tp32.p);
+                    lr_vb_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xp.p, eng.tmap.p, eng.Tpan.p, eng.Tp32.p);
                 HMG_HIP(hipGetLastError());
             }
             if (eng.n_lv_slices > 0) {
-                lr_v32_finalize_kernel<<<eng.n_lv_slices, kThreads, 0, s>>>(eng.lv_slices.p, eng.Tp32.p,
-                                                                            eng.tmap.p, eng.T32.p);
+                lr_vb_finalize_kernel<<<eng.n_lv_slices, kThreads, 0, s>>>(eng.lv_slices.p, eng.Tp32.p,
+                                                                           eng.tmap.p, eng.Tpan.p);
                 HMG_HIP(hipGetLastError());
             }
             eng.prof.end(PROF_LR_VTX, s);
@@ -580,11 +564,11 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
                 float* Pg = eng.P32.p + (int64_t)b * kMxLrRows * kMxRhs;
                 const unsigned gr = grid_for(e - b, kWaves);
                 if (g == 0)
-                    lr_u32_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.T32.p, Pg);
+                    lr_ub_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg);
                 else if (g == 1)
-                    lr_u32_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.T32.p, Pg);
+                    lr_ub_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg);
                 else
-                    lr_u32_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.T32.p, Pg);
+                    lr_ub_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg);
                 HMG_HIP(hipGetLastError());
             }
             eng.prof.end(PROF_LR_U, s);

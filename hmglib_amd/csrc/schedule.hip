@@ -79,15 +79,6 @@ void group_by_row(const std::vector<int4>& bl, int b0, int b1, int base, std::ve
     for (int b = b0; b < b1; b++) idx[fill[g_of[b - b0]]++] = b - base;
 }
 
-template <class T>
-hmg_status upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s)
-{
-    HMG_TRY(d.alloc(std::max<size_t>(h.size(), 1)));
-    if (!h.empty())
-        HMG_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
-    return HMG_OK;
-}
-
 // cut a run of column counts into chunks of at most kNearChunkCols columns
 std::vector<int> column_chunks(const std::vector<int>& m2s)
 {
@@ -332,12 +323,8 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
 {
     const int na = eng.naca;
     const bool mixed = prec == HMG_PREC_MIXED;
-    // leading dims in elements: multiples of 128 bytes
-    const int kPad = mixed ? 32 : 16;
-    // columns per V^T x item: one wavefront of columns (FP64), four 32-column MFMA
-    // tiles (mixed)
-    const int vslice = mixed ? kMxVCols : kSlice;
-    const int vchunk = mixed ? kMxVChunkRows : kVChunkRows;
+    // FP64 leading dims in elements: multiples of 128 bytes
+    const int kPad = 16;
     int ntp = 0;
     eng.h_kb.assign(na, 0);
     eng.kmax = 0;
@@ -355,118 +342,193 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
     eng.h_uoff.assign(na + 1, 0);
     eng.h_toff.assign(na + 1, 0);
     eng.h_ldu.assign(std::max(na, 1), 0);
+    eng.h_voff.assign(na + 1, 0);
+    eng.h_ldv.assign(std::max(na, 1), 0);
     std::vector<LrItem> items;
+    std::vector<LvItem> vitems;
+    std::vector<LvSlice> slices;
+    std::vector<int> tmap;
     const int lrows = mixed ? kMxLrRows : kSlice;
-    int64_t urun = 0, trun = 0;
-    const int ng = (int)eng.h_agrp_ptr.size() - 1;
-    for (int g = 0; g < ng; g++) {
-        const int gb0 = eng.h_agrp_ptr[g], gb1 = eng.h_agrp_ptr[g + 1];
-        const int4 first = eng.h_blk[eng.ndense + eng.h_agrp_blk[gb0]];
-        const int l1 = first.x, m1 = first.y - first.x + 1;
-        const int ldu = (int)round_up(m1, kPad);
-        const int64_t uC = urun, tC = trun;
-        int K = 0;
-        for (int q = gb0; q < gb1; q++) {
-            const int a = eng.h_agrp_blk[q];
-            eng.h_uoff[a] = uC + (int64_t)K * ldu;
-            eng.h_toff[a] = trun;
-            eng.h_ldu[a] = ldu;
-            trun += eng.h_kb[a];
-            K += eng.h_kb[a];
-        }
-        // mixed precision: K padded with zero columns (the T rows they meet hold
-        // the next cluster's values, or zeros past the end)
-        const int Kp = mixed ? (int)round_up(K, kMxLayoutPadK) : K;
-        urun += (int64_t)ldu * Kp;
-        if (K == 0) continue;
-        for (int r0 = 0; r0 < m1; r0 += lrows)
-            items.push_back(LrItem{l1 + r0, std::min(lrows, m1 - r0), ldu, Kp, uC + r0, tC});
-    }
-    eng.h_uoff[na] = urun;
-    eng.h_toff[na] = trun;
-    eng.tsize = trun;
-    eng.n_lr_items = (int)items.size();
     // interleaved 32-wide MFMA tiles an item of n rows or columns uses
     auto tiles_of = [](int n) { return n <= 32 ? 1 : n <= 64 ? 2 : 4; };
     for (int g = 0; g < 4; g++) eng.lr_nt_off[g] = eng.lv_nt_off[g] = 0;
-    if (mixed) {
+    if (!mixed) {
+        int64_t urun = 0, trun = 0;
+        const int ng = (int)eng.h_agrp_ptr.size() - 1;
+        for (int g = 0; g < ng; g++) {
+            const int gb0 = eng.h_agrp_ptr[g], gb1 = eng.h_agrp_ptr[g + 1];
+            const int4 first = eng.h_blk[eng.ndense + eng.h_agrp_blk[gb0]];
+            const int l1 = first.x, m1 = first.y - first.x + 1;
+            const int ldu = (int)round_up(m1, kPad);
+            const int64_t uC = urun, tC = trun;
+            int K = 0;
+            for (int q = gb0; q < gb1; q++) {
+                const int a = eng.h_agrp_blk[q];
+                eng.h_uoff[a] = uC + (int64_t)K * ldu;
+                eng.h_toff[a] = trun;
+                eng.h_ldu[a] = ldu;
+                trun += eng.h_kb[a];
+                K += eng.h_kb[a];
+            }
+            urun += (int64_t)ldu * K;
+            if (K == 0) continue;
+            for (int r0 = 0; r0 < m1; r0 += lrows)
+                items.push_back(LrItem{l1 + r0, std::min(lrows, m1 - r0), ldu, K, uC + r0, tC});
+        }
+        eng.h_uoff[na] = urun;
+        eng.h_toff[na] = trun;
+        eng.tsize = trun;
+
+        // V in column-cluster order, each cluster's V row-major (m2 x K, ld = K
+        // padded); V^T x items cover 64 columns x up to kVChunkRows points, slices
+        // reduce their row chunks
+        tmap.assign((size_t)trun, 0);
+        int64_t vrun = 0;
+        int64_t col = 0;
+        const int ncg = (int)eng.h_cgrp_ptr.size() - 1;
+        for (int g = 0; g < ncg; g++) {
+            const int gb0 = eng.h_cgrp_ptr[g], gb1 = eng.h_cgrp_ptr[g + 1];
+            const int4 first = eng.h_blk[eng.ndense + eng.h_cgrp_blk[gb0]];
+            const int l2 = first.z, m2 = first.w - first.z + 1;
+            int K = 0;
+            for (int q = gb0; q < gb1; q++) K += eng.h_kb[eng.h_cgrp_blk[q]];
+            const int ld = (int)round_up(K, kPad);
+            const int64_t vD = vrun;
+            int pos = 0;
+            for (int q = gb0; q < gb1; q++) {
+                const int a = eng.h_cgrp_blk[q];
+                eng.h_voff[a] = vD + pos;
+                eng.h_ldv[a] = ld;
+                for (int l = 0; l < eng.h_kb[a]; l++) tmap[col + pos + l] = (int)(eng.h_toff[a] + l);
+                pos += eng.h_kb[a];
+            }
+            vrun += (int64_t)m2 * ld;
+            const int nch = (m2 + kVChunkRows - 1) / kVChunkRows;
+            for (int s0 = 0; s0 < K; s0 += kSlice) {
+                const int nc = std::min(kSlice, K - s0);
+                const int tp0 = (int)vitems.size();
+                for (int c = 0; c < nch; c++) {
+                    const int j0 = c * kVChunkRows;
+                    vitems.push_back(LvItem{l2 + j0, std::min(kVChunkRows, m2 - j0), ld, nc,
+                                            vD + s0 + (int64_t)j0 * ld, (int)(col + s0), tp0 + c});
+                }
+                slices.push_back(LvSlice{tp0, nch, nc, (int)(col + s0)});
+            }
+            col += K;
+        }
+        eng.h_voff[na] = vrun;
+    } else {
+        // BF16 hi + lo panels (layout described with MxPanel).  U panels and T slots
+        // per row cluster; T slots of a cluster padded to 32 (zero slots)
+        eng.h_mxp.assign(std::max(na, 1), MxPanel{});
+        int64_t urun = 0, trun = 0;
+        const int ng = (int)eng.h_agrp_ptr.size() - 1;
+        for (int g = 0; g < ng; g++) {
+            const int gb0 = eng.h_agrp_ptr[g], gb1 = eng.h_agrp_ptr[g + 1];
+            const int4 first = eng.h_blk[eng.ndense + eng.h_agrp_blk[gb0]];
+            const int l1 = first.x, m1 = first.y - first.x + 1;
+            const int m1p = (int)round_up(m1, 32);
+            const int64_t uC = urun, tC = trun;
+            int K = 0;
+            for (int q = gb0; q < gb1; q++) {
+                const int a = eng.h_agrp_blk[q];
+                MxPanel& mp = eng.h_mxp[a];
+                mp.ubase = uC;
+                mp.m1p = m1p;
+                mp.koff = K;
+                eng.h_uoff[a] = uC;
+                eng.h_toff[a] = tC + K;
+                K += eng.h_kb[a];
+            }
+            // whole pairs of 16-column reduction steps
+            const int K32 = (int)round_up(K, 32);
+            urun += (int64_t)K32 * m1p * 2;
+            trun += K32;
+            if (K == 0) continue;
+            for (int r0 = 0; r0 < m1; r0 += lrows)
+                items.push_back(LrItem{l1 + r0, std::min(lrows, m1 - r0), m1p, K32, uC + (int64_t)r0 * 8, tC / 8});
+        }
+        eng.h_uoff[na] = urun;
+        eng.h_toff[na] = trun;
+        eng.tsize = trun;
+        eng.up_size = urun + kMxPanelSlack;
+
+        // V panels per column cluster; V^T X items: 128 columns x up to 512 point
+        // groups (4096 points); a slice with one chunk writes T itself
+        tmap.assign((size_t)std::max<int64_t>(trun, 1), 0);
+        int64_t vrun = 0;
+        int64_t col = 0;
+        const int vgroups = kMxVChunkRows / 8;
+        const int ncg = (int)eng.h_cgrp_ptr.size() - 1;
+        for (int g = 0; g < ncg; g++) {
+            const int gb0 = eng.h_cgrp_ptr[g], gb1 = eng.h_cgrp_ptr[g + 1];
+            const int4 first = eng.h_blk[eng.ndense + eng.h_cgrp_blk[gb0]];
+            const int l2 = first.z, m2 = first.w - first.z + 1;
+            int K = 0;
+            for (int q = gb0; q < gb1; q++) K += eng.h_kb[eng.h_cgrp_blk[q]];
+            const int Kp = (int)round_up(K, 32);
+            const int glo = l2 / 8, ghi = (l2 + m2 - 1) / 8;
+            // whole pairs of 16-point reduction steps
+            const int ngp = (int)round_up(ghi - glo + 1, 4);
+            const int64_t vD = vrun;
+            int pos = 0;
+            for (int q = gb0; q < gb1; q++) {
+                const int a = eng.h_cgrp_blk[q];
+                MxPanel& mp = eng.h_mxp[a];
+                mp.vbase = vD;
+                mp.kp = Kp;
+                mp.pos = pos;
+                mp.glo = glo;
+                eng.h_voff[a] = vD;
+                for (int l = 0; l < eng.h_kb[a]; l++) tmap[col + pos + l] = (int)(eng.h_toff[a] + l);
+                pos += eng.h_kb[a];
+            }
+            vrun += (int64_t)ngp * 16 * Kp;
+            const int nch = (ngp + vgroups - 1) / vgroups;
+            for (int s0 = 0; s0 < K; s0 += kMxVCols) {
+                const int nc = std::min(kMxVCols, K - s0);
+                const bool direct = nch == 1;
+                const int tp0 = direct ? -1 : ntp;
+                for (int c = 0; c < nch; c++) {
+                    const int gg0 = c * vgroups;
+                    vitems.push_back(LvItem{glo + gg0, std::min(vgroups, ngp - gg0), Kp, nc,
+                                            vD + (int64_t)gg0 * 16 * Kp + (int64_t)s0 * 8, (int)(col + s0),
+                                            direct ? -1 : tp0 + c});
+                }
+                if (!direct) {
+                    slices.push_back(LvSlice{tp0, nch, nc, (int)(col + s0)});
+                    ntp += nch;
+                }
+            }
+            col += K;
+        }
+        eng.h_voff[na] = vrun;
+        eng.vp_size = vrun + kMxPanelSlack;
         // one kernel instance per tile count: items grouped 4, 2, 1 (each item
         // carries its own rows and output slots)
         std::vector<LrItem> grouped;
         grouped.reserve(items.size());
-        int g = 0;
+        int gi = 0;
         for (int nt : {4, 2, 1}) {
-            eng.lr_nt_off[g++] = (int)grouped.size();
+            eng.lr_nt_off[gi++] = (int)grouped.size();
             for (const LrItem& it : items)
                 if (tiles_of(it.nrows) == nt) grouped.push_back(it);
         }
         eng.lr_nt_off[3] = (int)grouped.size();
         items.swap(grouped);
-    }
-
-    // V in column-cluster order, each cluster's V row-major (m2 x K, ld = K padded);
-    // V^T x items cover 64 columns x up to kVChunkRows points, slices reduce their
-    // row chunks
-    eng.h_voff.assign(na + 1, 0);
-    eng.h_ldv.assign(std::max(na, 1), 0);
-    std::vector<LvItem> vitems;
-    std::vector<LvSlice> slices;
-    std::vector<int> tmap((size_t)trun);
-    int64_t vrun = 0;
-    int64_t col = 0;
-    const int ncg = (int)eng.h_cgrp_ptr.size() - 1;
-    for (int g = 0; g < ncg; g++) {
-        const int gb0 = eng.h_cgrp_ptr[g], gb1 = eng.h_cgrp_ptr[g + 1];
-        const int4 first = eng.h_blk[eng.ndense + eng.h_cgrp_blk[gb0]];
-        const int l2 = first.z, m2 = first.w - first.z + 1;
-        int K = 0;
-        for (int q = gb0; q < gb1; q++) K += eng.h_kb[eng.h_cgrp_blk[q]];
-        const int ld = (int)round_up(K, kPad);
-        const int64_t vD = vrun;
-        int pos = 0;
-        for (int q = gb0; q < gb1; q++) {
-            const int a = eng.h_cgrp_blk[q];
-            eng.h_voff[a] = vD + pos;
-            eng.h_ldv[a] = ld;
-            for (int l = 0; l < eng.h_kb[a]; l++) tmap[col + pos + l] = (int)(eng.h_toff[a] + l);
-            pos += eng.h_kb[a];
-        }
-        // mixed precision: m2 padded with zero rows (they meet X rows of the next
-        // points, or the zero rows past the end of X)
-        const int m2p = mixed ? (int)round_up(m2, kMxLayoutPadK) : m2;
-        vrun += (int64_t)m2p * ld;
-        const int nch = (m2p + vchunk - 1) / vchunk;
-        for (int s0 = 0; s0 < K; s0 += vslice) {
-            const int nc = std::min(vslice, K - s0);
-            // mixed precision: a single-chunk slice writes T directly
-            const bool direct = mixed && nch == 1;
-            const int tp0 = direct ? -1 : (mixed ? ntp : (int)vitems.size());
-            for (int c = 0; c < nch; c++) {
-                const int j0 = c * vchunk;
-                vitems.push_back(LvItem{l2 + j0, std::min(vchunk, m2p - j0), ld, nc,
-                                        vD + s0 + (int64_t)j0 * ld, (int)(col + s0),
-                                        direct ? -1 : tp0 + c});
-            }
-            if (!direct) {
-                slices.push_back(LvSlice{tp0, nch, nc, (int)(col + s0)});
-                if (mixed) ntp += nch;
-            }
-        }
-        col += K;
-    }
-    eng.h_voff[na] = vrun;
-    if (mixed) {
-        std::vector<LvItem> grouped;
-        grouped.reserve(vitems.size());
-        int g = 0;
+        std::vector<LvItem> vgrouped;
+        vgrouped.reserve(vitems.size());
+        gi = 0;
         for (int nt : {4, 2, 1}) {
-            eng.lv_nt_off[g++] = (int)grouped.size();
+            eng.lv_nt_off[gi++] = (int)vgrouped.size();
             for (const LvItem& it : vitems)
-                if (tiles_of(it.ncols) == nt) grouped.push_back(it);
+                if (tiles_of(it.ncols) == nt) vgrouped.push_back(it);
         }
-        eng.lv_nt_off[3] = (int)grouped.size();
-        vitems.swap(grouped);
+        eng.lv_nt_off[3] = (int)vgrouped.size();
+        vitems.swap(vgrouped);
+        eng.tpan_ready = false;
     }
+    eng.n_lr_items = (int)items.size();
     eng.n_lv_items = (int)vitems.size();
     eng.n_lv_slices = (int)slices.size();
     eng.n_tp = mixed ? ntp : eng.n_lv_items;
@@ -529,6 +591,7 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
     HMG_TRY(upload(eng.lv_items, vitems, s));
     HMG_TRY(upload(eng.lv_slices, slices, s));
     HMG_TRY(upload(eng.tmap, tmap, s));
+    if (mixed) HMG_TRY(upload(eng.mxp, eng.h_mxp, s));
     if (!mixed) {
         HMG_TRY(eng.Tp.alloc((size_t)std::max(eng.n_lv_items, 1) * kSlice));
         HMG_TRY(eng.P.alloc((size_t)(col0 + eng.n_col_slots)));

@@ -164,7 +164,10 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
  *   the FP32 factors (float arrays) and a stored FP64 near field is not used.
  * HMG_OPT_ACA_ROW_SCAN [1]: when the residual-row search of the ACA rejects a
  *   row, compute the norms of the following rows four at a time (rank steps up
- *   to 64) instead of one at a time.  The factors are bitwise the same.
+ *   to 64) instead of one at a time, and give rank 0 without evaluating it to a
+ *   block whose entries are provably all too small for any row to pass the
+ *   search (decreasing kernels, bound from the points' bounding boxes).  The
+ *   factors are bitwise the same.
  * HMG_OPT_ACA_BIG_BLOCK [4096]: admissible blocks with more rows plus columns
  *   than this run their ACA on a workgroup of 4 wavefronts instead of one
  *   wavefront (row norms summed in another order; same factors up to the

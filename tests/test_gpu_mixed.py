@@ -1,17 +1,19 @@
-"""GPU parity of the mixed-precision H-MVP (FP32 factors, FP32 MFMA low-rank
-products, near field evaluated in FP32 and multiplied as BF16 hi+lo split MFMA
-GEMMs) against the FP64 engine on the same H-matrix and against the dense FP64
-product of the oracle.
+"""GPU parity of the mixed-precision H-MVP (FP64 ACA with factors stored as BF16
+hi + lo panels, low-rank products and the near field as BF16 hi+lo split MFMA
+GEMMs accumulated in FP32, near field evaluated in FP32) against the FP64 engine
+on the same H-matrix and against the dense FP64 product of the oracle.
 
 Tolerances (relative 2-norm error per right-hand side):
   mixed vs dense FP64: 1e-4, the stated mixed-precision bar;
-  mixed vs the FP64 H-MVP of the same tree: 1e-4 in general.  FP32-stored
-    residuals can flip an ACA column pivot where two candidates are within
-    rounding of each other, which moves the product by the ACA truncation error at
-    that rank (up to ~3e-5 at rank 16 for a 2D Gaussian of length scale 0.1);
+  mixed vs the FP64 H-MVP of the same tree: 1e-4 (the ACA is the FP64 one, so
+    ranks and pivots are identical; the difference is arithmetic);
   arithmetic alone, where truncation cannot matter (every block dense, or ranks
-    high enough that the ACA is exact to ~1e-9): 2e-5 (FP32 rounding of factors
-    and partial sums, ~2^-17 per entry for the BF16 hi+lo split).
+    high enough that the ACA is exact to ~1e-9), or added to the FP64 H-MVP's own
+    error: 4e-5.  Each operand is split into BF16 hi + lo (16 significant bits:
+    representation error <= 2^-18 relative) and the lo*lo term is dropped
+    (<= 2^-18), so every product carries up to ~3 * 2^-18 = 1.1e-5 relative error
+    before FP32 accumulation; sums with cancellation (smooth kernels, random
+    right-hand sides) show up to ~2.4e-5 relative to |y|.
 """
 import numpy as np
 import pytest
@@ -20,7 +22,7 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-TOL_MIXED_ARITH = 2e-5
+TOL_MIXED_ARITH = 4e-5
 TOL_MIXED_VS_FP64 = 1e-4
 TOL_MIXED_VS_DENSE = 1e-4
 
