@@ -460,8 +460,8 @@ hmg_status aca_layout(Engine& eng, int k, int prec = HMG_PREC_FP64);
 
 // ---- mixed precision (mixed.hip) -------------------------------------------------
 // Y = H X for nrhs right-hand sides (X, Y FP64 column-major in the caller's point
-// order) with FP32 factors, FP32 MFMA low-rank products and the near field
-// evaluated in FP32 and multiplied as BF16 hi+lo split MFMA GEMMs.
+// order) with the BF16 hi + lo factor panels: low-rank products and the near field
+// (evaluated in FP32) as BF16 hi+lo split MFMA GEMMs accumulated in FP32.
 hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
                       const uint64_t* order1, const uint64_t* order2, const double* X, double* Y,
                       int nrhs);

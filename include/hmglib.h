@@ -156,12 +156,12 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
  * HMG_OPT_SERIALIZE [0]: run the near field on the main stream (no overlap).
  * HMG_OPT_PRECISION [HMG_PREC_FP64]: arithmetic of the next precompute_aca and
  *   of every product.  HMG_PREC_MIXED computes the ACA in FP64 exactly as
- *   HMG_PREC_FP64 does and stores its factors rounded to FP32, runs the low-rank
- *   products as FP32 MFMA GEMMs and evaluates the near field in FP32, multiplied
- *   as BF16 hi+lo split MFMA GEMMs accumulated in FP32, for up to 32 right-hand
- *   sides per pass.
- *   X and Y stay FP64 at the boundary.  In this mode data->U and data->V point at
- *   the FP32 factors (float arrays) and a stored FP64 near field is not used.
+ *   HMG_PREC_FP64 does and stores its factors as BF16 hi + lo pairs (the value
+ *   rounded to FP32, then split), runs the low-rank products and the near field
+ *   (evaluated in FP32) as BF16 hi+lo split MFMA GEMMs accumulated in FP32, for up
+ *   to 32 right-hand sides per pass.  X and Y stay FP64 at the boundary.  In this
+ *   mode data->U and data->V point at the BF16 factor panels and a stored FP64
+ *   near field is not used.
  * HMG_OPT_ACA_ROW_SCAN [1]: when the residual-row search of the ACA rejects a
  *   row, compute the norms of the following rows four at a time (rank steps up
  *   to 64) instead of one at a time, and give rank 0 without evaluating it to a
