@@ -160,6 +160,8 @@ SIGNATURES = {
     "hmg_get_profile": (ctypes.c_int, [_D, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64)]),
     "hmg_copy_probe": (ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
     "hmg_memory": (ctypes.c_int, [_D, _P(ctypes.c_int64)]),
+    "hmg_save": (ctypes.c_int, [_D, ctypes.c_char_p]),
+    "hmg_load": (ctypes.c_int, [_D, ctypes.c_char_p]),
     # serving (include/hmglib_server.h)
     "hmg_server_create": (ctypes.c_int, [_P(ServerConfig), _P(_vp)]),
     "hmg_server_destroy": (ctypes.c_int, [_vp]),

@@ -1003,6 +1003,10 @@ hmg_status hmg_destroy_h_matrix_data(struct h_matrix_data* data)
     }
     (void)hipEventDestroy(e->ev_fork);
     (void)hipEventDestroy(e->ev_join);
+    if (e->own_kernel) {
+        if (data->assem == e->own_kernel) data->assem = nullptr;
+        hmg_kernel_free(e->own_kernel);
+    }
     if (e->own_streams) {
         (void)hipStreamDestroy(e->stream2);
         (void)hipStreamDestroy(e->stream);

@@ -344,6 +344,8 @@ struct Engine {
     // false: stream == stream2 belongs to the caller (the serving loop's slot
     // streams), which rebinds it per job and keeps its scratch buffers
     bool own_streams = true;
+    // kernel object created by hmg_load (freed with the engine)
+    system_assembler* own_kernel = nullptr;
     bool aca_row_scan = true;
     int aca_big_entries = 4096;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;

@@ -7,6 +7,7 @@ precompute_aca / precompute_dense, apply.  Every compute step is a call into
 libhmglib.so; Python only moves host arrays to and from device buffers.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -77,6 +78,25 @@ class HMatrix:
             for k_ in range(self.dim):
                 memcpy_htod(d.coords_d[s][k_], X[k_])
         self.kernel = None
+
+    # ---- on-disk format ------------------------------------------------------
+    def save(self, path):
+        """hmg_save: parameters, kernel, points and the precomputed factors."""
+        _abi.check(self.L.hmg_save(ctypes.byref(self.data), os.fsencode(path)), "hmg_save")
+
+    @classmethod
+    def load(cls, path):
+        """hmg_load: a ready H-matrix (setup rebuilt, factors installed); its kernel
+        object belongs to the library."""
+        self = cls.__new__(cls)
+        self.L = _abi.lib()
+        self.data = _abi.HMatrixData()
+        _abi.check(self.L.hmg_load(ctypes.byref(self.data), os.fsencode(path)), "hmg_load")
+        self._alive = True
+        self.dim = self.data.dim
+        self.n1, self.n2 = self.data.point_count[0], self.data.point_count[1]
+        self.kernel = None
+        return self
 
     # ---- lifecycle ---------------------------------------------------------
     def setup(self):

@@ -181,6 +181,20 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
 #define HMG_PREC_FP64 0
 #define HMG_PREC_MIXED 1
 hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value);
+/* On-disk / wire format of a built H-matrix (no counterpart in the reference,
+ * whose only persistence is debug text dumps, tree.cu:173-183, helper.cu:203-255).
+ * hmg_save writes the parameters, the kernel, the points in the caller's order,
+ * the effective ranks and the precomputed ACA factors (FP64, or the mixed
+ * precision BF16 panels) with a checksum; it needs precompute_aca first.
+ * hmg_load takes an uninitialised h_matrix_data, rebuilds the identical block list
+ * and layout (setup is deterministic and bit-exact), checks them against the file
+ * and installs the factors: products are bitwise those of the saved H-matrix.
+ * The loaded H-matrix owns its kernel object (data->assem), freed by
+ * destroy_h_matrix_data.  A file that is truncated, of another version or whose
+ * checksum fails is refused with HMG_ERR_INVALID before any device memory is
+ * touched for the header and points, and with the H-matrix destroyed otherwise. */
+hmg_status hmg_save(const struct h_matrix_data* data, const char* path);
+hmg_status hmg_load(struct h_matrix_data* data, const char* path);
 /* Per-kernel timing with HIP events recorded on the engine's streams around each
  * launch.  enable != 0 starts (and clears) recording.  Kernel classes: 0 = x
  * permutation, 1 = low-rank V^T x, 2 = near field, 3 = ACA, 4 = near-field fill,

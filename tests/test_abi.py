@@ -139,3 +139,42 @@ def test_server_host_logic_without_the_gpu():
     assert L.hmg_server_resident(h, ids, 4) == 0
     assert L.hmg_server_drain(h) == _abi.HMG_OK
     assert L.hmg_server_destroy(h) == _abi.HMG_OK
+
+
+def _store_header(**kw):
+    """A header of the H-matrix file format (layout of StoreHeader in store.hip)."""
+    import struct
+    f = dict(magic=b"HMGAMDHM", version=1, header_bytes=160, dim=2, bits=32, n0=10, n1=10, max_level=50,
+             c_leaf=4, k=2, precision=0, r1=0, r2=0, kind=3, same=1, nsym=1, pad=0, eta=1.5, eps=0.0,
+             reg=0.0, ls=0.1, nblocks=1, ndense=1, naca=0, u=0, v=0, eb=8, checksum=0)
+    f.update(kw)
+    return struct.pack("<8sII14i4d6qQ", f["magic"], f["version"], f["header_bytes"], f["dim"], f["bits"],
+                       f["n0"], f["n1"], f["max_level"], f["c_leaf"], f["k"], f["precision"], f["r1"], f["r2"],
+                       f["kind"], f["same"], f["nsym"], f["pad"], f["eta"], f["eps"], f["reg"], f["ls"],
+                       f["nblocks"], f["ndense"], f["naca"], f["u"], f["v"], f["eb"], f["checksum"])
+
+
+def test_load_refuses_bad_files_without_the_gpu(tmp_path):
+    """hmg_load checks the header and reads the points before it touches the
+    device: missing files, foreign magic, other versions, inconsistent headers and
+    truncated point arrays are refused with HMG_ERR_INVALID on a machine without
+    a GPU."""
+    L = hmglib_amd.lib()
+    d = _abi.HMatrixData()
+    assert len(_store_header()) == 160
+    cases = {
+        "missing": None,
+        "magic": _store_header(magic=b"NOTHMGLB") + bytes(160),
+        "short": _store_header()[:100],
+        "version": _store_header(version=2) + bytes(160),
+        "precision": _store_header(precision=1, eb=8) + bytes(160),
+        "dim": _store_header(dim=9) + bytes(160),
+        "points": _store_header() + bytes(80),   # 2 x 10 doubles needed, 80 bytes given
+    }
+    for name, raw in cases.items():
+        p = tmp_path / f"{name}.hmg"
+        if raw is not None:
+            p.write_bytes(raw)
+        st = L.hmg_load(ctypes.byref(d), str(p).encode())
+        assert st == _abi.HMG_ERR_INVALID, name
+        assert d.engine is None or d.engine == 0, name
