@@ -159,6 +159,17 @@ struct Team {
     }
 };
 
+// Tolerance stop of the ACA (the reference's batched path has none: its epsilon
+// is unused).  mode 1: linear_algebra.cu:498-528 with :379 corrected, checked when
+// r % 5 == 0; mode 2: checked at every step.  Stop after rank step r when
+// ||u_r|| ||v_r|| <= c ||U V^T||_F over the first r+1 terms, c = eps (1-eta) /
+// (1+eps) (mode 1) or eps (mode 2).  ||U V^T||_F^2 is kept as a running sum:
+// + 2 sum_{l<r} (u_l.u_r)(v_l.v_r) + ||u_r||^2 ||v_r||^2.
+struct AcaTol {
+    int mode;
+    double c;
+};
+
 // Rows scanned at once when the residual-row search rejects a row, and the
 // largest rank step that scans (its U terms are staged per wavefront in LDS;
 // later steps keep the one-row search).
@@ -311,11 +322,11 @@ __device__ bool block_negligible(const Team<NW>& team, Coords c1, Coords c2, con
 // computed in FP64 from the stored values and rounded to T when stored.
 // Entry j of a row (i of a column) is always handled by the same thread, so each
 // entry is computed exactly as in a single-wavefront team.
-template <class T, int NW, int KIND>
+template <class T, int NW, int KIND, bool TOL>
 __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const KernelParams& kp, int4 b, int k,
                           int64_t ld, int64_t lu, T* __restrict__ Ub, T* __restrict__ Vb, int* keff_out,
                           int tie_last, int scan, T* __restrict__ sU, double* __restrict__ sX,
-                          const work_item& wi)
+                          const work_item& wi, AcaTol tol)
 {
     const int tid = team.tid();
     constexpr int TS = Team<NW>::size;
@@ -330,6 +341,7 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
     }
     int i_r = -1;
     int ke = 0;
+    double fro2 = 0.0;  // ||U V^T||_F^2 of the terms so far (tolerance stop)
     for (int r = 0; r < k; r++) {
         bool stop = false;
         // residual-row search (compute_current_batched_v_r, :2068-2132)
@@ -401,6 +413,43 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
         }
         team.sync();
         ke = r + 1;
+        if (TOL) {
+            const T* __restrict__ ur = Ub + (int64_t)r * lu;
+            double nu = 0.0, nv = 0.0;
+            for (int i = tid; i < m1; i += TS) nu = __fma_rn((double)ur[i], (double)ur[i], nu);
+            for (int j = tid; j < m2; j += TS) {
+                const double x = (double)Vb[(int64_t)j * ld + r];
+                nv = __fma_rn(x, x, nv);
+            }
+            nu = team.sum(nu);
+            nv = team.sum(nv);
+            double cross = 0.0;
+            for (int l0 = 0; l0 < r; l0 += kScanRows) {
+                double pu[kScanRows], pv[kScanRows];
+#pragma unroll
+                for (int e = 0; e < kScanRows; e++) pu[e] = pv[e] = 0.0;
+                for (int i = tid; i < m1; i += TS) {
+                    const double x = (double)ur[i];
+#pragma unroll
+                    for (int e = 0; e < kScanRows; e++)
+                        if (l0 + e < r) pu[e] = __fma_rn((double)Ub[(int64_t)(l0 + e) * lu + i], x, pu[e]);
+                }
+                for (int j = tid; j < m2; j += TS) {
+                    const T* __restrict__ vj = Vb + (int64_t)j * ld;
+                    const double x = (double)vj[r];
+#pragma unroll
+                    for (int e = 0; e < kScanRows; e++)
+                        if (l0 + e < r) pv[e] = __fma_rn((double)vj[l0 + e], x, pv[e]);
+                }
+                team.template sum_n<kScanRows>(pu);
+                team.template sum_n<kScanRows>(pv);
+#pragma unroll
+                for (int e = 0; e < kScanRows; e++)
+                    if (l0 + e < r) cross = __fma_rn(pu[e], pv[e], cross);
+            }
+            fro2 = fro2 + 2.0 * cross + nu * nv;
+            if ((tol.mode == 2 || r % 5 == 0) && sqrt(nu) * sqrt(nv) <= tol.c * sqrt(fmax(fro2, 0.0))) break;
+        }
     }
     if (tid == 0) *keff_out = ke;
 }
@@ -411,13 +460,13 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
 // workgroup runs kAcaTeam small blocks (list[nbig ..)), one per wavefront.  Four
 // wavefronts per SIMD: the rank loop is latency-bound, so occupancy pays.
 constexpr int kAcaTeam = 4;
-template <class T, int KIND>
+template <class T, int KIND, bool TOL>
 __global__ void __launch_bounds__(kAcaTeam * kWave) __attribute__((amdgpu_waves_per_eu(4)))
 aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk,
            const work_item* __restrict__ items, const int* __restrict__ list,
            int nbig, int nsmall, const int* __restrict__ kb, const int64_t* __restrict__ uoff,
            const int* __restrict__ ldu, const int64_t* __restrict__ voff, const int* __restrict__ ldv,
-           T* __restrict__ U, T* __restrict__ V, int* __restrict__ keff, int tie_last, int scan)
+           T* __restrict__ U, T* __restrict__ V, int* __restrict__ keff, int tie_last, int scan, AcaTol tol)
 {
     __shared__ double s_v[kAcaTeam * kScanRows];
     __shared__ int s_i[kAcaTeam];
@@ -429,15 +478,15 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk,
     if ((int)blockIdx.x < nbig) {
         const int a = list[blockIdx.x];
         const Team<kAcaTeam> team{s_v, s_i, s_x};
-        aca_block<T, kAcaTeam, KIND>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
-                                     keff + a, tie_last, scan, sU[w], sX[w], items[a]);
+        aca_block<T, kAcaTeam, KIND, TOL>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
+                                     keff + a, tie_last, scan, sU[w], sX[w], items[a], tol);
     } else {
         const int q = ((int)blockIdx.x - nbig) * kAcaTeam + w;
         if (q >= nsmall) return;
         const int a = list[nbig + q];
         const Team<1> team{nullptr, nullptr, nullptr};
-        aca_block<T, 1, KIND>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
-                              keff + a, tie_last, scan, sU[w], sX[w], items[a]);
+        aca_block<T, 1, KIND, TOL>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
+                              keff + a, tie_last, scan, sU[w], sX[w], items[a], tol);
     }
 }
 
@@ -454,9 +503,18 @@ static hmg_status aca_launch(Engine& eng, const Coords& c1, const Coords& c2, co
     const int4* blk = eng.blk.p + eng.ndense;
     const unsigned grid = (unsigned)(nbig + (nsmall + kAcaTeam - 1) / kAcaTeam);
     if (grid == 0) return HMG_OK;
-    HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND><<<grid, kAcaTeam * kWave, 0, eng.stream>>>(
-                                   c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, nbig, nsmall, eng.kb.p,
-                                   uoff, eng.ldu.p, voff, eng.ldv.p, U, V, keff, tie_last, scan));
+    const AcaTol tol{eng.aca_tol_mode, eng.aca_tol_c};
+    if (tol.mode != 0) {
+        HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND, true><<<grid, kAcaTeam * kWave, 0, eng.stream>>>(
+                                       c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, nbig, nsmall,
+                                       eng.kb.p, uoff, eng.ldu.p, voff, eng.ldv.p, U, V, keff, tie_last, scan,
+                                       tol));
+    } else {
+        HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND, false><<<grid, kAcaTeam * kWave, 0, eng.stream>>>(
+                                       c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, nbig, nsmall,
+                                       eng.kb.p, uoff, eng.ldu.p, voff, eng.ldv.p, U, V, keff, tie_last, scan,
+                                       tol));
+    }
     HMG_HIP(hipGetLastError());
     return HMG_OK;
 }

@@ -245,9 +245,16 @@ void release_build(Engine& e, h_matrix_data* d)
 }
 
 // FP64 ACA factors into (U, V, keff) for the current block list and rank k
+// the tolerance stop's constant from the caller's epsilon and eta
+void set_tolerance(Engine& e, const h_matrix_data* d)
+{
+    e.aca_tol_c = e.aca_tol_mode == 1 ? d->epsilon * (1.0 - d->eta) / (1.0 + d->epsilon) : d->epsilon;
+}
+
 hmg_status build_factors(Engine& e, h_matrix_data* d, const KernelParams& kp, int k, int tie_last,
                          DevBuf<double>& U, DevBuf<double>& V)
 {
+    set_tolerance(e, d);
     HMG_TRY(aca_layout(e, k, HMG_PREC_FP64));
     const int na = e.naca;
     HMG_TRY(U.alloc(e.h_uoff[na] + 256));
@@ -263,6 +270,7 @@ hmg_status build_factors(Engine& e, h_matrix_data* d, const KernelParams& kp, in
 // panels e.Up / e.Vp
 hmg_status build_factors_mixed(Engine& e, h_matrix_data* d, const KernelParams& kp, int k, int tie_last)
 {
+    set_tolerance(e, d);
     HMG_TRY(aca_layout(e, k, HMG_PREC_MIXED));
     HMG_TRY(e.Up.alloc((size_t)e.up_size));
     HMG_TRY(e.Vp.alloc((size_t)e.vp_size));
@@ -902,6 +910,13 @@ hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value)
         return HMG_OK;
     case HMG_OPT_ACA_ROW_SCAN:
         e->aca_row_scan = value != 0;
+        return HMG_OK;
+    case HMG_OPT_ACA_TOLERANCE:
+        if (value < 0 || value > 2) {
+            set_error("ACA tolerance mode must be 0, 1 or 2");
+            return HMG_ERR_INVALID;
+        }
+        e->aca_tol_mode = (int)value;
         return HMG_OK;
     case HMG_OPT_ACA_BIG_BLOCK:
         if (value < 0 || value > (int64_t)1 << 30) {

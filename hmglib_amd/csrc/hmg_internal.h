@@ -347,6 +347,9 @@ struct Engine {
     // kernel object created by hmg_load (freed with the engine)
     system_assembler* own_kernel = nullptr;
     bool aca_row_scan = true;
+    // ACA tolerance stop: mode (option) and its constant for the current build
+    int aca_tol_mode = 0;
+    double aca_tol_c = 0.0;
     int aca_big_entries = 4096;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 

@@ -171,13 +171,22 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
  * HMG_OPT_ACA_BIG_BLOCK [4096]: admissible blocks with more rows plus columns
  *   than this run their ACA on a workgroup of 4 wavefronts instead of one
  *   wavefront (row norms summed in another order; same factors up to the
- *   rounding of those norms). */
+ *   rounding of those norms).
+ * HMG_OPT_ACA_TOLERANCE [0]: stopping rule of the ACA.  0: fixed rank k (the
+ *   reference's batched path, whose epsilon is unused).  1: the reference's
+ *   tolerance criterion (linear_algebra.cu:498-528 with the V'*V of :379
+ *   corrected), checked at every fifth rank step: stop after step r when
+ *   ||u_r|| ||v_r|| <= epsilon (1 - eta) / (1 + epsilon) ||U V^T||_F (never for
+ *   eta >= 1).  2: the plain relative criterion, checked at every step:
+ *   ||u_r|| ||v_r|| <= epsilon ||U V^T||_F.  Uses data->epsilon and data->eta at
+ *   precompute_aca; ranks stay capped at k and the factor layout keeps k columns. */
 #define HMG_OPT_NEAR_SYMMETRIC 1
 #define HMG_OPT_NEAR_LDS_BYTES 2
 #define HMG_OPT_SERIALIZE 3
 #define HMG_OPT_PRECISION 4
 #define HMG_OPT_ACA_ROW_SCAN 5
 #define HMG_OPT_ACA_BIG_BLOCK 6
+#define HMG_OPT_ACA_TOLERANCE 7
 #define HMG_PREC_FP64 0
 #define HMG_PREC_MIXED 1
 hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value);

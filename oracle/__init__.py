@@ -104,6 +104,13 @@ def lib():
         L.orc_aca_matrix.argtypes = [P(ctypes.c_double), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double),
                                      P(ctypes.c_int), P(ctypes.c_int)]
+        L.orc_aca_tol.argtypes = [ctypes.c_void_p, P(Kernel), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_double]
+        L.orc_aca_matrix_tol.restype = ctypes.c_int
+        L.orc_aca_matrix_tol.argtypes = [P(ctypes.c_double), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_int),
+                                         P(ctypes.c_int)]
         L.orc_aca_u_size.restype = ctypes.c_int64
         L.orc_aca_v_size.restype = ctypes.c_int64
         L.orc_aca_u_size.argtypes = [ctypes.c_void_p]
@@ -189,8 +196,10 @@ def find_split(codes, first, last):
     return lib().orc_find_split(_u64p(c), first, last)
 
 
-def aca_matrix(A, k, tie_last=True):
-    """ACA (reference pivot rules) of an explicit matrix; returns (U, V, row_piv, col_piv, rank)."""
+def aca_matrix(A, k, tie_last=True, tol_mode=0, eps=0.0, eta=0.0):
+    """ACA (reference pivot rules) of an explicit matrix; returns (U, V, row_piv, col_piv, rank).
+    tol_mode 1: the reference's tolerance stop (every 5th step, eps (1-eta)/(1+eps));
+    tol_mode 2: relative stop ||u_r|| ||v_r|| <= eps ||U V^T||_F at every step."""
     A = np.asfortranarray(A, dtype=np.float64)
     m1, m2 = A.shape
     kb = max(0, min(k, m1, m2))
@@ -198,9 +207,10 @@ def aca_matrix(A, k, tie_last=True):
     V = np.zeros((m2, max(kb, 1)), order="F")
     rp = np.full(max(kb, 1), -1, dtype=np.int32)
     cp = np.full(max(kb, 1), -1, dtype=np.int32)
-    r = lib().orc_aca_matrix(_dp(A), m1, m2, k, 1 if tie_last else 0, _dp(U), _dp(V),
-                             rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                             cp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    r = lib().orc_aca_matrix_tol(_dp(A), m1, m2, k, 1 if tie_last else 0, tol_mode, eps, eta,
+                                 _dp(U), _dp(V),
+                                 rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                 cp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     return U[:, :kb], V[:, :kb], rp[:kb], cp[:kb], r
 
 
@@ -279,8 +289,8 @@ class HMatrix:
     def kernel_entry(self, kern, i, j):
         return lib().orc_kernel_entry(self._h, ctypes.byref(kern), i, j)
 
-    def aca(self, kern, k, tie_last=True):
-        lib().orc_aca(self._h, ctypes.byref(kern), k, 1 if tie_last else 0)
+    def aca(self, kern, k, tie_last=True, tol_mode=0, eps=0.0, eta=0.0):
+        lib().orc_aca_tol(self._h, ctypes.byref(kern), k, 1 if tie_last else 0, tol_mode, eps, eta)
 
     def aca_sample(self, kern, k, stride):
         return lib().orc_aca_sample(self._h, ctypes.byref(kern), k, stride)

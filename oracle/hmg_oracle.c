@@ -570,8 +570,34 @@ static double dense_entry(const void* c, int i, int j)
     return d->A[(int64_t)j * d->m1 + i];
 }
 
+/* Tolerance stop after rank step r (factors u_r, v_r in place), or never when
+ * tol->mode == 0.  Mode 1 is the reference's criterion (linear_algebra.cu:498-528,
+ * the V'*V of :379 corrected from V'*U): checked when r % 5 == 0, stop when
+ * ||u_r|| ||v_r|| <= eps (1 - eta) / (1 + eps) ||U V^T||_F over the first r+1
+ * terms (never true for eta >= 1).  Mode 2 is the plain relative criterion,
+ * checked at every step: ||u_r|| ||v_r|| <= eps ||U V^T||_F.  ||U V^T||_F^2 =
+ * sum_{a,b<=r} (u_a . u_b)(v_a . v_b), summed directly. */
+static int tol_stop(const orc_tol* tol, int m1, int m2, int r, const double* U, const double* V)
+{
+    if (!tol || tol->mode == 0) return 0;
+    if (tol->mode == 1 && r % 5 != 0) return 0;
+    const double c = tol->mode == 1 ? tol->eps * (1.0 - tol->eta) / (1.0 + tol->eps) : tol->eps;
+    double fro2 = 0.0;
+    for (int a = 0; a <= r; a++)
+        for (int b = 0; b <= r; b++) {
+            double uu = 0.0, vv = 0.0;
+            for (int i = 0; i < m1; i++) uu += U[(int64_t)a * m1 + i] * U[(int64_t)b * m1 + i];
+            for (int j = 0; j < m2; j++) vv += V[(int64_t)a * m2 + j] * V[(int64_t)b * m2 + j];
+            fro2 += uu * vv;
+        }
+    double nu = 0.0, nv = 0.0;
+    for (int i = 0; i < m1; i++) nu += U[(int64_t)r * m1 + i] * U[(int64_t)r * m1 + i];
+    for (int j = 0; j < m2; j++) nv += V[(int64_t)r * m2 + j] * V[(int64_t)r * m2 + j];
+    return sqrt(nu) * sqrt(nv) <= c * sqrt(fro2 > 0.0 ? fro2 : 0.0);
+}
+
 static void aca_core(entry_fn entry, const void* ctx, int m1, int m2, int kb, int tie_last,
-                     double* U, double* V, int* keff, int* rp, int* cp)
+                     const orc_tol* tol, double* U, double* V, int* keff, int* rp, int* cp)
 {
     int i_r = -1;
     int stop = 0;
@@ -612,31 +638,39 @@ static void aca_core(entry_fn entry, const void* ctx, int m1, int m2, int kb, in
         rp[r] = i_r;
         cp[r] = jr;
         *keff = r + 1;
+        if (tol_stop(tol, m1, m2, r, U, V)) break;
     }
 }
 
 static void aca_block(const orc_hm* hm, const orc_kernel* kern, const orc_work_item* w, int kb,
-                      int tie_last, double* U, double* V, int* keff, int* rp, int* cp)
+                      int tie_last, const orc_tol* tol, double* U, double* V, int* keff, int* rp, int* cp)
 {
     block_ctx c = {hm, kern, w->set1_l, w->set2_l};
     aca_core(block_entry, &c, w->set1_u - w->set1_l + 1, w->set2_u - w->set2_l + 1, kb, tie_last,
-             U, V, keff, rp, cp);
+             tol, U, V, keff, rp, cp);
 }
 
-int orc_aca_matrix(const double* A, int m1, int m2, int k, int tie_last, double* U, double* V,
-                   int* rp, int* cp)
+int orc_aca_matrix_tol(const double* A, int m1, int m2, int k, int tie_last, int tol_mode, double eps,
+                       double eta, double* U, double* V, int* rp, int* cp)
 {
     dense_ctx c = {A, m1};
+    orc_tol tol = {tol_mode, eps, eta};
     int kb = k;
     if (kb > m1) kb = m1;
     if (kb > m2) kb = m2;
     for (int r = 0; r < kb; r++) { rp[r] = -1; cp[r] = -1; }
     int keff = 0;
-    aca_core(dense_entry, &c, m1, m2, kb, tie_last, U, V, &keff, rp, cp);
+    aca_core(dense_entry, &c, m1, m2, kb, tie_last, &tol, U, V, &keff, rp, cp);
     return keff;
 }
 
-static void aca_run(orc_hm* hm, const orc_kernel* kern, int k, int tie_last, int stride)
+int orc_aca_matrix(const double* A, int m1, int m2, int k, int tie_last, double* U, double* V,
+                   int* rp, int* cp)
+{
+    return orc_aca_matrix_tol(A, m1, m2, k, tie_last, 0, 0.0, 0.0, U, V, rp, cp);
+}
+
+static void aca_run(orc_hm* hm, const orc_kernel* kern, int k, int tie_last, int stride, const orc_tol* tol)
 {
     free(hm->kb); free(hm->keff); free(hm->uoff); free(hm->voff);
     free(hm->row_piv); free(hm->col_piv); free(hm->U); free(hm->V);
@@ -669,7 +703,7 @@ static void aca_run(orc_hm* hm, const orc_kernel* kern, int k, int tie_last, int
 #pragma omp parallel for schedule(dynamic, 16)
     for (int a = 0; a < na; a++) {
         if (hm->kb[a] == 0) continue;
-        aca_block(hm, kern, &hm->blocks[hm->ndense + a], hm->kb[a], tie_last,
+        aca_block(hm, kern, &hm->blocks[hm->ndense + a], hm->kb[a], tie_last, tol,
                   hm->U + hm->uoff[a], hm->V + hm->voff[a], &hm->keff[a],
                   hm->row_piv + (int64_t)a * kmax, hm->col_piv + (int64_t)a * kmax);
     }
@@ -677,12 +711,18 @@ static void aca_run(orc_hm* hm, const orc_kernel* kern, int k, int tie_last, int
 
 void orc_aca(orc_hm* hm, const orc_kernel* kern, int k, int tie_last)
 {
-    aca_run(hm, kern, k, tie_last, 1);
+    aca_run(hm, kern, k, tie_last, 1, NULL);
+}
+
+void orc_aca_tol(orc_hm* hm, const orc_kernel* kern, int k, int tie_last, int tol_mode, double eps, double eta)
+{
+    orc_tol tol = {tol_mode, eps, eta};
+    aca_run(hm, kern, k, tie_last, 1, &tol);
 }
 
 int64_t orc_aca_sample(orc_hm* hm, const orc_kernel* kern, int k, int stride)
 {
-    aca_run(hm, kern, k, 1, stride);
+    aca_run(hm, kern, k, 1, stride, NULL);
     int64_t e = 0;
     for (int a = 0; a < hm->naca; a++) {
         const orc_work_item* w = &hm->blocks[hm->ndense + a];

@@ -112,6 +112,14 @@ double orc_kernel_entry(const orc_hm* hm, const orc_kernel* k, int i, int j);
  * (tie_last=0, the unbatched path linear_algebra.cu:473).  Factors are stored per
  * block, column-major: U_b (m1 x k_b) at uoff[b], V_b (m2 x k_b) at voff[b]. */
 void orc_aca(orc_hm* hm, const orc_kernel* kern, int k, int tie_last);
+/* tolerance stop (not used by the reference's batched path, whose epsilon is dead):
+ * mode 1 = linear_algebra.cu:498-528 with :379 corrected (every 5th step,
+ * ||u_r|| ||v_r|| <= eps (1-eta)/(1+eps) ||U V^T||_F); mode 2 = every step,
+ * ||u_r|| ||v_r|| <= eps ||U V^T||_F */
+typedef struct { int mode; double eps; double eta; } orc_tol;
+void orc_aca_tol(orc_hm* hm, const orc_kernel* kern, int k, int tie_last, int tol_mode, double eps, double eta);
+int orc_aca_matrix_tol(const double* A, int m1, int m2, int k, int tie_last, int tol_mode, double eps,
+                       double eta, double* U, double* V, int* rp, int* cp);
 /* The same ACA on an explicit column-major m1 x m2 matrix (pivot-rule checks).
  * U (m1 x min(k,m1,m2)), V (m2 x ...), row/col pivots; returns the rank reached. */
 int orc_aca_matrix(const double* A, int m1, int m2, int k, int tie_last, double* U, double* V,

@@ -342,3 +342,64 @@ def test_aca_large_blocks_workgroup_path(gpu, kind, ls):
     y_dense = hm.apply_full(x)
     assert _rel(ym, y_dense) < _rel(y, y_dense) + 4e-5
 
+
+
+def _tol_pair(gpu, X, eta, k, kind, ls, mode, eps, prec=None):
+    hm = gpu.HMatrix(X, X, 32, eta, 50, 64, k, epsilon=eps)
+    hm.setup()
+    kern = gpu.Kernel(kind, 0.0, ls)
+    hm.set_kernel(kern)
+    hm.set_option(gpu.OPT_ACA_TOLERANCE, mode)
+    if prec is not None:
+        hm.set_precision(prec)
+    hm.precompute_aca()
+    return hm, kern
+
+
+@pytest.mark.parametrize("eps", [1e-5, 1e-8])
+def test_aca_tolerance_relative_mode_against_oracle(gpu, eps):
+    """Relative stop (mode 2) on the batching configuration's kernel (Gaussian,
+    length scale 0.1, leaf 64, eta 1.2): per-block ranks as the oracle's (sums
+    within 0.5%: the criterion compares norms at rounding level only near the
+    threshold), products within the tolerance of each other and of the dense
+    product; FP64 and mixed precision share the FP64 ACA, hence the ranks."""
+    n = 3000
+    X = _points(4401, 2, n)
+    om = O.HMatrix(X, X, 32, 1.2, 50, 64)
+    ok = O.kernel(O.K_GAUSSIAN, 2, 0.0, 0.1)
+    om.aca(ok, 32, tol_mode=2, eps=eps)
+    f = om.aca_factors()
+    keff = f["keff"]
+    assert keff.sum() < f["kb"].sum()
+    hm, kern = _tol_pair(gpu, X, 1.2, 32, O.K_GAUSSIAN, 0.1, 2, eps)
+    w = hm.work()
+    assert abs(w["rank_sum"] - int(keff.sum())) <= 5e-3 * keff.sum()
+    x = O.normal(4403, n)
+    y = hm.apply(x)
+    yd = om.dense_mvp(ok, x)
+    assert _rel(y, om.hmvp(ok, x)) < 10 * eps
+    assert _rel(y, yd) < 100 * eps
+    hmx, kx = _tol_pair(gpu, X, 1.2, 32, O.K_GAUSSIAN, 0.1, 2, eps, prec=gpu.PREC_MIXED)
+    assert hmx.work()["rank_sum"] == w["rank_sum"]
+
+
+def test_aca_tolerance_reference_mode(gpu):
+    """The reference's criterion (mode 1): with eta >= 1 it never fires, so the
+    factors are bitwise the fixed-rank ones; with eta = 0.5 it stops at ranks
+    1, 6, 11, ... as the oracle does."""
+    n = 3000
+    X = _points(4501, 2, n)
+    x = O.normal(4503, n)
+    h0, k0 = _tol_pair(gpu, X, 1.2, 32, O.K_GAUSSIAN, 0.1, 0, 1e-3)
+    h1, k1 = _tol_pair(gpu, X, 1.2, 32, O.K_GAUSSIAN, 0.1, 1, 1e-3)
+    assert h0.work() == h1.work()
+    np.testing.assert_array_equal(h0.apply(x), h1.apply(x))
+    om = O.HMatrix(X, X, 32, 0.5, 50, 64)
+    ok = O.kernel(O.K_GAUSSIAN, 2, 0.0, 0.1)
+    om.aca(ok, 32, tol_mode=1, eps=1e-4, eta=0.5)
+    keff = om.aca_factors()["keff"]
+    hm, kern = _tol_pair(gpu, X, 0.5, 32, O.K_GAUSSIAN, 0.1, 1, 1e-4)
+    w = hm.work()
+    assert abs(w["rank_sum"] - int(keff.sum())) <= 5e-3 * keff.sum()
+    assert keff.sum() < 0.9 * om.aca_factors()["kb"].sum()
+    assert _rel(hm.apply(x), om.hmvp(ok, x)) < 1e-3

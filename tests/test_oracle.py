@@ -334,6 +334,62 @@ def test_aca_rank_deficient_stops_when_rows_run_out():
     np.testing.assert_allclose(U @ V.T, A, atol=1e-12)
 
 
+def _smooth_block(m1=60, m2=50, ls=0.3):
+    """Kernel block between two separated point clusters: numerically low rank."""
+    rng = np.random.default_rng(11)
+    P = rng.random((m1, 2))
+    Q = rng.random((m2, 2)) + np.array([2.0, 0.5])
+    return np.exp(-np.sqrt(((P[:, None, :] - Q[None, :, :]) ** 2).sum(-1)) / ls)
+
+
+def _first_stop(U, V, c, every):
+    """First rank step r (r % every == 0) where ||u_r|| ||v_r|| <= c ||U V^T||_F over
+    the first r+1 terms, the norm taken of the explicit product."""
+    for r in range(U.shape[1]):
+        if r % every:
+            continue
+        fro = np.linalg.norm(U[:, :r + 1] @ V[:, :r + 1].T)
+        if np.linalg.norm(U[:, r]) * np.linalg.norm(V[:, r]) <= c * fro:
+            return r + 1
+    return U.shape[1]
+
+
+def test_aca_tolerance_stop_rank_matches_the_explicit_product():
+    """The tolerance stop only truncates the fixed-rank run (pivots are unchanged),
+    at the first checked step where the criterion holds for the explicit product
+    U V^T; mode 1 is the reference's rule (every 5th step, eps (1-eta)/(1+eps)),
+    mode 2 the plain relative rule (every step)."""
+    A = _smooth_block()
+    U, V, rp, cp, r = O.aca_matrix(A, 40)
+    assert r >= 30  # the rows' residuals reach the 1e-13 floor near rank 40
+    U, V = U[:, :r], V[:, :r]
+    for eps in (1e-3, 1e-6, 1e-9):
+        Ut, Vt, rpt, cpt, rt = O.aca_matrix(A, 40, tol_mode=2, eps=eps)
+        assert rt == _first_stop(U, V, eps, 1) < r
+        np.testing.assert_array_equal(Ut[:, :rt], U[:, :rt])
+        np.testing.assert_array_equal(cpt[:rt], cp[:rt])
+        # the classic ACA heuristic: the truncation error is of the order of eps
+        assert np.linalg.norm(A - Ut[:, :rt] @ Vt[:, :rt].T) <= 10 * eps * np.linalg.norm(A)
+    eta = 0.5
+    for eps in (1e-4, 1e-8):
+        _, _, _, _, rt = O.aca_matrix(A, 40, tol_mode=1, eps=eps, eta=eta)
+        assert rt == _first_stop(U, V, eps * (1 - eta) / (1 + eps), 5)
+        assert (rt - 1) % 5 == 0
+
+
+def test_aca_tolerance_limits():
+    """eps ~ 0 never stops early; the reference rule with eta >= 1 (every
+    configuration here) never stops, since eps (1 - eta) <= 0; smaller eps never
+    gives a smaller rank."""
+    A = _smooth_block()
+    _, _, _, _, r = O.aca_matrix(A, 30)
+    assert O.aca_matrix(A, 30, tol_mode=2, eps=1e-300)[4] == r
+    assert O.aca_matrix(A, 30, tol_mode=1, eps=1e-2, eta=1.0)[4] == r
+    assert O.aca_matrix(A, 30, tol_mode=1, eps=1e-2, eta=1.5)[4] == r
+    ranks = [O.aca_matrix(A, 30, tol_mode=2, eps=e)[4] for e in (1e-2, 1e-4, 1e-6, 1e-8, 1e-10)]
+    assert ranks == sorted(ranks) and ranks[0] < ranks[-1]
+
+
 def test_hmvp_full_rank_is_exact_against_numpy_dense():
     n = 600
     X = O.uniform(8, 2 * n).reshape(2, n)
