@@ -117,7 +117,8 @@ class BuildRequest(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("bits", ctypes.c_int), ("n", ctypes.c_int * 2),
                 ("coords", c_double_p * 2), ("eta", ctypes.c_double), ("max_level", ctypes.c_int),
                 ("c_leaf", ctypes.c_int), ("k", ctypes.c_int), ("kernel_kind", ctypes.c_int),
-                ("regularization", ctypes.c_double), ("length_scale", ctypes.c_double)]
+                ("regularization", ctypes.c_double), ("length_scale", ctypes.c_double),
+                ("epsilon", ctypes.c_double), ("aca_tolerance", ctypes.c_int)]
 
 
 class RequestInfo(ctypes.Structure):

@@ -1137,7 +1137,8 @@ int64_t hmg_struct_layout(int which, int field)
                                 OFF(hmg_build_request, eta), OFF(hmg_build_request, max_level),
                                 OFF(hmg_build_request, c_leaf), OFF(hmg_build_request, k),
                                 OFF(hmg_build_request, kernel_kind), OFF(hmg_build_request, regularization),
-                                OFF(hmg_build_request, length_scale)};
+                                OFF(hmg_build_request, length_scale), OFF(hmg_build_request, epsilon),
+                                OFF(hmg_build_request, aca_tolerance)};
         if (field < 0) return (int64_t)sizeof(hmg_build_request);
         if (field < (int)(sizeof(offs) / sizeof(offs[0]))) return offs[field];
         return -1;

@@ -237,10 +237,11 @@ hmg_status run_build(hmg_server* s, Matrix* m, Request* r, hipStream_t slot, std
         d.max_level = b.max_level;
         d.c_leaf = b.c_leaf;
         d.k = b.k;
-        d.epsilon = 0.0;
+        d.epsilon = b.epsilon;
         d.max_batched_dense_size = 1 << 20;
         d.max_batched_aca_size = 1 << 30;
         st = hmg_set_option(&d, HMG_OPT_PRECISION, s->cfg.precision);
+        if (st == HMG_OK) st = hmg_set_option(&d, HMG_OPT_ACA_TOLERANCE, b.aca_tolerance);
     }
     if (st == HMG_OK) st = hmg_setup_h_matrix(&d);
     if (st == HMG_OK) {

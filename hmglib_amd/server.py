@@ -22,9 +22,10 @@ class Server:
         self._keep = {}   # host / device buffers that must outlive their requests
 
     def submit_build(self, X1, X2, bits, eta, max_level, c_leaf, k, kind, regularization=0.0,
-                     length_scale=1.0):
+                     length_scale=1.0, epsilon=0.0, aca_tolerance=0):
         """X1, X2: (dim, n) point arrays (X2 is X1 for one point set).  Returns
-        (request id, matrix id)."""
+        (request id, matrix id).  aca_tolerance: HMG_OPT_ACA_TOLERANCE mode with
+        epsilon."""
         X1 = np.ascontiguousarray(X1, dtype=np.float64)
         same = X2 is X1
         X2 = X1 if same else np.ascontiguousarray(X2, dtype=np.float64)
@@ -36,6 +37,7 @@ class Server:
         b.coords[1] = X2.ctypes.data_as(_abi.c_double_p)
         b.eta, b.max_level, b.c_leaf, b.k = eta, max_level, c_leaf, k
         b.kernel_kind, b.regularization, b.length_scale = kind, regularization, length_scale
+        b.epsilon, b.aca_tolerance = epsilon, aca_tolerance
         r = ctypes.c_int64()
         m = ctypes.c_int64()
         _abi.check(self.L.hmg_server_submit_build(self.h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(m)),

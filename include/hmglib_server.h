@@ -53,6 +53,8 @@ typedef struct {
 	int kernel_kind;            /* HMG_KERNEL_* */
 	double regularization;
 	double length_scale;
+	double epsilon;             /* ACA tolerance (used when aca_tolerance != 0) */
+	int aca_tolerance;          /* HMG_OPT_ACA_TOLERANCE mode: 0 fixed rank k, 1, 2 */
 } hmg_build_request;
 
 typedef struct {
