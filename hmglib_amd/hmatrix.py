@@ -213,6 +213,12 @@ class HMatrix:
                 "lr_m1": out[4], "lr_m2": out[5], "partials": out[6], "near_symmetric": bool(out[7]),
                 "col_slots": out[8], "same_sets": bool(out[9]), "near_evals": out[10]}
 
+    def memory(self):
+        """hmg_memory: (bytes held now, further bytes the next build/product needs)."""
+        out = (ctypes.c_int64 * 2)()
+        _abi.check(self.L.hmg_memory(ctypes.byref(self.data), out), "hmg_memory")
+        return int(out[0]), int(out[1])
+
     def set_option(self, option, value):
         """hmg_set_option: option is one of _abi.OPT_* (near-field symmetry takes
         effect at the next setup())."""

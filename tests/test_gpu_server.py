@@ -125,7 +125,7 @@ def test_budget_admission_and_eviction(gpu):
     from hmglib_amd.hmatrix import HMatrix
     from hmglib_amd.server import Server
     case = CASES[1]
-    # measure one H-matrix's footprint, then allow two of them
+    # measure one H-matrix's footprint
     hm, kern, X = _standalone(gpu, case, gpu.PREC_MIXED)
     x = O.normal(21, case[2])
     y_ref = hm.apply(x)
@@ -135,7 +135,18 @@ def test_budget_admission_and_eviction(gpu):
     one = srv.stats()["bytes"]
     srv.destroy()
     assert one > 0
-    srv = Server(budget_bytes=int(2.5 * one), slots=2, precision=gpu.PREC_MIXED)
+    # a build is admitted at its peak (the tree's bytes plus the estimate for the
+    # factors, the FP64 ACA temporaries and the product buffers): the budget holds
+    # one built H-matrix beside one build in progress, not two
+    dim, bits, n, eta, c_leaf, k, kind, ls, seed = case
+    hb = gpu.HMatrix(X, X, bits, eta, 50, c_leaf, k)
+    hb.setup()
+    hb.set_kernel(kern)
+    hb.set_precision(gpu.PREC_MIXED)
+    need = sum(hb.memory())
+    assert need >= one
+    budget = one + need + one // 4
+    srv = Server(budget_bytes=budget, slots=2, precision=gpu.PREC_MIXED)
     try:
         ids = []
         for _ in range(4):
@@ -143,7 +154,7 @@ def test_budget_admission_and_eviction(gpu):
             assert srv.wait(r)["status"] == gpu.HMG_OK
             ids.append(m)
             st = srv.stats()
-            assert st["bytes"] <= int(2.5 * one)
+            assert st["bytes"] <= budget
         st = srv.stats()
         assert st["evictions"] == 2 and st["cached"] == 2
         assert srv.resident() == ids[2:]          # least recently used went first
