@@ -95,6 +95,64 @@ def test_block_list_bit_exact(gpu, case):
     assert hm.work()["levels"] == om.levels
 
 
+def test_code_width_without_interleave_gives_zero_codes(gpu):
+    """The reference interleaves only (dim, bits) = (2, 32), (3, 20) and (10, 4)
+    (morton.cu:21-72); any other width gives every point code 0, so each cluster
+    splits at its middle index and no block is admissible.  Engine and oracle agree
+    bit for bit, and the product is the dense one."""
+    n = 3000
+    X = _points(5, 2, n)
+    hm, kern, om, ok = _pair(gpu, X, X, 20, 1.5, 64, 16, O.K_GAUSSIAN, 0.1)
+    assert not hm.codes(0).any() and not om.codes(0).any()
+    assert om.naca == 0 and om.ndense > 1
+    _assert_blocks_equal(hm.blocks(O.WORK_ITEM_DTYPE), om.blocks(), 2)
+    x = O.normal(6, n)
+    assert _rel(hm.apply(x), om.dense_mvp(ok, x)) < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_tiny_sets_single_dense_block(gpu, prec):
+    """Fewer points than a leaf (5 x 9): one DENSE block, well inside every
+    padding the kernels use (8-point groups, 32-row tiles, 128-row items)."""
+    X1, X2 = _points(7, 2, 5), _points(8, 2, 9)
+    hm = gpu.HMatrix(X1, X2, 32, 1.5, 50, 64, 8)
+    hm.setup()
+    kern = gpu.Kernel(O.K_GAUSSIAN, 0.0, 0.5)
+    hm.set_kernel(kern)
+    hm.set_precision(gpu.PREC_MIXED if prec == "mixed" else gpu.PREC_FP64)
+    hm.precompute_aca()
+    b = hm.blocks(O.WORK_ITEM_DTYPE)
+    assert len(b) == 1 and (b["set1_l"][0], b["set1_u"][0], b["set2_l"][0], b["set2_u"][0]) == (0, 4, 0, 8)
+    om = O.HMatrix(X1, X2, 32, 1.5, 50, 64)
+    ok = O.kernel(O.K_GAUSSIAN, 2, 0.0, 0.5)
+    X = O.normal(9, 9 * 3).reshape(9, 3, order="F")
+    Y = hm.apply(X)
+    for c in range(3):
+        yd = om.dense_mvp(ok, X[:, c])
+        if prec == "fp64":
+            assert _rel(Y[:, c], yd) < 1e-14
+        else:
+            # positive entries: the BF16x3 bound is relative to sum_j K_ij |x_j|
+            mag = np.linalg.norm(om.dense_mvp(ok, np.abs(X[:, c])))
+            assert np.linalg.norm(Y[:, c] - yd) / mag < 4e-5
+
+
+@pytest.mark.parametrize("case", ["single_point", "flat_dimension"])
+def test_degenerate_extent_is_refused(gpu, case):
+    """A point set with max == min in some dimension (one point, or a constant
+    coordinate) would divide by zero in the Morton quantisation (morton.cu:119-123);
+    setup refuses it with HMG_ERR_INVALID and names the dimension."""
+    if case == "single_point":
+        X = _points(10, 3, 1)
+    else:
+        X = _points(11, 2, 100)
+        X[1, :] = 0.25
+    hm = gpu.HMatrix(X, X, 20 if X.shape[0] == 3 else 32, 1.5, 50, 64, 8)
+    with pytest.raises(gpu.HMGError, match="no extent in dimension") as e:
+        hm.setup()
+    assert e.value.status == gpu.HMG_ERR_INVALID
+
+
 def test_c1_hmvp_against_oracle_and_dense(gpu):
     """Config 1: N=4096 uniform [0,1]^2, Gaussian exp(-r^2/0.02), leaf 64, eta 1.5, k 32."""
     X = _points(1001, 2, 4096)
