@@ -515,8 +515,13 @@ def main():
     log(f"[bench] kernels {json.dumps(kernels)}")
     hm.destroy()
 
+    def log_free(what):
+        f, t = hip.mem_info()
+        log(f"[bench] device memory before {what}: {f / 2**30:.1f} GiB free of {t / 2**30:.1f}")
+
     served = None
     if mixed and not args.no_serve:
+        log_free("the serving legs")
         reqs = serving_requests(5001, 5002, args.serve_qps, args.serve_seconds)
         served = {"qps": args.serve_qps, "seconds": args.serve_seconds, "budget_GB": 200,
                   "slots": args.serve_slots,
@@ -528,6 +533,7 @@ def main():
 
     batched = None
     if mixed and not args.no_batch:
+        log_free("the batching legs")
         clouds = batch_clouds(args.batch_clouds)
         batched = {"workload": "512 clouds N=2^(10+6u), 2D/3D alternating, 8 Gaussian blobs sigma 0.05; "
                                "Gaussian l=0.1, leaf 64, eta 1.2, relative ACA tolerance 1e-5 (rank cap 32); "
@@ -535,6 +541,7 @@ def main():
                    "slots": args.serve_slots}
         for name, prec in (("fp64", hmglib_amd.PREC_FP64), ("mixed", hmglib_amd.PREC_MIXED)):
             batched[name] = batch_session(clouds, prec, args.serve_slots, name)
+            log_free(f"what follows the {name} batching leg")
 
     cpu = None
     if not args.no_cpu:

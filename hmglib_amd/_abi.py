@@ -25,6 +25,7 @@ OPT_PRECISION = 4
 OPT_ACA_ROW_SCAN = 5
 OPT_ACA_BIG_BLOCK = 6
 OPT_ACA_TOLERANCE = 7
+OPT_ACA_HUGE_BLOCK = 8
 
 PREC_FP64 = 0
 PREC_MIXED = 1

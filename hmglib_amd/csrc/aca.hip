@@ -174,6 +174,9 @@ struct AcaTol {
 // largest rank step that scans (its U terms are staged per wavefront in LDS;
 // later steps keep the one-row search).
 constexpr int kScanRows = 4;
+// Smallest block width whose rejected-row search first bounds the residual rows
+// (below it a batch of rows costs about as much as the bounds).
+constexpr int kBoundMinCols = 512;
 constexpr int kScanL = 64;
 
 // Squared 2-norms of the residual rows i0 .. i0+kScanRows-1 at rank step r (rows
@@ -282,16 +285,21 @@ __device__ bool block_negligible(const Team<NW>& team, Coords c1, Coords c2, con
     const double thr = 0.5e-13;
     // pre-screen with the tree's boxes, loosely
     double r2 = 0.0;
-    for (int d = 0; d < kp.dim; d++) {
-        const double gap = fmax(0.0, fmax(wi.min1[d] - wi.max2[d], wi.min2[d] - wi.max1[d]));
-        r2 += gap * gap;
+#pragma unroll
+    for (int d = 0; d < MAX_POINT_DIMENSION; d++) {
+        if (d < kp.dim) {
+            const double gap = fmax(0.0, fmax(wi.min1[d] - wi.max2[d], wi.min2[d] - wi.max1[d]));
+            r2 += gap * gap;
+        }
     }
     if (sqrt(m2) * kernel_of_r2_k<KIND>(kp, 0.5 * r2) >= thr) return false;
     // exact boxes of the points
     r2 = 0.0;
     const int tid = team.tid();
     constexpr int TS = Team<NW>::size;
-    for (int d = 0; d < kp.dim; d++) {
+#pragma unroll
+    for (int d = 0; d < MAX_POINT_DIMENSION; d++) {
+        if (d >= kp.dim) continue;
         double lo1 = 1e300, hi1 = -1e300, lo2 = 1e300, hi2 = -1e300;
         for (int i = b.x + tid; i <= b.y; i += TS) {
             const double x = c1.x[d][i];
@@ -314,6 +322,91 @@ __device__ bool block_negligible(const Team<NW>& team, Coords c1, Coords c2, con
     return sqrt(m2) * kernel_of_r2_k<KIND>(kp, r2 * (1.0 - 1e-9)) < thr;
 }
 
+// Kernels whose entries are bounded by k(distance to a box), k decreasing.
+template <int KIND>
+constexpr bool kind_monotone()
+{
+    return KIND == HMG_KERNEL_GAUSSIAN || KIND == HMG_KERNEL_EXPONENTIAL || KIND == HMG_KERNEL_GAUSSIAN_REF;
+}
+
+// Columns' bounding box of the block into LDS: low end of dimension d at box[d],
+// high end at box[MAX_POINT_DIMENSION + d] (each wavefront of a team writes its
+// own copy of the same values).
+template <int NW>
+__device__ void column_box(const Team<NW>& team, Coords c2, const KernelParams& kp, int4 b, double* __restrict__ box)
+{
+    const int tid = team.tid();
+    const int lane = threadIdx.x % kWave;
+    constexpr int TS = Team<NW>::size;
+#pragma unroll
+    for (int d = 0; d < MAX_POINT_DIMENSION; d++) {
+        if (d < kp.dim) {
+            double lo = 1e300, hi = -1e300;
+            for (int j = b.z + tid; j <= b.w; j += TS) {
+                const double x = c2.x[d][j];
+                lo = fmin(lo, x);
+                hi = fmax(hi, x);
+            }
+            lo = team.template extreme<false>(lo);
+            hi = team.template extreme<true>(hi);
+            if (lane == 0) {
+                box[d] = lo;
+                box[MAX_POINT_DIMENSION + d] = hi;
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// First row i >= start of the block (relative index; m1 when none) whose residual
+// at rank step r the search could accept.  Row i's residual is
+// K(i,:) - sum_{l<r} U(i,l) V(:,l)^T, so its norm is at most
+//   sqrt(m2) k(d_i) + sum_{l<r} |U(i,l)| ||V(:,l)||,
+// d_i the distance from point i to the columns' bounding box (vn[l] >= ||V(:,l)||,
+// rounded up); rows whose bound is below 1e-13 / 2 have norms below the row
+// threshold, so the search rejects them, and they are passed over unevaluated.
+// The factor two covers the rounding of the evaluated norms (the same margin as
+// block_negligible).
+template <class T, int NW, int KIND>
+__device__ int first_row_over_bound(const Team<NW>& team, Coords c1, const KernelParams& kp, int4 b, int start,
+                                    int r, const T* __restrict__ Ub, int64_t lu, const double* __restrict__ box,
+                                    const double* __restrict__ vn)
+{
+    const int m1 = b.y - b.x + 1;
+    const int tid = team.tid();
+    constexpr int TS = Team<NW>::size;
+    const double sm2 = sqrt((double)(b.w - b.z + 1));
+    const double thr = 0.5e-13;
+    int found = m1;
+    for (int i0 = start; i0 < m1; i0 += TS) {
+        double cand = (double)m1;
+        const int i = i0 + tid;
+        if (i < m1) {
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < MAX_POINT_DIMENSION; d++) {
+                if (d < kp.dim) {
+                    const double x = c1.x[d][b.x + i];
+                    const double gap = fmax(0.0, fmax(box[d] - x, x - box[MAX_POINT_DIMENSION + d]));
+                    r2 += gap * gap;
+                }
+            }
+            // r2 shrunk by far more than its rounding: the bound stays an upper bound
+            double bound = sm2 * kernel_of_r2_k<KIND>(kp, r2 * (1.0 - 1e-9));
+            for (int l = 0; l < r; l++) bound = __fma_rn(fabs((double)Ub[(int64_t)l * lu + i]), vn[l], bound);
+            if (bound * (1.0 + 1e-9) >= thr) cand = (double)i;
+        }
+        cand = team.template extreme<false>(cand);
+        if (cand < (double)m1) {
+            found = (int)cand;
+            break;
+        }
+    }
+    return found;
+}
+
 // Factor layout: U_b[i, l] at U + uoff[a] + l * ldu[a] + i (column-major, ldu >= m1:
 // the row cluster's padded height); V_b is stored transposed, V_b[j, l] at
 // V + voff[a] + j * ldv[a] + l, i.e. row j of V_b is contiguous (ldv >= total rank
@@ -326,7 +419,7 @@ template <class T, int NW, int KIND, bool TOL>
 __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const KernelParams& kp, int4 b, int k,
                           int64_t ld, int64_t lu, T* __restrict__ Ub, T* __restrict__ Vb, int* keff_out,
                           int tie_last, int scan, T* __restrict__ sU, double* __restrict__ sX,
-                          const work_item& wi, AcaTol tol)
+                          double* __restrict__ box, double* __restrict__ vn, const work_item& wi, AcaTol tol)
 {
     const int tid = team.tid();
     constexpr int TS = Team<NW>::size;
@@ -342,6 +435,10 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
     int i_r = -1;
     int ke = 0;
     double fro2 = 0.0;  // ||U V^T||_F^2 of the terms so far (tolerance stop)
+    // residual bounds for the rejected-row search (wide blocks, decreasing kernels)
+    const bool bound_rows = scan && m2 >= kBoundMinCols && kind_monotone<KIND>();
+    bool box_ready = false;
+    const int lane = threadIdx.x % kWave;
     for (int r = 0; r < k; r++) {
         bool stop = false;
         // residual-row search (compute_current_batched_v_r, :2068-2132)
@@ -365,8 +462,19 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
             // time; i_r is left just before it (the one-row search above then
             // recomputes and stores it), or at m1 - 1 when every row is rejected
             if (!scan || r > kScanL) continue;
+            // wide blocks: rows the residual bound rejects are passed over unevaluated
+            // before each batch
+            const bool skip = bound_rows && r <= kScanL;
+            if (skip && !box_ready) {
+                column_box<NW>(team, c2, kp, b, box);
+                box_ready = true;
+            }
             int found = m1;
             for (int i0 = i_r + 1; i0 < m1; i0 += kScanRows) {
+                if (skip) {
+                    i0 = first_row_over_bound<T, NW, KIND>(team, c1, kp, b, i0, r, Ub, lu, box, vn);
+                    if (i0 >= m1) break;
+                }
                 double rn[kScanRows];
                 scan_row_norms<T, NW, KIND>(team, c1, c2, kp, l1, m1, l2, m2, i0, r, ld, lu, Ub, Vb, sU, sX, rn);
                 for (int c = 0; c < kScanRows && i0 + c < m1; c++) {
@@ -400,8 +508,17 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
         team.argmax(best, bj, bval, tie_last);
         const int jr = bj;
         const double p = bval;
-        for (int j = tid; j < m2; j += TS)
-            Vb[(int64_t)j * ld + r] = (T)((double)Vb[(int64_t)j * ld + r] / p);
+        double vv = 0.0;
+        for (int j = tid; j < m2; j += TS) {
+            const T v = (T)((double)Vb[(int64_t)j * ld + r] / p);
+            Vb[(int64_t)j * ld + r] = v;
+            vv = __fma_rn((double)v, (double)v, vv);
+        }
+        if (bound_rows && r < kScanL) {
+            // ||V(:,r)|| for the residual bounds, rounded up
+            vv = team.sum(vv);
+            if (lane == 0) vn[r] = sqrt(vv) * (1.0 + 1e-9);
+        }
         team.sync();
         // residual column (batched_fill_kernel_vector_and_scaled_substraction_for_u_r)
         const T* __restrict__ vpiv = Vb + (int64_t)jr * ld;
@@ -473,49 +590,112 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk,
     __shared__ double s_x[kAcaTeam];
     __shared__ T sU[kAcaTeam][kScanRows * kScanL];
     __shared__ double sX[kAcaTeam][kScanRows * MAX_POINT_DIMENSION];
+    __shared__ double sBox[kAcaTeam][2 * MAX_POINT_DIMENSION];
+    __shared__ double sVn[kAcaTeam][kScanL];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     // large blocks first: their long rank loops start at once
     if ((int)blockIdx.x < nbig) {
         const int a = list[blockIdx.x];
         const Team<kAcaTeam> team{s_v, s_i, s_x};
         aca_block<T, kAcaTeam, KIND, TOL>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
-                                     keff + a, tie_last, scan, sU[w], sX[w], items[a], tol);
+                                     keff + a, tie_last, scan, sU[w], sX[w], sBox[w], sVn[w], items[a], tol);
     } else {
         const int q = ((int)blockIdx.x - nbig) * kAcaTeam + w;
         if (q >= nsmall) return;
         const int a = list[nbig + q];
         const Team<1> team{nullptr, nullptr, nullptr};
         aca_block<T, 1, KIND, TOL>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
-                              keff + a, tie_last, scan, sU[w], sX[w], items[a], tol);
+                              keff + a, tie_last, scan, sU[w], sX[w], sBox[w], sVn[w], items[a], tol);
     }
+}
+
+// The largest blocks, one 8-wavefront workgroup each, in a launch of their own
+// that runs beside the main one: their rank loops and rejected-row searches
+// (m1 x m2 x rank work when most rows are rejected) set the length of the build.
+// Two wavefronts per SIMD leave room for every value in registers (no scratch).
+constexpr int kAcaHuge = 8;
+template <class T, int KIND, bool TOL>
+__global__ void __launch_bounds__(kAcaHuge * kWave) __attribute__((amdgpu_waves_per_eu(2)))
+aca_huge_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk,
+                const work_item* __restrict__ items, const int* __restrict__ list,
+                const int* __restrict__ kb, const int64_t* __restrict__ uoff, const int* __restrict__ ldu,
+                const int64_t* __restrict__ voff, const int* __restrict__ ldv, T* __restrict__ U,
+                T* __restrict__ V, int* __restrict__ keff, int tie_last, int scan, AcaTol tol)
+{
+    __shared__ double s_v[kAcaHuge * kScanRows];
+    __shared__ int s_i[kAcaHuge];
+    __shared__ double s_x[kAcaHuge];
+    __shared__ T sU[kAcaHuge][kScanRows * kScanL];
+    __shared__ double sX[kAcaHuge][kScanRows * MAX_POINT_DIMENSION];
+    __shared__ double sBox[kAcaHuge][2 * MAX_POINT_DIMENSION];
+    __shared__ double sVn[kAcaHuge][kScanL];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int a = list[blockIdx.x];
+    const Team<kAcaHuge> team{s_v, s_i, s_x};
+    aca_block<T, kAcaHuge, KIND, TOL>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
+                                      keff + a, tie_last, scan, sU[w], sX[w], sBox[w], sVn[w], items[a], tol);
 }
 
 }  // namespace
 
-// one launch over list[0 .. nbig + nsmall): nbig large blocks first; factor
-// offsets uoff / voff (leading dims eng.ldu / eng.ldv) into U / V
+// Block class of the ACA launches: 0 = huge (an 8-wavefront workgroup, launched
+// beside the rest), 1 = large (a 4-wavefront workgroup), 2 = one wavefront.
+static int aca_class(const Engine& eng, int a)
+{
+    const int4 b = eng.h_blk[eng.ndense + a];
+    const int64_t e = (int64_t)(b.y - b.x + 1) + (b.w - b.z + 1);
+    return e > eng.aca_huge_entries ? 0 : e > eng.aca_big_entries ? 1 : 2;
+}
+
+// list[0 .. nhuge) huge blocks (their own launch on the second stream, joined
+// back), then nbig large and nsmall other blocks in one launch; factor offsets
+// uoff / voff (leading dims eng.ldu / eng.ldv) into U / V
 template <class T>
 static hmg_status aca_launch(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
-                             int tie_last, const int* list, int nbig, int nsmall, const int64_t* uoff,
+                             int tie_last, const int* list, int nhuge, int nbig, int nsmall, const int64_t* uoff,
                              const int64_t* voff, T* U, T* V, int* keff)
 {
     const int scan = eng.aca_row_scan ? 1 : 0;
     const int4* blk = eng.blk.p + eng.ndense;
-    const unsigned grid = (unsigned)(nbig + (nsmall + kAcaTeam - 1) / kAcaTeam);
-    if (grid == 0) return HMG_OK;
     const AcaTol tol{eng.aca_tol_mode, eng.aca_tol_c};
+    hipStream_t s = eng.stream, s2 = eng.serialize ? eng.stream : eng.stream2;
+    const bool fork = nhuge > 0 && s2 != s;
+    if (nhuge > 0) {
+        if (fork) {
+            HMG_HIP(hipEventRecord(eng.ev_fork, s));
+            HMG_HIP(hipStreamWaitEvent(s2, eng.ev_fork, 0));
+        }
+        if (tol.mode != 0) {
+            HMG_DISPATCH_KIND(kp.kind, aca_huge_kernel<T, KIND, true><<<nhuge, kAcaHuge * kWave, 0, s2>>>(
+                                           c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, eng.kb.p, uoff,
+                                           eng.ldu.p, voff, eng.ldv.p, U, V, keff, tie_last, scan, tol));
+        } else {
+            HMG_DISPATCH_KIND(kp.kind, aca_huge_kernel<T, KIND, false><<<nhuge, kAcaHuge * kWave, 0, s2>>>(
+                                           c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, eng.kb.p, uoff,
+                                           eng.ldu.p, voff, eng.ldv.p, U, V, keff, tie_last, scan, tol));
+        }
+        HMG_HIP(hipGetLastError());
+        if (fork) HMG_HIP(hipEventRecord(eng.ev_join, s2));
+        list += nhuge;
+    }
+    const unsigned grid = (unsigned)(nbig + (nsmall + kAcaTeam - 1) / kAcaTeam);
+    if (grid == 0) {
+        if (fork) HMG_HIP(hipStreamWaitEvent(s, eng.ev_join, 0));
+        return HMG_OK;
+    }
     if (tol.mode != 0) {
-        HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND, true><<<grid, kAcaTeam * kWave, 0, eng.stream>>>(
+        HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND, true><<<grid, kAcaTeam * kWave, 0, s>>>(
                                        c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, nbig, nsmall,
                                        eng.kb.p, uoff, eng.ldu.p, voff, eng.ldv.p, U, V, keff, tie_last, scan,
                                        tol));
     } else {
-        HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND, false><<<grid, kAcaTeam * kWave, 0, eng.stream>>>(
+        HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND, false><<<grid, kAcaTeam * kWave, 0, s>>>(
                                        c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, nbig, nsmall,
                                        eng.kb.p, uoff, eng.ldu.p, voff, eng.ldv.p, U, V, keff, tie_last, scan,
                                        tol));
     }
     HMG_HIP(hipGetLastError());
+    if (fork) HMG_HIP(hipStreamWaitEvent(s, eng.ev_join, 0));
     return HMG_OK;
 }
 
@@ -526,23 +706,21 @@ hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const Kern
     const int na = eng.naca;
     if (na == 0) return HMG_OK;
     hipStream_t s = eng.stream;
-    // large blocks (a workgroup each) first, then the rest (a wavefront each)
+    // huge blocks, then large blocks (a workgroup each), then the rest (a wavefront each)
     std::vector<int>& order = eng.h_aca_list;
     order.clear();
     order.reserve(na);
-    int nbig = 0;
-    for (int pass = 0; pass < 2; pass++) {
-        for (int a = 0; a < na; a++) {
-            const int4 b = eng.h_blk[eng.ndense + a];
-            const bool big = (b.y - b.x + 1) + (b.w - b.z + 1) > eng.aca_big_entries;
-            if (big == (pass == 0)) order.push_back(a);
-        }
-        if (pass == 0) nbig = (int)order.size();
+    int cnt[3] = {0, 0, 0};
+    for (int cls = 0; cls < 3; cls++) {
+        const int before = (int)order.size();
+        for (int a = 0; a < na; a++)
+            if (aca_class(eng, a) == cls) order.push_back(a);
+        cnt[cls] = (int)order.size() - before;
     }
     HMG_TRY(eng.aca_list.alloc(na));
     HMG_HIP(hipMemcpyAsync(eng.aca_list.p, order.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
     eng.prof.begin(PROF_ACA, s);
-    HMG_TRY(aca_launch<double>(eng, c1, c2, kp, tie_last, eng.aca_list.p, nbig, na - nbig, eng.uoff.p,
+    HMG_TRY(aca_launch<double>(eng, c1, c2, kp, tie_last, eng.aca_list.p, cnt[0], cnt[1], cnt[2], eng.uoff.p,
                                eng.voff.p, U, V, keff));
     eng.prof.end(PROF_ACA, s);
     return HMG_OK;
@@ -662,19 +840,18 @@ hmg_status aca_build_mixed(Engine& eng, const Coords& c1, const Coords& c2, cons
     HMG_TRY(upload(d_svoff, svoff, s));
     HMG_TRY(upload(eng.ldu, ldu, s));
     HMG_TRY(upload(eng.ldv, ldv, s));
-    // per batch: its large blocks first, then the rest (the ACA kernel's order)
+    // per batch: its huge, then large blocks, then the rest (the ACA launches' order)
     std::vector<int> list;
     list.reserve(na);
-    std::vector<int> bptr(nb + 1, 0), bbig(nb, 0);
+    std::vector<int> bptr(nb + 1, 0), bhuge(nb, 0), bbig(nb, 0);
     for (int q = 0; q < nb; q++) {
         bptr[q] = (int)list.size();
-        for (int pass = 0; pass < 2; pass++) {
-            for (int a = q; a < na; a += nb) {
-                const int4 b = eng.h_blk[eng.ndense + a];
-                const bool big = (b.y - b.x + 1) + (b.w - b.z + 1) > eng.aca_big_entries;
-                if (big == (pass == 0)) list.push_back(a);
-            }
-            if (pass == 0) bbig[q] = (int)list.size() - bptr[q];
+        for (int cls = 0; cls < 3; cls++) {
+            const int before = (int)list.size();
+            for (int a = q; a < na; a += nb)
+                if (aca_class(eng, a) == cls) list.push_back(a);
+            if (cls == 0) bhuge[q] = (int)list.size() - before;
+            if (cls == 1) bbig[q] = (int)list.size() - before;
         }
     }
     bptr[nb] = (int)list.size();
@@ -683,8 +860,8 @@ hmg_status aca_build_mixed(Engine& eng, const Coords& c1, const Coords& c2, cons
     for (int q = 0; q < nb; q++) {
         const int n = bptr[q + 1] - bptr[q];
         if (n == 0) continue;
-        HMG_TRY(aca_launch<double>(eng, c1, c2, kp, tie_last, d_list.p + bptr[q], bbig[q], n - bbig[q],
-                                   d_suoff.p, d_svoff.p, S.p, S.p, eng.keff.p));
+        HMG_TRY(aca_launch<double>(eng, c1, c2, kp, tie_last, d_list.p + bptr[q], bhuge[q], bbig[q],
+                                   n - bhuge[q] - bbig[q], d_suoff.p, d_svoff.p, S.p, S.p, eng.keff.p));
         mx_panel_kernel<<<n, kConvThreads, 0, s>>>(d_list.p + bptr[q], eng.blk.p + eng.ndense, eng.kb.p,
                                                   eng.keff.p, d_suoff.p, d_svoff.p, eng.mxp.p, S.p, Up, Vp);
         HMG_HIP(hipGetLastError());

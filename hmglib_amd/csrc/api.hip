@@ -925,6 +925,13 @@ hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value)
         }
         e->aca_big_entries = (int)value;
         return HMG_OK;
+    case HMG_OPT_ACA_HUGE_BLOCK:
+        if (value < 0 || value > (int64_t)1 << 30) {
+            set_error("ACA 16-wavefront threshold must be in [0, 2^30]");
+            return HMG_ERR_INVALID;
+        }
+        e->aca_huge_entries = (int)value;
+        return HMG_OK;
     default:
         set_error("unknown option " + std::to_string(option));
         return HMG_ERR_INVALID;
@@ -1022,13 +1029,18 @@ hmg_status hmg_destroy_h_matrix_data(struct h_matrix_data* data)
         if (data->assem == e->own_kernel) data->assem = nullptr;
         hmg_kernel_free(e->own_kernel);
     }
-    if (e->own_streams) {
-        (void)hipStreamDestroy(e->stream2);
-        (void)hipStreamDestroy(e->stream);
-    }
+    // buffers the engine still holds go back on their streams, which live until then
+    const bool own = e->own_streams;
+    hipStream_t st = e->stream, st2 = e->stream2;
     delete e;
     data->engine = nullptr;
     data->magma_queue = nullptr;
+    if (own) {
+        (void)hipStreamSynchronize(st2);
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st2);
+        (void)hipStreamDestroy(st);
+    }
     return HMG_OK;
 }
 

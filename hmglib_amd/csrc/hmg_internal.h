@@ -83,6 +83,26 @@ struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
     hipStream_t s = nullptr;  // allocation stream (nullptr: hipMalloc)
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), s(o.s) { o.p = nullptr; o.n = 0; o.s = nullptr; }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            s = o.s;
+            o.p = nullptr;
+            o.n = 0;
+            o.s = nullptr;
+        }
+        return *this;
+    }
+    // a buffer still held at scope exit (error paths included) is returned on its
+    // allocation stream
+    ~DevBuf() { release(); }
     hmg_status alloc(size_t count) {
         if (count <= n && p) return HMG_OK;
         release();
@@ -351,6 +371,7 @@ struct Engine {
     int aca_tol_mode = 0;
     double aca_tol_c = 0.0;
     int aca_big_entries = 4096;
+    int aca_huge_entries = 16384;               // above: a 16-wavefront workgroup, launched beside
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     // ---- mixed precision ------------------------------------------------------

@@ -412,10 +412,7 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
                 HMG_HIP(hipMemcpyAsync(bigger.p, out.p, sizeof(work_item) * nout,
                                        hipMemcpyDeviceToDevice, s));
             HMG_HIP(hipStreamSynchronize(s));
-            out.release();
-            out = bigger;
-            bigger.p = nullptr;
-            bigger.n = 0;
+            out = std::move(bigger);
             out_cap = cap;
         }
         HMG_TRY(next.alloc(std::max(tot_ch, 1)));

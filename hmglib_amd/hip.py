@@ -66,6 +66,14 @@ def synchronize():
     _chk(_load().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
+def trim_pool():
+    """Return the unused memory the device's default stream-ordered pool keeps
+    cached (the library frees into that pool) to the device."""
+    pool = ctypes.c_void_p()
+    _chk(_load().hipDeviceGetDefaultMemPool(ctypes.byref(pool), 0), "hipDeviceGetDefaultMemPool")
+    _chk(_load().hipMemPoolTrimTo(pool, ctypes.c_size_t(0)), "hipMemPoolTrimTo")
+
+
 def mem_info():
     f = ctypes.c_size_t(0)
     t = ctypes.c_size_t(0)

@@ -172,6 +172,9 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
  *   than this run their ACA on a workgroup of 4 wavefronts instead of one
  *   wavefront (row norms summed in another order; same factors up to the
  *   rounding of those norms).
+ * HMG_OPT_ACA_HUGE_BLOCK [16384]: blocks with more rows plus columns than this
+ *   run on a workgroup of 8 wavefronts, in a launch on the engine's second
+ *   stream beside the others (same factors up to the rounding of the norms).
  * HMG_OPT_ACA_TOLERANCE [0]: stopping rule of the ACA.  0: fixed rank k (the
  *   reference's batched path, whose epsilon is unused).  1: the reference's
  *   tolerance criterion (linear_algebra.cu:498-528 with the V'*V of :379
@@ -187,6 +190,7 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
 #define HMG_OPT_ACA_ROW_SCAN 5
 #define HMG_OPT_ACA_BIG_BLOCK 6
 #define HMG_OPT_ACA_TOLERANCE 7
+#define HMG_OPT_ACA_HUGE_BLOCK 8
 #define HMG_PREC_FP64 0
 #define HMG_PREC_MIXED 1
 hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value);

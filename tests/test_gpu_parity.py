@@ -365,12 +365,50 @@ def test_aca_row_scan_same_factors(gpu, k, ls, prec):
         assert _rel(y1, om.hmvp(ok, x)) < 1e-4
 
 
+def _blobs(seed, dim, n, sigma=0.05):
+    """8-blob Gaussian mixture in [0,1]^dim (centres uniform, clipped)."""
+    centres = O.uniform(seed, 8 * dim).reshape(8, dim)
+    lab = (O.uniform(seed + 1, n) * 8).astype(np.int64)
+    noise = O.normal(seed + 2, dim * n).reshape(dim, n)
+    return np.clip(centres[lab].T + sigma * noise, 0.0, 1.0)
+
+
+@pytest.mark.parametrize("kind,ls", [(O.K_GAUSSIAN, 0.1), (O.K_GAUSSIAN, 0.03), (O.K_EXPONENTIAL, 0.02)])
+def test_aca_first_step_row_skip_same_factors(gpu, kind, ls):
+    """Clustered points (8 blobs): in many admissible blocks the leading rows sit
+    far from the column cluster and every entry of theirs is below the row
+    threshold.  The first rank step's search skips rows whose distance to the
+    columns' bounding box proves that, unevaluated; factors, ranks and products
+    must be bitwise those of the one-row search."""
+    n = 6000
+    X = _blobs(4301, 2, n)
+    x = O.normal(4303, n)
+    res = []
+    for scan in (0, 1):
+        hm, _, om, ok = _pair(gpu, X, X, 32, 1.5, 64, 32, kind, ls, precompute=False)
+        hm.set_option(gpu.OPT_ACA_ROW_SCAN, scan)
+        hm.precompute_aca()
+        res.append((hm.work(), hm.apply(x)))
+    (w0, y0), (w1, y1) = res
+    assert w0["rank_sum"] == w1["rank_sum"] and w0["E_lr"] == w1["E_lr"]
+    np.testing.assert_array_equal(y0, y1)
+    om.aca(ok, 32)
+    keff = om.aca_factors()["keff"]
+    assert keff.sum() < 0.98 * om.aca_factors()["kb"].sum()   # searches reject rows
+    assert abs(w1["rank_sum"] - int(keff.sum())) <= 1e-3 * keff.sum()
+    # the stated FP64 bar: with clustered points many blocks run late rank steps at
+    # rounding level, where a last-ulp exp difference flips pivots (3e-8 observed)
+    assert _rel(y1, om.hmvp(ok, x)) < 1e-6
+
+
 @pytest.mark.parametrize("kind,ls", [(O.K_EXPONENTIAL, 0.1), (O.K_GAUSSIAN, 0.2)])
 def test_aca_large_blocks_workgroup_path(gpu, kind, ls):
     """N = 40000 in 2D gives admissible blocks with more than 4096 rows plus
-    columns, which run on a whole workgroup each.  Ranks and product match the
-    oracle and the all-wavefront run (threshold raised past every block), in FP64
-    and in mixed precision."""
+    columns, which run on a workgroup of 4 wavefronts each (or, with the 16384
+    threshold lowered to 4096, on 8 wavefronts in a launch beside the rest).
+    Ranks and product match the oracle, the 8-wavefront run and the
+    all-wavefront run (thresholds raised past every block), in FP64 and in mixed
+    precision."""
     n = 40000
     X = _points(4201, 2, n)
     hm, _, om, ok = _pair(gpu, X, X, 32, 1.5, 64, 16, kind, ls)
@@ -385,9 +423,19 @@ def test_aca_large_blocks_workgroup_path(gpu, kind, ls):
     x = O.normal(4203, n)
     y = hm.apply(x)
     assert _rel(y, om.hmvp(ok, x)) < TOL_FP64
+    # every large block on an 8-wavefront workgroup (launched beside the rest), and
+    # none (4-wavefront workgroups only)
+    for huge in (4096, 1 << 30):
+        hm.set_option(gpu.OPT_ACA_HUGE_BLOCK, huge)
+        hm.precompute_aca()
+        assert hm.work()["rank_sum"] == w["rank_sum"]
+        assert _rel(hm.apply(x), y) < TOL_FP64
+    hm.set_option(gpu.OPT_ACA_HUGE_BLOCK, 16384)
     hm.set_option(gpu.OPT_ACA_BIG_BLOCK, 1 << 30)
+    hm.set_option(gpu.OPT_ACA_HUGE_BLOCK, 1 << 30)
     hm.precompute_aca()
     assert _rel(hm.apply(x), y) < TOL_FP64
+    hm.set_option(gpu.OPT_ACA_HUGE_BLOCK, 16384)
     hm.set_precision(gpu.PREC_MIXED)
     hm.precompute_aca()
     ym_wave = hm.apply(x)
