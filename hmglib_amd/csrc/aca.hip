@@ -179,23 +179,22 @@ constexpr int kScanRows = 4;
 constexpr int kBoundMinCols = 512;
 constexpr int kScanL = 64;
 
-// Squared 2-norms of the residual rows i0 .. i0+kScanRows-1 at rank step r (rows
+// Squared 2-norms of the residual rows i0 .. i0+R-1 at rank step r (rows
 // past m1 - 1 give values that the caller ignores), without storing the rows.
 // Each entry and each per-thread partial sum is formed exactly as the one-row
 // search forms it (same l order of the fma chain, same j order of the squares),
 // and the team sums are the same, so a row's norm here equals its norm there
-// bitwise.  The kScanRows rows are independent chains sharing one load of each
+// bitwise.  The R rows are independent chains sharing one load of each
 // V entry, and their U terms (the same for every lane) are read from the
 // wavefront's LDS slice sU[l][c], so a scan of the many rejected rows of a
 // numerically rank-deficient block runs at arithmetic rate rather than one
 // memory latency per term.
-template <class T, int NW, int KIND>
+template <class T, int NW, int KIND, int R>
 __device__ void scan_row_norms(const Team<NW>& team, Coords c1, Coords c2, const KernelParams& kp, int l1,
                                int m1, int l2, int m2, int i0, int r, int64_t ld, int64_t lu,
                                const T* __restrict__ Ub, const T* __restrict__ Vb, T* __restrict__ sU,
-                               double* __restrict__ sX, double (&nrm)[kScanRows])
+                               double* __restrict__ sX, double (&nrm)[R])
 {
-    constexpr int R = kScanRows;
     constexpr int TS = Team<NW>::size;
     const int lane = threadIdx.x % kWave;
     int ic[R];
@@ -423,6 +422,9 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
 {
     const int tid = team.tid();
     constexpr int TS = Team<NW>::size;
+    // rows per batch of the rejected-row search: the largest teams scan twice as
+    // many at once (one load of each V entry serves all of them)
+    constexpr int SR = NW >= 8 ? 2 * kScanRows : kScanRows;
     const int l1 = b.x, m1 = b.y - b.x + 1;
     const int l2 = b.z, m2 = b.w - b.z + 1;
     if (scan && k > 0 && block_negligible<NW, KIND>(team, c1, c2, kp, b, wi)) {
@@ -458,7 +460,7 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
             }
             nrm = team.sum(nrm);
             if (sqrt(nrm) >= 1.0e-13) break;
-            // row i_r rejected: look for the next acceptable row kScanRows rows at a
+            // row i_r rejected: look for the next acceptable row SR rows at a
             // time; i_r is left just before it (the one-row search above then
             // recomputes and stores it), or at m1 - 1 when every row is rejected
             if (!scan || r > kScanL) continue;
@@ -470,14 +472,15 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
                 box_ready = true;
             }
             int found = m1;
-            for (int i0 = i_r + 1; i0 < m1; i0 += kScanRows) {
+            for (int i0 = i_r + 1; i0 < m1; i0 += SR) {
                 if (skip) {
                     i0 = first_row_over_bound<T, NW, KIND>(team, c1, kp, b, i0, r, Ub, lu, box, vn);
                     if (i0 >= m1) break;
                 }
-                double rn[kScanRows];
-                scan_row_norms<T, NW, KIND>(team, c1, c2, kp, l1, m1, l2, m2, i0, r, ld, lu, Ub, Vb, sU, sX, rn);
-                for (int c = 0; c < kScanRows && i0 + c < m1; c++) {
+                double rn[SR];
+                scan_row_norms<T, NW, KIND, SR>(team, c1, c2, kp, l1, m1, l2, m2, i0, r, ld, lu, Ub, Vb, sU, sX,
+                                                rn);
+                for (int c = 0; c < SR && i0 + c < m1; c++) {
                     if (sqrt(rn[c]) >= 1.0e-13) {
                         found = i0 + c;
                         break;
@@ -622,11 +625,11 @@ aca_huge_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ 
                 const int64_t* __restrict__ voff, const int* __restrict__ ldv, T* __restrict__ U,
                 T* __restrict__ V, int* __restrict__ keff, int tie_last, int scan, AcaTol tol)
 {
-    __shared__ double s_v[kAcaHuge * kScanRows];
+    __shared__ double s_v[kAcaHuge * 2 * kScanRows];
     __shared__ int s_i[kAcaHuge];
     __shared__ double s_x[kAcaHuge];
-    __shared__ T sU[kAcaHuge][kScanRows * kScanL];
-    __shared__ double sX[kAcaHuge][kScanRows * MAX_POINT_DIMENSION];
+    __shared__ T sU[kAcaHuge][2 * kScanRows * kScanL];
+    __shared__ double sX[kAcaHuge][2 * kScanRows * MAX_POINT_DIMENSION];
     __shared__ double sBox[kAcaHuge][2 * MAX_POINT_DIMENSION];
     __shared__ double sVn[kAcaHuge][kScanL];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);

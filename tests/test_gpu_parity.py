@@ -429,7 +429,14 @@ def test_aca_large_blocks_workgroup_path(gpu, kind, ls):
         hm.set_option(gpu.OPT_ACA_HUGE_BLOCK, huge)
         hm.precompute_aca()
         assert hm.work()["rank_sum"] == w["rank_sum"]
-        assert _rel(hm.apply(x), y) < TOL_FP64
+        yh = hm.apply(x)
+        assert _rel(yh, y) < TOL_FP64
+        if huge == 4096:
+            # the 8-wavefront teams' 8-row batched search against their one-row search
+            hm.set_option(gpu.OPT_ACA_ROW_SCAN, 0)
+            hm.precompute_aca()
+            np.testing.assert_array_equal(hm.apply(x), yh)
+            hm.set_option(gpu.OPT_ACA_ROW_SCAN, 1)
     hm.set_option(gpu.OPT_ACA_HUGE_BLOCK, 16384)
     hm.set_option(gpu.OPT_ACA_BIG_BLOCK, 1 << 30)
     hm.set_option(gpu.OPT_ACA_HUGE_BLOCK, 1 << 30)
