@@ -110,6 +110,37 @@ def test_code_width_without_interleave_gives_zero_codes(gpu):
     assert _rel(hm.apply(x), om.dense_mvp(ok, x)) < 1e-12
 
 
+@pytest.mark.parametrize("dim,prec", [(1, "fp64"), (4, "fp64"), (5, "fp64"), (1, "mixed"), (4, "mixed"),
+                                      (5, "mixed")])
+def test_dimensions_without_morton_interleave(gpu, dim, prec):
+    """Dimensions 1, 4 and 5 have no interleave in the reference (morton.cu:21-72):
+    codes are 0, every block is DENSE, and the products run the kernels' generic
+    dimension paths (coordinates in loops over dim).  Bit-exact block list; the
+    FP64 product against the oracle's dense product, mixed precision against it
+    within the BF16x3 bound (relative to sum_j K_ij |x_j|, positive kernel)."""
+    n = 1500
+    X = _points(4601 + dim, dim, n)
+    hm = gpu.HMatrix(X, X, 12, 1.5, 50, 64, 8)
+    hm.setup()
+    kern = gpu.Kernel(O.K_EXPONENTIAL, 0.0, 0.3)
+    hm.set_kernel(kern)
+    hm.set_precision(gpu.PREC_MIXED if prec == "mixed" else gpu.PREC_FP64)
+    hm.precompute_aca()
+    om = O.HMatrix(X, X, 12, 1.5, 50, 64)
+    ok = O.kernel(O.K_EXPONENTIAL, dim, 0.0, 0.3)
+    assert not hm.codes(0).any() and om.naca == 0
+    _assert_blocks_equal(hm.blocks(O.WORK_ITEM_DTYPE), om.blocks(), dim)
+    Xr = O.normal(4611, n * 2).reshape(n, 2, order="F")
+    Y = hm.apply(Xr)
+    for c in range(2):
+        yd = om.dense_mvp(ok, Xr[:, c])
+        if prec == "fp64":
+            assert _rel(Y[:, c], yd) < 1e-12
+        else:
+            mag = np.linalg.norm(om.dense_mvp(ok, np.abs(Xr[:, c])))
+            assert np.linalg.norm(Y[:, c] - yd) / mag < 4e-5
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_tiny_sets_single_dense_block(gpu, prec):
     """Fewer points than a leaf (5 x 9): one DENSE block, well inside every
