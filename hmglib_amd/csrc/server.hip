@@ -10,7 +10,8 @@
 //     least recently used idle H-matrices), ACA factors.
 // At most one job runs on an H-matrix at a time (its engine holds per-product
 // scratch); jobs on different H-matrices run concurrently on the device.  The
-// oldest waiting request decides which job a free slot takes next.
+// oldest waiting request decides which job a free slot takes next, except that
+// builds occupy at most all slots but one.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -82,6 +83,7 @@ struct hmg_server {
     std::vector<hipStream_t> slot_streams;
     bool stop = false;
     int running = 0;                  // jobs on slots
+    int running_builds = 0;           // builds among them
     int waiting = 0;                  // builds waiting for memory to free up
     int64_t used = 0;                 // bytes of cached and building H-matrices
     int64_t next_req = 1, next_mat = 1;
@@ -172,7 +174,10 @@ Matrix* pick_job(hmg_server* s, bool* is_build)
             *is_build = false;
         }
     }
-    if (!s->builds.empty()) {
+    // builds take at most all slots but one, so a product never waits for a slot
+    // behind builds (a build runs for up to a second, a product for milliseconds)
+    const int build_slots = std::max(1, s->cfg.slots - 1);
+    if (!s->builds.empty() && s->running_builds < build_slots) {
         Matrix* m = s->mats.at(s->builds.front()).get();
         const double t = m->build_req->info.t_submit;
         if (!best || t < best_t) {
@@ -342,7 +347,9 @@ void worker(hmg_server* s, int slot_index)
             Request* r = m->build_req;
             m->build_req = nullptr;
             r->info.t_start = t_start;
+            s->running_builds++;
             const hmg_status st = run_build(s, m, r, slot, lk);
+            s->running_builds--;
             const double t_end = s->now();
             if (st == HMG_OK) {
                 m->ready = true;

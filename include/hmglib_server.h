@@ -11,7 +11,9 @@
  *     independently, so a result is bitwise the same alone and in the mix).
  *   - Up to `slots` jobs (passes or builds on different H-matrices) run on the
  *     device at once, each on its H-matrix's own streams; requests arriving while
- *     others run are admitted as slots free up.
+ *     others run are admitted as slots free up.  Builds hold at most slots - 1
+ *     of them (all of them when slots is 1), so products never wait behind builds
+ *     for a slot.
  *   - A build is admitted when its device memory (measured after the tree is
  *     built, predicted for the factors and product buffers) fits in
  *     hbm_budget_bytes next to the cached H-matrices; least recently used idle
