@@ -252,7 +252,9 @@ def serve_session(reqs, precision, budget, slots, log_tag, trace_path=None):
     # outputs: a ring of buffers, each large enough for any request
     Xpool = hip.DeviceArray(np.asfortranarray(np.random.default_rng(7).standard_normal((nmax * 16,))))
     ring = [hip.DeviceArray.empty(nmax * 16) for _ in range(96)]
-    srv = Server(budget_bytes=budget, slots=slots, max_rhs_per_pass=32, precision=precision)
+    # one slot kept for products: applies to cached H-matrices never wait behind builds
+    srv = Server(budget_bytes=budget, slots=slots, max_rhs_per_pass=32, precision=precision,
+                 product_slots=1)
     ids = []
     mat_n = {}
     t0 = time.perf_counter()
@@ -524,7 +526,7 @@ def main():
         log_free("the serving legs")
         reqs = serving_requests(5001, 5002, args.serve_qps, args.serve_seconds)
         served = {"qps": args.serve_qps, "seconds": args.serve_seconds, "budget_GB": 200,
-                  "slots": args.serve_slots,
+                  "slots": args.serve_slots, "product_slots": 1,
                   "workload": "Poisson arrivals; N=2^(12+6u); uniform or 8-blob; 2D/3D; exp or Gaussian "
                               "l=0.1; 1-16 RHS; 70% applies to cached H-matrices, 30% builds"}
         for name, prec in (("fp64", hmglib_amd.PREC_FP64), ("mixed", hmglib_amd.PREC_MIXED)):

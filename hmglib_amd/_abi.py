@@ -110,7 +110,8 @@ class HMatrixData(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     """hmg_server_config (include/hmglib_server.h)."""
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("slots", ctypes.c_int),
-                ("max_rhs_per_pass", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("max_rhs_per_pass", ctypes.c_int), ("precision", ctypes.c_int),
+                ("product_slots", ctypes.c_int)]
 
 
 class BuildRequest(ctypes.Structure):

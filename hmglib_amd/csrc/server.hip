@@ -11,7 +11,7 @@
 // At most one job runs on an H-matrix at a time (its engine holds per-product
 // scratch); jobs on different H-matrices run concurrently on the device.  The
 // oldest waiting request decides which job a free slot takes next, except that
-// builds occupy at most all slots but one.
+// builds never occupy the last product_slots slots.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -174,9 +174,9 @@ Matrix* pick_job(hmg_server* s, bool* is_build)
             *is_build = false;
         }
     }
-    // builds take at most all slots but one, so a product never waits for a slot
-    // behind builds (a build runs for up to a second, a product for milliseconds)
-    const int build_slots = std::max(1, s->cfg.slots - 1);
+    // builds leave product_slots slots to product passes (a build runs for up to a
+    // second, a product for milliseconds)
+    const int build_slots = s->cfg.slots - s->cfg.product_slots;
     if (!s->builds.empty() && s->running_builds < build_slots) {
         Matrix* m = s->mats.at(s->builds.front()).get();
         const double t = m->build_req->info.t_submit;
@@ -427,7 +427,8 @@ hmg_status hmg_server_create(const hmg_server_config* cfg, hmg_server** out)
     hmg_server_config c{(int64_t)200e9, 4, kMxRhs, HMG_PREC_MIXED};
     if (cfg) c = *cfg;
     if (c.hbm_budget_bytes <= 0 || c.slots < 1 || c.slots > 64 || c.max_rhs_per_pass < 1 ||
-        (c.precision != HMG_PREC_FP64 && c.precision != HMG_PREC_MIXED)) {
+        (c.precision != HMG_PREC_FP64 && c.precision != HMG_PREC_MIXED) || c.product_slots < 0 ||
+        c.product_slots >= c.slots) {
         set_error("invalid server configuration");
         return HMG_ERR_INVALID;
     }

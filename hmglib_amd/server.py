@@ -13,9 +13,11 @@ from . import _abi
 
 
 class Server:
-    def __init__(self, budget_bytes=200e9, slots=4, max_rhs_per_pass=32, precision=_abi.PREC_MIXED):
+    def __init__(self, budget_bytes=200e9, slots=4, max_rhs_per_pass=32, precision=_abi.PREC_MIXED,
+                 product_slots=0):
         self.L = _abi.lib()
-        cfg = _abi.ServerConfig(int(budget_bytes), int(slots), int(max_rhs_per_pass), int(precision))
+        cfg = _abi.ServerConfig(int(budget_bytes), int(slots), int(max_rhs_per_pass), int(precision),
+                                int(product_slots))
         h = ctypes.c_void_p()
         _abi.check(self.L.hmg_server_create(ctypes.byref(cfg), ctypes.byref(h)), "hmg_server_create")
         self.h = h

@@ -11,8 +11,8 @@
  *     independently, so a result is bitwise the same alone and in the mix).
  *   - Up to `slots` jobs (passes or builds on different H-matrices) run on the
  *     device at once, each on its H-matrix's own streams; requests arriving while
- *     others run are admitted as slots free up.  Builds hold at most slots - 1
- *     of them (all of them when slots is 1), so products never wait behind builds
+ *     others run are admitted as slots free up.  Builds never hold the last
+ *     product_slots of them, so products need not wait behind second-long builds
  *     for a slot.
  *   - A build is admitted when its device memory (measured after the tree is
  *     built, predicted for the factors and product buffers) fits in
@@ -38,6 +38,8 @@ typedef struct {
 	int slots;                  /* jobs on the device at once (default 4) */
 	int max_rhs_per_pass;       /* right-hand sides merged into one pass (default 32) */
 	int precision;              /* HMG_PREC_FP64 or HMG_PREC_MIXED (default mixed) */
+	int product_slots;          /* slots builds never take, kept for product passes
+	                               (0 .. slots - 1; default 0) */
 } hmg_server_config;
 
 typedef struct {

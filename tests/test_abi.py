@@ -116,6 +116,8 @@ def test_server_host_logic_without_the_gpu():
     assert L.hmg_server_create(ctypes.byref(bad), ctypes.byref(h)) == _abi.HMG_ERR_INVALID
     bad = _abi.ServerConfig(1 << 30, 0, 32, _abi.PREC_MIXED)
     assert L.hmg_server_create(ctypes.byref(bad), ctypes.byref(h)) == _abi.HMG_ERR_INVALID
+    bad = _abi.ServerConfig(1 << 30, 2, 32, _abi.PREC_MIXED, 2)   # every slot kept from builds
+    assert L.hmg_server_create(ctypes.byref(bad), ctypes.byref(h)) == _abi.HMG_ERR_INVALID
     bad = _abi.ServerConfig(1 << 30, 2, 32, 7)
     assert L.hmg_server_create(ctypes.byref(bad), ctypes.byref(h)) == _abi.HMG_ERR_INVALID
     cfg = _abi.ServerConfig(1 << 30, 3, 32, _abi.PREC_FP64)
