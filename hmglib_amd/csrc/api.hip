@@ -927,7 +927,7 @@ hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value)
         return HMG_OK;
     case HMG_OPT_ACA_HUGE_BLOCK:
         if (value < 0 || value > (int64_t)1 << 30) {
-            set_error("ACA 16-wavefront threshold must be in [0, 2^30]");
+            set_error("ACA 8-wavefront threshold must be in [0, 2^30]");
             return HMG_ERR_INVALID;
         }
         e->aca_huge_entries = (int)value;

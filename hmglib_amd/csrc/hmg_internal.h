@@ -371,7 +371,7 @@ struct Engine {
     int aca_tol_mode = 0;
     double aca_tol_c = 0.0;
     int aca_big_entries = 4096;
-    int aca_huge_entries = 16384;               // above: a 16-wavefront workgroup, launched beside
+    int aca_huge_entries = 16384;               // above: an 8-wavefront workgroup, launched beside
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     // ---- mixed precision ------------------------------------------------------
