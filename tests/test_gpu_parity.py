@@ -481,8 +481,13 @@ def test_aca_large_blocks_workgroup_path(gpu, kind, ls):
     hm.precompute_aca()
     ym = hm.apply(x)
     assert _rel(ym, ym_wave) < 1e-6
-    # mixed precision: FP32 factors change late pivots, so the product is held to
-    # the FP64 H-matrix's own approximation error against the dense product
+    hm.set_option(gpu.OPT_ACA_HUGE_BLOCK, 4096)   # the mixed build's batches, 8-wavefront teams
+    hm.precompute_aca()
+    assert _rel(hm.apply(x), ym_wave) < 1e-6
+    hm.set_option(gpu.OPT_ACA_HUGE_BLOCK, 16384)
+    # mixed precision: the FP64 ACA's factors rounded to BF16 hi + lo panels; the
+    # product is held to the FP64 H-matrix's own approximation error against the
+    # dense product plus the BF16x3 arithmetic bound
     y_dense = hm.apply_full(x)
     assert _rel(ym, y_dense) < _rel(y, y_dense) + 4e-5
 
