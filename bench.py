@@ -499,6 +499,10 @@ def main():
         Xh = np.asfortranarray(rng_normal(cfg["seed_X"], n * R).reshape(n, R, order="F"))
         Xd = hip.DeviceArray(Xh)
         Yd = hip.DeviceArray.empty((n, R))
+        # launch shape of the product (near-field LDS reservation, one or two
+        # streams) picked by the library's autotuner on these operands
+        tuned = hm.autotune(Xd, Yd, R, reps=2)
+        log(f"[bench] autotuned product: {tuned}")
         dt = timed_products(hm, Xd, Yd, R, args.steps, args.warmup, hip)
         ms = dt / args.steps * 1e3
         value = R * args.steps / dt
@@ -511,7 +515,7 @@ def main():
         roof, kernels = kernel_roofline(hm, prof, R, True, "hmvp_mixed_n2p20_hbm_traffic.json")
         errs = rel_errors(hm, np.asfortranarray(Yd.download()), Xh, rows, R)
         log(f"[bench] mixed rel err per RHS: max {max(errs):.3e} median {float(np.median(errs)):.3e}")
-        result = {"value": value, "ms": ms, "aca_s": t_aca, "errs": errs}
+        result = {"value": value, "ms": ms, "aca_s": t_aca, "errs": errs, "autotuned": tuned}
         Xd.free()
         Yd.free()
     log(f"[bench] kernels {json.dumps(kernels)}")
@@ -561,6 +565,7 @@ def main():
                        "rel_err_fp64_rows_per_rhs_max": max(result["errs"]),
                        "rel_err_fp64_rows_per_rhs": result["errs"]})
         config["build_s"]["aca"] = result["aca_s"]
+        config["autotuned"] = result["autotuned"]
         if plain is not None:
             config["plain_fp64"] = {"H-MVP/s": plain["value"], "ms_per_step": plain["ms_per_step"],
                                     "steps": plain["steps"], "nrhs": 1,

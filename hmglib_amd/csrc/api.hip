@@ -3,6 +3,7 @@
 // Host orchestration only; every computation runs in the HIP kernels of
 // prims/morton/tree/aca/mvp.hip on the engine's stream.  All calls are
 // synchronous, as in the reference.
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdlib>
@@ -623,6 +624,60 @@ hmg_status hmg_apply_h_matrix_mvp_multi(const double* X, double* Y, int nrhs, st
     tU.release();
     tV.release();
     if (!pre_aca) e->aca_k = -1;
+    return HMG_OK;
+}
+
+// Launch shapes of the product timed on the caller's operands (Y is overwritten);
+// the fastest is kept in the engine's options.  Candidates: the near field's LDS
+// reservation (0, 32, 48, 64, 80 KB: workgroups per CU beside the low-rank
+// kernels) with the two streams overlapped, and both parts on one stream.  The
+// product's values do not depend on the choice (every partial is summed in a
+// fixed order).
+hmg_status hmg_autotune_product(struct h_matrix_data* data, const double* X, double* Y, int nrhs, int reps,
+                                int64_t out[3])
+{
+    Engine* e;
+    HMG_TRY(check_ready(data, &e));
+    if (!X || !Y || nrhs < 1 || reps < 1 || !out) {
+        set_error("invalid x / y / nrhs / reps / out");
+        return HMG_ERR_INVALID;
+    }
+    if (data->U == nullptr || e->aca_k < 0) {
+        set_error("autotuning the product needs precompute_aca first");
+        return HMG_ERR_NOT_READY;
+    }
+    struct Cand {
+        int lds;
+        bool serial;
+    };
+    const Cand cands[] = {{0, false}, {32 << 10, false}, {48 << 10, false}, {64 << 10, false}, {80 << 10, false},
+                          {0, true}};
+    const Cand keep{e->near_lds_pad, e->serialize};
+    Cand best_c = keep;
+    double best = 1e300;
+    for (const Cand& c : cands) {
+        e->near_lds_pad = c.lds;
+        e->serialize = c.serial;
+        hmg_status st = hmg_apply_h_matrix_mvp_multi(X, Y, nrhs, data);  // warm-up
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; st == HMG_OK && r < reps; r++) st = hmg_apply_h_matrix_mvp_multi(X, Y, nrhs, data);
+        if (st != HMG_OK) {
+            e->near_lds_pad = keep.lds;
+            e->serialize = keep.serial;
+            return st;
+        }
+        const double us =
+            std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / reps;
+        if (us < best) {
+            best = us;
+            best_c = c;
+        }
+    }
+    e->near_lds_pad = best_c.lds;
+    e->serialize = best_c.serial;
+    out[0] = best_c.lds;
+    out[1] = best_c.serial ? 1 : 0;
+    out[2] = (int64_t)best;
     return HMG_OK;
 }
 

@@ -128,6 +128,14 @@ class HMatrix:
         _abi.check(self.L.hmg_apply_h_matrix_mvp_multi(xd.ptr, yd.ptr, nrhs, ctypes.byref(self.data)),
                    "apply_h_matrix_mvp")
 
+    def autotune(self, xd, yd, nrhs=1, reps=2):
+        """hmg_autotune_product on device arrays: keeps the fastest launch shape of
+        the product and returns it (yd is overwritten)."""
+        out = (ctypes.c_int64 * 3)()
+        _abi.check(self.L.hmg_autotune_product(ctypes.byref(self.data), xd.ptr, yd.ptr, nrhs, reps, out),
+                   "hmg_autotune_product")
+        return {"near_lds_bytes": int(out[0]), "one_stream": bool(out[1]), "us_per_product": int(out[2])}
+
     # ---- products on host arrays ---------------------------------------------
     def apply(self, x):
         x = np.asarray(x, dtype=np.float64)

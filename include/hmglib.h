@@ -127,6 +127,16 @@ hmg_status hmg_set_gaussian_kernel_rhs(double* b, struct h_matrix_data* data);
  * row of Y is summed in a fixed order, so a result is bitwise the same whatever
  * else runs on the device. */
 hmg_status hmg_apply_h_matrix_mvp_multi(const double* X, double* Y, int nrhs, struct h_matrix_data* data);
+/* Autotuner of the product's launch shape (no counterpart in the reference): times
+ * `reps` products (after one warm-up) for each candidate -- the near field's LDS
+ * reservation of 0, 32, 48, 64 or 80 KB per workgroup with the near field and the
+ * low-rank kernels on two streams, or everything on one stream -- on the caller's
+ * device arrays X, Y (Y is overwritten; results do not depend on the choice) and
+ * keeps the fastest (HMG_OPT_NEAR_LDS_BYTES, HMG_OPT_SERIALIZE).  out[0] = LDS
+ * bytes, out[1] = 1 for one stream, out[2] = microseconds per product.  Needs
+ * precompute_aca. */
+hmg_status hmg_autotune_product(struct h_matrix_data* data, const double* X, double* Y, int nrhs, int reps,
+                                int64_t out[3]);
 /* dense product for nrhs right-hand sides on sampled ORIGINAL rows (checker helper
  * at scale): Yrows[q + c*nrows] = sum_j K(rows[q], j) X[j + c*n2]. rows is a device
  * int64 array. */

@@ -200,6 +200,29 @@ def test_builds_return_their_device_memory(gpu):
     assert f1 >= f0 - (512 << 20), f"{(f0 - f1) / 2**30:.2f} GiB not returned"
 
 
+@pytest.mark.parametrize("prec_name", ["mixed", "fp64"])
+def test_autotune_keeps_the_product(gpu, prec_name):
+    """The autotuner times the product's launch shapes on the caller's arrays and
+    keeps the fastest; the product's values do not depend on the choice."""
+    from hmglib_amd import hip
+    n = 20000
+    X = _points(9201, 3, n)
+    prec = gpu.PREC_MIXED if prec_name == "mixed" else gpu.PREC_FP64
+    hm, kern = _build(gpu, X, X, 20, 1.0, 256, 16, O.K_EXPONENTIAL, 0.1, prec=prec)
+    R = 32 if prec_name == "mixed" else 2
+    xd = hip.DeviceArray(np.asfortranarray(O.normal(9203, n * R).reshape(n, R, order="F")))
+    yd = hip.DeviceArray.empty((n, R))
+    hm.apply_dev(xd, yd, R)
+    y0 = yd.download()
+    t = hm.autotune(xd, yd, R, reps=1)
+    assert t["near_lds_bytes"] in (0, 32768, 49152, 65536, 81920) and t["us_per_product"] > 0
+    assert not (t["one_stream"] and t["near_lds_bytes"])
+    hm.apply_dev(xd, yd, R)
+    np.testing.assert_array_equal(yd.download(), y0)
+    xd.free()
+    yd.free()
+
+
 def test_mixed_not_precomputed_matches_precomputed(gpu):
     n = 5000
     X = _points(4001, 3, n)
