@@ -230,6 +230,7 @@ void release_build(Engine& e, h_matrix_data* d)
     e.inv1.release();
     e.inv1_ready = false;
     e.Xp.release();
+    e.Xt32.release();
     e.cpan[0].release();
     e.cpan[1].release();
     e.cpan_ready = false;
@@ -861,7 +862,7 @@ int64_t engine_bytes(const Engine& e)
     t += bytes_of(e.fly_items) + bytes_of(e.fly_refs) + bytes_of(e.lr_items) + bytes_of(e.lv_items);
     t += bytes_of(e.lv_slices) + bytes_of(e.tmap) + bytes_of(e.Tp) + bytes_of(e.P) + bytes_of(e.xs);
     t += bytes_of(e.T) + bytes_of(e.Up) + bytes_of(e.Vp) + bytes_of(e.mx_items) + bytes_of(e.mx_refs);
-    t += bytes_of(e.Tpan) + bytes_of(e.Tp32) + bytes_of(e.P32) + bytes_of(e.Xp) + bytes_of(e.mxp);
+    t += bytes_of(e.Tpan) + bytes_of(e.Tp32) + bytes_of(e.P32) + bytes_of(e.Xp) + bytes_of(e.Xt32) + bytes_of(e.mxp);
     t += bytes_of(e.Ys32) + bytes_of(e.inv1);
     t += bytes_of(e.cpan[0]) + bytes_of(e.cpan[1]);
     for (const Combine* c : {&e.comb_stored, &e.comb_fly, &e.comb_mx})
@@ -925,7 +926,7 @@ hmg_status hmg_memory(const struct h_matrix_data* data, int64_t out[2])
             if (nch > 1) ntp += nch * ((K + 127) / 128);
         }
         const int64_t slots = (int64_t)e->n_lr_items * 64 + mx_items * 128;
-        add += g2 * 2 * kMxRhs * 8 * 2;                                         // Xp
+        add += g2 * 2 * kMxRhs * 8 * 2 + (int64_t)e->n[1] * kMxRhs * 4;         // Xp, Xt32
         add += (e->tsize + 32 * (int64_t)ncg) * kMxRhs * 4 + ntp * kMxVCols * kMxRhs * 4;  // Tpan, Tp32
         add += slots * kMxRhs * 4 + slots * 4 + n1 * 4;              // P32, combine list
         add += n1 * kMxRhs * 4 + n1 * 4;                             // Ys32, inv1

@@ -401,6 +401,7 @@ struct Engine {
     DevBuf<int> inv1;                           // sorted position of each original row
     bool inv1_ready = false;
     DevBuf<__bf16> Xp;
+    DevBuf<float> Xt32;                         // a pass's right-hand sides as 32-wide FP32 rows
     DevBuf<float> cpan[2];
     bool cpan_ready = false;
 };

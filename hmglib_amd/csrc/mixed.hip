@@ -75,17 +75,43 @@ __device__ __forceinline__ float kval_f32(const KernelParamsF& kp, float r2)
 }
 
 // ---- per-pass inputs --------------------------------------------------------------
-// Xp panels: group g = p/8 holds [hi | lo][32 right-hand sides][8 points] BF16 of
-// X[order2[p] + (c0+c)*n2] rounded to FP32 (0 past n2 or nr)
+// Xt[i*32 + c] = X[i + (c0+c)*n2] rounded to FP32 (0 past nr): the caller's columns
+// turned into 128-byte rows, 64 points per workgroup through LDS (reads and writes
+// coalesced), so that the permuted gather below reads whole rows
+constexpr int kRowTile = 64;
 __global__ void __launch_bounds__(kThreads)
-prep_x_kernel(const double* __restrict__ X, const uint64_t* __restrict__ order2, int n2, int n2p, int c0,
-              int nr, __bf16* __restrict__ Xp)
+x_rows_kernel(const double* __restrict__ X, int n2, int c0, int nr, float* __restrict__ Xt)
+{
+    __shared__ float tile[kMxRhs][kRowTile + 1];
+    const int i0 = blockIdx.x * kRowTile;
+#pragma unroll
+    for (int k = 0; k < kRowTile * kMxRhs / kThreads; k++) {
+        const int e = threadIdx.x + kThreads * k;
+        const int c = e / kRowTile, il = e % kRowTile;
+        const int i = i0 + il;
+        tile[c][il] = (i < n2 && c < nr) ? (float)X[i + (int64_t)(c0 + c) * n2] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kRowTile * kMxRhs / kThreads; k++) {
+        const int e = threadIdx.x + kThreads * k;
+        const int il = e / kMxRhs, c = e % kMxRhs;
+        const int i = i0 + il;
+        if (i < n2) Xt[(int64_t)i * kMxRhs + c] = tile[c][il];
+    }
+}
+
+// Xp panels: group g = p/8 holds [hi | lo][32 right-hand sides][8 points] BF16 of
+// Xt[order2[p]*32 + c] (0 past n2)
+__global__ void __launch_bounds__(kThreads)
+prep_x_kernel(const float* __restrict__ Xt, const uint64_t* __restrict__ order2, int n2, int n2p,
+              __bf16* __restrict__ Xp)
 {
     const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (t >= (int64_t)n2p * kMxRhs) return;
     const int p = (int)(t / kMxRhs), c = (int)(t % kMxRhs);
     float v = 0.f;
-    if (p < n2 && c < nr) v = (float)X[order2[p] + (int64_t)(c0 + c) * n2];
+    if (p < n2) v = Xt[(int64_t)order2[p] * kMxRhs + c];
     const __bf16 h = (__bf16)v;
     const __bf16 l = (__bf16)(v - (float)h);
     const int64_t gbase = (int64_t)(p / kGroup) * 2 * kMxRhs * kGroup + c * kGroup + (p % kGroup);
@@ -385,8 +411,18 @@ combine_mx_kernel(const int* __restrict__ ptr, const int* __restrict__ idx, cons
     const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     const int i = (int)(t / kMxRhs), c = (int)(t % kMxRhs);
     if (i >= n1) return;
+    // four partials loaded at a time, added in list order
     float s = 0.f;
-    for (int q = ptr[i]; q < ptr[i + 1]; q++) s += P[(int64_t)idx[q] * kMxRhs + c];
+    int q = ptr[i];
+    const int qe = ptr[i + 1];
+    for (; q + 4 <= qe; q += 4) {
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = P[(int64_t)idx[q + u] * kMxRhs + c];
+#pragma unroll
+        for (int u = 0; u < 4; u++) s += v[u];
+    }
+    for (; q < qe; q++) s += P[(int64_t)idx[q] * kMxRhs + c];
     Ys[t] = s;
 }
 
@@ -482,6 +518,7 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
     const int64_t near0 = (int64_t)eng.n_lr_items * kMxLrRows;
     const int64_t slots = near0 + (int64_t)eng.n_mx_items * kMxRowsItem;
     HMG_TRY(eng.Xp.alloc((size_t)g2 * 2 * kMxRhs * kGroup));
+    HMG_TRY(eng.Xt32.alloc((size_t)n2 * kMxRhs));
     // T panels: slots no V column maps to (each row cluster's padding) stay zero
     if (!eng.tpan_ready) {
         const size_t n = (size_t)(eng.tsize / kGroup + 2) * 2 * kMxRhs * kGroup;
@@ -505,8 +542,10 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
     for (int c0 = 0; c0 < nrhs; c0 += kMxRhs) {
         const int nr = std::min(kMxRhs, nrhs - c0);
         eng.prof.begin(PROF_GATHER, s);
+        x_rows_kernel<<<grid_for(n2, kRowTile), kThreads, 0, s>>>(X, n2, c0, nr, eng.Xt32.p);
+        HMG_HIP(hipGetLastError());
         prep_x_kernel<<<grid_for((int64_t)n2p * kMxRhs, kThreads), kThreads, 0, s>>>(
-            X, order2, n2, n2p, c0, nr, eng.Xp.p);
+            eng.Xt32.p, order2, n2, n2p, eng.Xp.p);
         HMG_HIP(hipGetLastError());
         eng.prof.end(PROF_GATHER, s);
         HMG_HIP(hipEventRecord(eng.ev_fork, s));
