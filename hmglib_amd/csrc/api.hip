@@ -416,8 +416,40 @@ hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
     const bool spread_defined = (dim == 3 && data->bits == 20) || (dim == 2 && data->bits == 32) ||
                                 (dim == 10 && data->bits == 4);
     const int key_bits = spread_defined ? std::min(64, dim * data->bits) : 8;
+    // Both sets bitwise equal as given (one point cloud against itself): every
+    // step below is deterministic, so the second set's extrema, codes, order and
+    // sorted coordinates are copies of the first set's.
+    bool same_input = false;
+    if (data->point_count[0] == data->point_count[1] && data->point_count[0] > 0) {
+        DevBuf<int> flag;
+        HMG_TRY(flag.alloc(1));
+        HMG_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+        for (int d = 0; d < dim; d++)
+            HMG_TRY(flag_differ_f64(data->coords_d[0][d], data->coords_d[1][d], data->point_count[0],
+                                    flag.p, s));
+        int differ = 1;
+        HMG_HIP(hipMemcpyAsync(&differ, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HMG_HIP(hipStreamSynchronize(s));
+        same_input = differ == 0;
+    }
     for (int i = 0; i < 2; i++) {
         const int n = data->point_count[i];
+        if (i == 1 && same_input) {
+            HMG_HIP(hipMemcpyAsync(data->min_per_dim_d[1], data->min_per_dim_d[0], sizeof(double) * dim,
+                                   hipMemcpyDeviceToDevice, s));
+            HMG_HIP(hipMemcpyAsync(data->max_per_dim_d[1], data->max_per_dim_d[0], sizeof(double) * dim,
+                                   hipMemcpyDeviceToDevice, s));
+            HMG_HIP(hipMemcpyAsync(data->code_d[1], data->code_d[0], sizeof(uint64_t) * n,
+                                   hipMemcpyDeviceToDevice, s));
+            HMG_HIP(hipMemcpyAsync(data->order[1], data->order[0], sizeof(uint64_t) * n,
+                                   hipMemcpyDeviceToDevice, s));
+            HMG_HIP(hipMemcpyAsync(data->point_ids_d[1], data->point_ids_d[0], sizeof(unsigned) * n,
+                                   hipMemcpyDeviceToDevice, s));
+            for (int d = 0; d < dim; d++)
+                HMG_HIP(hipMemcpyAsync(data->coords_d[1][d], data->coords_d[0][d], sizeof(double) * n,
+                                       hipMemcpyDeviceToDevice, s));
+            break;
+        }
         // extremal values (helper.cu:108-138)
         for (int d = 0; d < dim; d++)
             HMG_TRY(minmax_f64(data->coords_d[i][d], n, data->min_per_dim_d[i] + d,
@@ -455,8 +487,8 @@ hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
     }
     // the same point set on both sides (sorted coordinates bitwise equal) makes
     // the kernel matrix bitwise symmetric outside the diagonal
-    e->same_sets = false;
-    if (data->point_count[0] == data->point_count[1] && data->point_count[0] > 0) {
+    e->same_sets = same_input;
+    if (!same_input && data->point_count[0] == data->point_count[1] && data->point_count[0] > 0) {
         DevBuf<int> flag;
         HMG_TRY(flag.alloc(1));
         HMG_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));

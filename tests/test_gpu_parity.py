@@ -53,11 +53,22 @@ def _assert_blocks_equal(g, o, dim):
                                       err_msg=f)
 
 
-@pytest.mark.parametrize("dim,bits,n1,n2,seed", [(2, 32, 4096, 4096, 1001), (3, 20, 5000, 3000, 2001),
-                                                 (2, 32, 1, 7, 7)][:2])
-def test_morton_codes_order_bit_exact(gpu, dim, bits, n1, n2, seed):
+@pytest.mark.parametrize("dim,bits,n1,n2,seed,second", [(2, 32, 4096, 4096, 1001, "indep"),
+                                                        (3, 20, 5000, 3000, 2001, "indep"),
+                                                        (2, 32, 4096, 4096, 1003, "same"),
+                                                        (3, 20, 3000, 3000, 2003, "same"),
+                                                        (3, 20, 3000, 3000, 2005, "one_differs")])
+def test_morton_codes_order_bit_exact(gpu, dim, bits, n1, n2, seed, second):
+    """Extrema, codes, order and sorted coordinates of both sets, bit-exact.  With
+    both sets bitwise equal the second set's results are copies of the first's;
+    one differing coordinate must send the second set through its own sort."""
     X1 = _points(seed, dim, n1)
-    X2 = _points(seed + 1, dim, n2)
+    if second == "indep":
+        X2 = _points(seed + 1, dim, n2)
+    else:
+        X2 = X1.copy()
+        if second == "one_differs":
+            X2[1, n2 // 2] = np.nextafter(X2[1, n2 // 2], 2.0)
     hm, _, om, _ = _pair(gpu, X1, X2, bits, 1.5, 64, 16, O.K_GAUSSIAN, 0.1, precompute=False)
     for s in (0, 1):
         mn_g, mx_g = hm.minmax(s)
