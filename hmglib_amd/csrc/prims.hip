@@ -122,8 +122,6 @@ scan_add_kernel(int* __restrict__ out, int64_t n, const int* __restrict__ block_
     }
 }
 
-__global__ void copy_total_kernel(const int* block_sums, int* total) { *total = block_sums[0]; }
-
 hmg_status scan_rec(const int* in, int* out, int64_t n, bool inclusive, int* total,
                     hipStream_t s, int depth)
 {
@@ -133,17 +131,21 @@ hmg_status scan_rec(const int* in, int* out, int64_t n, bool inclusive, int* tot
     }
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
     void* p;
+    if (nb == 1) {
+        // one tile: its total is the scan's total, written where the caller wants it
+        int* sums = total;
+        if (!sums) {
+            HMG_TRY(scratch(s, 100 + depth, sizeof(int), &p));
+            sums = (int*)p;
+        }
+        scan_tile_kernel<<<1, kScanThreads, 0, s>>>(in, out, n, sums, inclusive ? 1 : 0);
+        HMG_HIP(hipGetLastError());
+        return HMG_OK;
+    }
     HMG_TRY(scratch(s, 100 + depth, sizeof(int) * (size_t)nb, &p));
     int* bsums = (int*)p;
     scan_tile_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, out, n, bsums, inclusive ? 1 : 0);
     HMG_HIP(hipGetLastError());
-    if (nb == 1) {
-        if (total) {
-            copy_total_kernel<<<1, 1, 0, s>>>(bsums, total);
-            HMG_HIP(hipGetLastError());
-        }
-        return HMG_OK;
-    }
     HMG_TRY(scan_rec(bsums, bsums, nb, false, total, s, depth + 1));
     scan_add_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(out, n, bsums);
     HMG_HIP(hipGetLastError());
