@@ -165,6 +165,10 @@ SIGNATURES = {
     "hmg_get_profile": (ctypes.c_int, [_D, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64)]),
     "hmg_copy_probe": (ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
     "hmg_memory": (ctypes.c_int, [_D, _P(ctypes.c_int64)]),
+    "hmg_setup_batch": (ctypes.c_int, [_P(_D), ctypes.c_int, ctypes.c_int]),
+    "hmg_precompute_aca_batch": (ctypes.c_int, [_P(_D), ctypes.c_int, ctypes.c_int]),
+    "hmg_apply_batch": (ctypes.c_int, [_P(_D), _P(_vp), _P(_vp), _P(ctypes.c_int), ctypes.c_int,
+                                       ctypes.c_int]),
     "hmg_save": (ctypes.c_int, [_D, ctypes.c_char_p]),
     "hmg_load": (ctypes.c_int, [_D, ctypes.c_char_p]),
     # serving (include/hmglib_server.h)

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhmglib.so")
 SOURCES = ["prims.hip", "morton.hip", "tree.hip", "aca.hip", "schedule.hip", "mvp.hip", "mixed.hip",
-           "api.hip", "server.hip", "store.hip"]
+           "api.hip", "server.hip", "store.hip", "batch.hip"]
 ARCH = os.environ.get("HMG_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -801,6 +801,16 @@ int64_t aca_mixed_scratch(const Engine& eng)
     return std::min(total, cap);
 }
 
+hmg_status aca_to_panels(Engine& eng, const int* list, int n, const int64_t* suoff, const int64_t* svoff,
+                         const double* S, __bf16* Up, __bf16* Vp, hipStream_t s)
+{
+    if (n <= 0) return HMG_OK;
+    mx_panel_kernel<<<n, kConvThreads, 0, s>>>(list, eng.blk.p + eng.ndense, eng.kb.p, eng.keff.p, suoff, svoff,
+                                              eng.mxp.p, S, Up, Vp);
+    HMG_HIP(hipGetLastError());
+    return HMG_OK;
+}
+
 hmg_status aca_build_mixed(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp, int tie_last,
                            __bf16* Up, __bf16* Vp)
 {

@@ -63,8 +63,6 @@ KernelParams make_kernel_params(const system_assembler& a, int dim)
     return kp;
 }
 
-namespace {
-
 Coords coords_of(const h_matrix_data* d, int set)
 {
     Coords c{};
@@ -113,6 +111,8 @@ hmg_status check_ready(const h_matrix_data* d, Engine** out)
     return HMG_OK;
 }
 
+namespace {
+
 // compute_current_dense_work_size_magma (linear_algebra.cu:4059-4108), int64 inside
 int dense_work_size_at(const std::vector<int4>& bl, int ndense, int64_t max_size, int start)
 {
@@ -141,6 +141,8 @@ int aca_work_size_at(const std::vector<int4>& bl, int nblocks, int64_t max_size,
     return cnt;
 }
 
+}  // namespace
+
 void release_factors(Engine& e, h_matrix_data* d)
 {
     e.U.release();
@@ -153,6 +155,8 @@ void release_factors(Engine& e, h_matrix_data* d)
     d->U = nullptr;
     d->V = nullptr;
 }
+
+namespace {
 
 void release_build(Engine& e, h_matrix_data* d)
 {
@@ -246,12 +250,17 @@ void release_build(Engine& e, h_matrix_data* d)
     d->aca_work_size = nullptr;
 }
 
-// FP64 ACA factors into (U, V, keff) for the current block list and rank k
+}  // namespace
+
 // the tolerance stop's constant from the caller's epsilon and eta
 void set_tolerance(Engine& e, const h_matrix_data* d)
 {
     e.aca_tol_c = e.aca_tol_mode == 1 ? d->epsilon * (1.0 - d->eta) / (1.0 + d->epsilon) : d->epsilon;
 }
+
+namespace {
+
+// FP64 ACA factors into (U, V, keff) for the current block list and rank k
 
 hmg_status build_factors(Engine& e, h_matrix_data* d, const KernelParams& kp, int k, int tie_last,
                          DevBuf<double>& U, DevBuf<double>& V)

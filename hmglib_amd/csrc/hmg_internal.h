@@ -237,6 +237,10 @@ constexpr int kMxRhs = 32;
 // U T work item (up to four interleaved 32-wide MFMA tiles).
 constexpr int kMxVCols = 128;
 constexpr int kMxLrRows = 128;
+// Rows of one mixed-precision near-field work item (four 32-row MFMA tiles); the
+// schedule spaces the items' partial-sum slots by it and the kernel writes them
+// at that spacing.
+constexpr int kMxNearRows = 128;
 // Mixed-precision factor panels (BF16 hi + lo, zero padded): for a row cluster C
 // with m1p = rows padded to 32 and its rank columns padded to 32, column group kg
 // (8 columns) is [hi: m1p x 8][lo: m1p x 8], U_C[i][8kg+e] at
@@ -445,13 +449,17 @@ struct TreeParams {
 hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uint64_t* codes1,
                       const uint64_t* codes2, const TreeParams& tp);
 
-// host vector -> device buffer (grown as needed), on stream s
+// host vector -> device buffer (grown as needed), on stream s.  The copy has
+// completed when this returns, so the caller may free or reuse the host vector
+// at once (the vectors are pageable host memory).
 template <class T>
 hmg_status upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s)
 {
     HMG_TRY(d.alloc(std::max<size_t>(h.size(), 1)));
-    if (!h.empty())
+    if (!h.empty()) {
         HMG_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+        HMG_HIP(hipStreamSynchronize(s));
+    }
     return HMG_OK;
 }
 
@@ -462,6 +470,18 @@ hmg_status engine_init(h_matrix_data* data, int point_count[2], int dim, int bit
 // moves a caller-owned-stream engine to another stream of the caller (between
 // jobs, with no work of this H-matrix in flight)
 void engine_bind_stream(Engine* e, hipStream_t s, h_matrix_data* data);
+
+// ---- helpers of the C ABI (api.hip) ---------------------------------------------------
+// set s of an H-matrix as the kernels see it (its sorted coordinates)
+Coords coords_of(const h_matrix_data* d, int set);
+// the kernel object behind d->assem, validated
+hmg_status read_kernel(const h_matrix_data* d, KernelParams* kp);
+// the engine of an H-matrix that has been set up
+hmg_status check_ready(const h_matrix_data* d, Engine** out);
+// drops stored factors (FP64 and mixed precision)
+void release_factors(Engine& e, h_matrix_data* d);
+// the tolerance stop's constant of the next ACA from d->epsilon and d->eta
+void set_tolerance(Engine& e, const h_matrix_data* d);
 
 // ---- ACA (aca.hip) -------------------------------------------------------------
 // tie_last: 1 = last argmax (batched path), 0 = first argmax (unbatched path).
@@ -475,6 +495,11 @@ hmg_status aca_build_mixed(Engine& eng, const Coords& c1, const Coords& c2, cons
                            int tie_last, __bf16* Up, __bf16* Vp);
 // FP64 scratch elements aca_build_mixed uses for the current layout
 int64_t aca_mixed_scratch(const Engine& eng);
+// BF16 hi + lo panels (Up / Vp) of ACA blocks list[0 .. n) of eng from FP64
+// factors in a scratch: U_b column-major (ld m1) at S + suoff[a], V_b row-major (ld
+// kb) at S + svoff[a]; ranks at or past keff[a] become zero columns.  On stream s.
+hmg_status aca_to_panels(Engine& eng, const int* list, int n, const int64_t* suoff, const int64_t* svoff,
+                         const double* S, __bf16* Up, __bf16* Vp, hipStream_t s);
 
 // ---- schedule (schedule.hip, host) ---------------------------------------------------
 // row-cluster groups and near-field work items (once per tree); the matrix-free

@@ -32,7 +32,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;
-constexpr int kMxRowsItem = 128; // rows per near-field item (four 32-row tiles)
+constexpr int kMxRowsItem = kMxNearRows; // rows per near-field item (four 32-row tiles)
 constexpr int kMxRowTiles = kMxRowsItem / 32;
 constexpr int kGroup = 8;        // points per panel group
 inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }

@@ -65,8 +65,10 @@ __device__ __forceinline__ int wave_incl_scan(int v)
 }
 
 // Scans one tile of 2048 ints; writes the tile's total to block_sums[blockIdx.x].
+// Callers scan in place (in == out): a workgroup reads its whole tile into LDS
+// before it writes any of it, and no other workgroup touches that tile.
 __global__ void __launch_bounds__(kScanThreads)
-scan_tile_kernel(const int* __restrict__ in, int* __restrict__ out, int64_t n, int* block_sums,
+scan_tile_kernel(const int* in, int* out, int64_t n, int* block_sums,
                  int inclusive)
 {
     __shared__ int tile[kScanTile];

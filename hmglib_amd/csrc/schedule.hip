@@ -31,7 +31,7 @@ constexpr int kNearSlice = 2 * kTileRows;  // rows per near-field work item (two
 constexpr int kNearChunkCols = 4096;       // near-field columns per work item
 constexpr int kVChunkRows = 1024;          // points per V^T x work item (FP64)
 constexpr int kMxVChunkRows = 4096;        // points per V^T x work item (mixed precision)
-constexpr int kMxRows = 128;               // rows per mixed-precision near-field item
+constexpr int kMxRows = kMxNearRows;       // rows per mixed-precision near-field item
 constexpr int kMxChunkCols = 4096;         // columns per mixed-precision near-field item
 
 inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }

@@ -231,7 +231,9 @@ hmg_status run_build(hmg_server* s, Matrix* m, Request* r, hipStream_t slot, std
                 }
             }
         }
-        if (st == HMG_OK && hipStreamSynchronize(e->stream) != hipSuccess) st = HMG_ERR_HIP;
+        // every path waits for the uploads issued so far before their host
+        // source goes away (an error above may leave earlier copies in flight)
+        if (hipStreamSynchronize(e->stream) != hipSuccess) st = HMG_ERR_HIP;
         r->coords[0].clear();
         r->coords[0].shrink_to_fit();
         r->coords[1].clear();

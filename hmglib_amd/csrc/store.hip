@@ -17,8 +17,10 @@
 //   effective ranks: naca int32
 //   U, then V: u_elems and v_elems elements of elem_bytes each (8: FP64 layout,
 //   2: BF16 hi + lo panels)
-// The header's checksum covers the payload (64-bit multiply-xor hash over 8-byte
-// words, the tail zero-padded).
+// The header's checksum covers the header itself (with its checksum field zero)
+// and the payload (64-bit multiply-xor hash over 8-byte words, the tail
+// zero-padded), so a damaged parameter or kernel field is refused like a damaged
+// factor.
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -30,7 +32,7 @@ namespace hmg {
 namespace {
 
 constexpr char kMagic[8] = {'H', 'M', 'G', 'A', 'M', 'D', 'H', 'M'};
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;
 
 struct StoreHeader {
     char magic[8];
@@ -232,7 +234,8 @@ hmg_status hmg_save(const struct h_matrix_data* data, const char* path)
     std::vector<double> p0, p1;
     HMG_TRY(original_points(data, 0, s, p0));
     HMG_TRY(original_points(data, 1, s, p1));
-    const bool same = e->n[0] == e->n[1] && p0 == p1;
+    // bitwise, as setup decides it (value equality would take -0.0 for +0.0)
+    const bool same = e->n[0] == e->n[1] && memcmp(p0.data(), p1.data(), sizeof(double) * p0.size()) == 0;
 
     StoreHeader h{};
     memcpy(h.magic, kMagic, 8);
@@ -274,6 +277,7 @@ hmg_status hmg_save(const struct h_matrix_data* data, const char* path)
         return HMG_ERR_INVALID;
     }
     Hash hash;
+    hash.add(&h, sizeof(h));  // checksum field still zero
     HMG_TRY(write_host(f.f, p0.data(), sizeof(double) * p0.size(), hash));
     if (!same) HMG_TRY(write_host(f.f, p1.data(), sizeof(double) * p1.size(), hash));
     std::vector<int> keff(std::max(na, 1), 0);
@@ -311,6 +315,11 @@ hmg_status hmg_load(struct h_matrix_data* data, const char* path)
     HMG_TRY(read_header(f.f, &h));
     // the host part of the payload first: nothing touches the GPU before it checks out
     Hash hash;
+    {
+        StoreHeader h0 = h;
+        h0.checksum = 0;
+        hash.add(&h0, sizeof(h0));
+    }
     std::vector<double> p0((size_t)h.dim * h.n0), p1;
     HMG_TRY(read_host(f.f, p0.data(), sizeof(double) * p0.size(), hash));
     if (!h.same_points) {

@@ -449,10 +449,10 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
     eng.ndense = ndense;
     eng.naca = nout - ndense;
     eng.h_blk.resize(nout);
-    if (nout > 0)
-        HMG_HIP(hipMemcpyAsync(eng.h_blk.data(), eng.blk.p, sizeof(int4) * nout, hipMemcpyDeviceToHost,
-                               eng.stream));
-        HMG_HIP(hipStreamSynchronize(eng.stream));
+    if (nout > 0) {
+        HMG_HIP(hipMemcpyAsync(eng.h_blk.data(), eng.blk.p, sizeof(int4) * nout, hipMemcpyDeviceToHost, s));
+        HMG_HIP(hipStreamSynchronize(s));
+    }
     for (DevBuf<int>* b : {&flags, &rank, &ub, &keys, &S, &runid, &Lc, &Uc, &rstart, &rend, &scal,
                            &nch, &nmv, &choff, &mvoff, &adm})
         b->release();

@@ -260,3 +260,33 @@ class HMatrix:
         return {"E_near": out[0], "E_lr": out[1], "dense": out[2], "aca": out[3], "levels": out[4],
                 "quirk_levels": out[5], "lr_m1_rank": out[6], "lr_m2_rank": out[7],
                 "rank_sum": out[8]}
+
+
+def _data_array(hms):
+    arr = (ctypes.POINTER(_abi.HMatrixData) * len(hms))()
+    for i, h in enumerate(hms):
+        arr[i] = ctypes.pointer(h.data)
+    return arr
+
+
+def setup_batch(hms, threads=8):
+    """hmg_setup_batch: setup() of every HMatrix in `hms`, run concurrently."""
+    _abi.check(_abi.lib().hmg_setup_batch(_data_array(hms), len(hms), threads), "hmg_setup_batch")
+
+
+def precompute_aca_batch(hms, threads=8):
+    """hmg_precompute_aca_batch: precompute_aca() of every HMatrix in `hms` (set up,
+    kernel set) in one launch set per kernel / dimension / stopping rule /
+    precision."""
+    _abi.check(_abi.lib().hmg_precompute_aca_batch(_data_array(hms), len(hms), threads),
+               "hmg_precompute_aca_batch")
+
+
+def apply_batch(hms, xds, yds, nrhs, threads=8):
+    """hmg_apply_batch: yds[i] = H_i xds[i] (device arrays, nrhs[i] columns each),
+    the products running concurrently."""
+    n = len(hms)
+    X = (ctypes.c_void_p * n)(*[x.ptr for x in xds])
+    Y = (ctypes.c_void_p * n)(*[y.ptr for y in yds])
+    R = (ctypes.c_int * n)(*[int(r) for r in nrhs])
+    _abi.check(_abi.lib().hmg_apply_batch(_data_array(hms), X, Y, R, n, threads), "hmg_apply_batch")

@@ -127,6 +127,25 @@ hmg_status hmg_set_gaussian_kernel_rhs(double* b, struct h_matrix_data* data);
  * row of Y is summed in a fixed order, so a result is bitwise the same whatever
  * else runs on the device. */
 hmg_status hmg_apply_h_matrix_mvp_multi(const double* X, double* Y, int nrhs, struct h_matrix_data* data);
+/* Many H-matrices at once (no counterpart in the reference; for callers with many
+ * independent point clouds).  Each entry of data[] is an H-matrix of its own
+ * (init_h_matrix_data, coordinates, parameters as usual); threads = host threads
+ * that drive them (0: 8).
+ * hmg_setup_batch: setup_h_matrix of every entry, the entries' setups running
+ *   concurrently on their own streams.
+ * hmg_precompute_aca_batch: precompute_aca of every entry (after setup and with
+ *   data[i]->assem set), in ONE launch set for all entries that share a kernel,
+ *   point dimension, stopping rule and precision: their admissible blocks run side
+ *   by side (largest first, 8-wavefront teams for the largest beside the rest).
+ *   Ranks and factors are those precompute_aca gives each entry alone.
+ * hmg_apply_batch: Y[i] = H_i X[i] for nrhs[i] right-hand sides each (as
+ *   hmg_apply_h_matrix_mvp_multi), the entries' products running concurrently.
+ * An entry may appear once per call.  The first failure (lowest index) is
+ * returned; hmg_last_error names the entry. */
+hmg_status hmg_setup_batch(struct h_matrix_data* const* data, int count, int threads);
+hmg_status hmg_precompute_aca_batch(struct h_matrix_data* const* data, int count, int threads);
+hmg_status hmg_apply_batch(struct h_matrix_data* const* data, const double* const* X, double* const* Y,
+                           const int* nrhs, int count, int threads);
 /* Autotuner of the product's launch shape (no counterpart in the reference): times
  * `reps` products (after one warm-up) for each candidate -- the near field's LDS
  * reservation of 0, 32, 48, 64 or 80 KB per workgroup with the near field and the

@@ -127,6 +127,9 @@ def lib():
                                      P(ctypes.c_double), P(ctypes.c_double)]
         L.orc_gaussian_rhs.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
         L.orc_work.argtypes = [ctypes.c_void_p, c_i64p]
+        L.orc_work_sizes.restype = ctypes.c_int64
+        L.orc_work_sizes.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_int), ctypes.c_int,
+                                     P(ctypes.c_int), P(ctypes.c_int), ctypes.c_int, P(ctypes.c_int)]
         _lib = L
     return _lib
 
@@ -355,6 +358,19 @@ class HMatrix:
         out = np.zeros(2, dtype=np.int64)
         lib().orc_work(self._h, _i64p(out))
         return int(out[0]), int(out[1])
+
+    def work_sizes(self, max_batched_aca_size):
+        """Batch sizing of setup (hmglib.cu:211-219): (dense cap, dense batch sizes,
+        ACA batch sizes); each list ends with the 0 the reference records."""
+        ci = ctypes.POINTER(ctypes.c_int)
+        nd, na = ctypes.c_int(), ctypes.c_int()
+        cap = lib().orc_work_sizes(self._h, int(max_batched_aca_size), None, 0, ctypes.byref(nd), None, 0,
+                                   ctypes.byref(na))
+        dws = np.zeros(max(nd.value, 1), dtype=np.int32)
+        aws = np.zeros(max(na.value, 1), dtype=np.int32)
+        lib().orc_work_sizes(self._h, int(max_batched_aca_size), dws.ctypes.data_as(ci), nd.value,
+                             ctypes.byref(nd), aws.ctypes.data_as(ci), na.value, ctypes.byref(na))
+        return int(cap), dws[:nd.value], aws[:na.value]
 
 
 def num_threads():

@@ -900,6 +900,51 @@ void orc_work(const orc_hm* hm, int64_t* out2)
     out2[1] = el;
 }
 
+/* Batch sizing of the reference's setup (hmglib.cu:211-219): the dense batch cap
+ * is overridden by 1.05 x the dense entries (predict_precomputing_memory_requirements,
+ * linear_algebra.cu:4176-4198), then precompute_work_sizes (:4201-4288) walks the
+ * DENSE blocks with compute_current_dense_work_size_magma (:4059-4108, batch size
+ * = blocks taken while the running sum of m1*m2 stays <= the cap, the block that
+ * crosses it given back) and the ACA blocks with compute_current_aca_work_size
+ * (:4158-4174, the same with the running sum of m1); each walk records batch sizes
+ * until a batch of size 0, which is recorded too.  Sums are 64-bit here (the
+ * reference's int overflows at the scale configuration).  Returns the dense cap;
+ * *nd / *na = batch counts, dense_ws / aca_ws receive up to cap_d / cap_a sizes. */
+static int batch_walk(const orc_hm* hm, int first, int end, int64_t cap, int by_entries, int* ws, int capn)
+{
+    int count = 0, at = first;
+    for (;;) {
+        int taken = 0;
+        int64_t sum = 0;
+        while (sum <= cap && at + taken < end) {
+            const orc_work_item* w = &hm->blocks[at + taken];
+            const int64_t m1 = w->set1_u - w->set1_l + 1, m2 = w->set2_u - w->set2_l + 1;
+            sum += by_entries ? m1 * m2 : m1;
+            taken++;
+        }
+        if (sum > cap) taken--;
+        if (count < capn) ws[count] = taken;
+        count++;
+        if (taken <= 0) break;
+        at += taken;
+    }
+    return count;
+}
+
+int64_t orc_work_sizes(const orc_hm* hm, int max_batched_aca_size, int* dense_ws, int cap_d, int* nd,
+                       int* aca_ws, int cap_a, int* na)
+{
+    int64_t entries = 0;
+    for (int b = 0; b < hm->ndense; b++) {
+        const orc_work_item* w = &hm->blocks[b];
+        entries += (int64_t)(w->set1_u - w->set1_l + 1) * (int64_t)(w->set2_u - w->set2_l + 1);
+    }
+    const int64_t dense_cap = (int64_t)((double)entries * 1.05);
+    *nd = batch_walk(hm, 0, hm->ndense, dense_cap, 1, dense_ws, cap_d);
+    *na = batch_walk(hm, hm->ndense, hm->nblocks, (int64_t)max_batched_aca_size, 0, aca_ws, cap_a);
+    return dense_cap;
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

@@ -150,6 +150,12 @@ void orc_gaussian_rhs(const orc_hm* hm, double* b);
  * E_lr = sum_aca (m1+m2)*keff (if ACA built, else k_b). out[0]=E_near out[1]=E_lr */
 void orc_work(const orc_hm* hm, int64_t* out2);
 
+/* Batch sizing of setup_h_matrix (hmglib.cu:211-219, linear_algebra.cu:4059-4288):
+ * returns the dense batch cap (1.05 x the dense entries); fills the dense and ACA
+ * batch sizes (each walk ends with a recorded 0) and their counts. */
+int64_t orc_work_sizes(const orc_hm* hm, int max_batched_aca_size, int* dense_ws, int cap_d, int* nd,
+                       int* aca_ws, int cap_a, int* na);
+
 #ifdef __cplusplus
 }
 #endif

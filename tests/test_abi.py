@@ -148,7 +148,7 @@ def test_server_host_logic_without_the_gpu():
 def _store_header(**kw):
     """A header of the H-matrix file format (layout of StoreHeader in store.hip)."""
     import struct
-    f = dict(magic=b"HMGAMDHM", version=1, header_bytes=160, dim=2, bits=32, n0=10, n1=10, max_level=50,
+    f = dict(magic=b"HMGAMDHM", version=2, header_bytes=160, dim=2, bits=32, n0=10, n1=10, max_level=50,
              c_leaf=4, k=2, precision=0, r1=0, r2=0, kind=3, same=1, nsym=1, pad=0, eta=1.5, eps=0.0,
              reg=0.0, ls=0.1, nblocks=1, ndense=1, naca=0, u=0, v=0, eb=8, checksum=0)
     f.update(kw)
@@ -170,7 +170,7 @@ def test_load_refuses_bad_files_without_the_gpu(tmp_path):
         "missing": None,
         "magic": _store_header(magic=b"NOTHMGLB") + bytes(160),
         "short": _store_header()[:100],
-        "version": _store_header(version=2) + bytes(160),
+        "version": _store_header(version=1) + bytes(160),
         "precision": _store_header(precision=1, eb=8) + bytes(160),
         "dim": _store_header(dim=9) + bytes(160),
         "points": _store_header() + bytes(80),   # 2 x 10 doubles needed, 80 bytes given

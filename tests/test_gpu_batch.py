@@ -1,0 +1,109 @@
+"""GPU tests of the many-cloud path (hmg_setup_batch, hmg_precompute_aca_batch,
+hmg_apply_batch): clouds of the batching workload's kind (8-blob Gaussian
+mixtures, 2D and 3D, Gaussian l = 0.1, leaf 64, eta 1.2, relative ACA tolerance
+1e-5 with rank cap 32, 1-8 right-hand sides) whose admissible blocks all run in
+one ACA launch set.  Ranks and products must be bitwise those of each H-matrix
+built alone (same kernels, same per-block arithmetic), and each cloud's product
+is checked against the oracle's tolerance ACA + H-MVP."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _clouds(n_clouds, seed0=4101, lo=10, span=3):
+    out = []
+    for c in range(n_clouds):
+        rng = np.random.default_rng(seed0 + c)
+        n = int(2 ** (lo + span * rng.random()))
+        dim = 2 if c % 2 == 0 else 3
+        centres = rng.random((8, dim))
+        lab = rng.integers(0, 8, n)
+        X = np.clip(centres[lab].T + 0.05 * rng.standard_normal((dim, n)), 0.0, 1.0)
+        nrhs = int(rng.integers(1, 9))
+        out.append((X, dim, n, nrhs, np.asfortranarray(rng.standard_normal((n, nrhs)))))
+    return out
+
+
+def _hm(gpu, X, dim, prec, kind=O.K_GAUSSIAN, ls=0.1, eps=1e-5):
+    hm = gpu.HMatrix(X, X, 32 if dim == 2 else 20, 1.2, 50, 64, 32, epsilon=eps)
+    hm.set_option(gpu.OPT_ACA_TOLERANCE, 2)
+    hm.set_precision(prec)
+    kern = gpu.Kernel(kind, 0.0, ls)
+    return hm, kern
+
+
+@pytest.mark.parametrize("prec_name", ["fp64", "mixed"])
+def test_batched_aca_same_as_alone_and_oracle(gpu, prec_name):
+    from hmglib_amd import hip
+    from hmglib_amd.hmatrix import apply_batch, precompute_aca_batch, setup_batch
+    prec = gpu.PREC_MIXED if prec_name == "mixed" else gpu.PREC_FP64
+    eps = 1e-5
+    clouds = _clouds(12)
+    hms, kerns = [], []
+    for X, dim, n, nrhs, Xh in clouds:
+        hm, kern = _hm(gpu, X, dim, prec)
+        hms.append(hm)
+        kerns.append(kern)
+    # one cloud with another kernel and precision: a group of its own in the same call
+    Xo, dimo, no, nro, Xho = _clouds(1, seed0=4301)[0]
+    other, kother = _hm(gpu, Xo, dimo, gpu.PREC_MIXED if prec_name == "fp64" else gpu.PREC_FP64,
+                        kind=O.K_EXPONENTIAL, ls=0.2)
+    hms.append(other)
+    kerns.append(kother)
+    clouds.append((Xo, dimo, no, nro, Xho))
+    setup_batch(hms, threads=6)
+    for hm, kern in zip(hms, kerns):
+        hm.set_kernel(kern)
+    precompute_aca_batch(hms, threads=6)
+    xds = [hip.DeviceArray(c[4]) for c in clouds]
+    yds = [hip.DeviceArray.empty((c[2], c[3])) for c in clouds]
+    apply_batch(hms, xds, yds, [c[3] for c in clouds], threads=6)
+    for i, ((X, dim, n, nrhs, Xh), hm) in enumerate(zip(clouds, hms)):
+        Y = yds[i].download().reshape(n, nrhs)
+        # alone: the same H-matrix through precompute_aca
+        kind, ls = (O.K_EXPONENTIAL, 0.2) if i == len(hms) - 1 else (O.K_GAUSSIAN, 0.1)
+        p_i = (gpu.PREC_MIXED if prec_name == "fp64" else gpu.PREC_FP64) if i == len(hms) - 1 else prec
+        ha, ka = _hm(gpu, X, dim, p_i, kind=kind, ls=ls)
+        ha.setup()
+        ha.set_kernel(ka)
+        ha.precompute_aca()
+        assert ha.work() == hm.work()
+        np.testing.assert_array_equal(Y, ha.apply(Xh).reshape(n, nrhs))
+        ha.destroy()
+        # the oracle
+        om = O.HMatrix(X, X, 32 if dim == 2 else 20, 1.2, 50, 64)
+        ok = O.kernel(kind, dim, 0.0, ls)
+        om.aca(ok, 32, tol_mode=2, eps=eps)
+        Yo = om.hmvp(ok, Xh).reshape(n, nrhs)
+        Yd = om.dense_mvp(ok, Xh).reshape(n, nrhs)
+        for c in range(nrhs):
+            if p_i == gpu.PREC_FP64:
+                # rank decisions at the tolerance threshold may differ in a few
+                # blocks (exp last ulps against glibc)
+                assert _rel(Y[:, c], Yo[:, c]) < 10 * eps
+            else:
+                assert _rel(Y[:, c], Yd[:, c]) < _rel(Yo[:, c], Yd[:, c]) + 4e-5
+    for hm in hms:
+        hm.destroy()
+
+
+def test_batch_rejects_repeated_and_unready(gpu):
+    from hmglib_amd.hmatrix import precompute_aca_batch
+    X, dim, n, nrhs, Xh = _clouds(1)[0]
+    hm, kern = _hm(gpu, X, dim, gpu.PREC_FP64)
+    with pytest.raises(gpu.HMGError, match="not set up"):
+        precompute_aca_batch([hm])
+    hm.setup()
+    hm.set_kernel(kern)
+    with pytest.raises(gpu.HMGError, match="listed twice"):
+        precompute_aca_batch([hm, hm])
+    precompute_aca_batch([hm])
+    assert hm.work()["rank_sum"] > 0
+    hm.destroy()

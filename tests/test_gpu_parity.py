@@ -233,13 +233,36 @@ def test_kernels_independent_sets(gpu, kind, ls):
 
 
 def test_precomputed_dense_matches_on_the_fly(gpu):
+    """precompute_dense (hmglib.cu:239-268, linear_algebra.cu:3140-3342): every
+    stored DENSE block, column-major m1 x m2, contiguous in block order, against
+    the kernel evaluated in numpy on the oracle's sorted points; the product with
+    the stored near field against the oracle's H-MVP and the on-the-fly product."""
+    from hmglib_amd.hip import memcpy_dtoh
     X = _points(1001, 2, 4096)
-    hm, _, om, ok = _pair(gpu, X, X, 32, 1.5, 64, 32, O.K_GAUSSIAN, 0.1)
+    hm, _, om, ok = _pair(gpu, X, X, 32, 1.5, 64, 32, O.K_GAUSSIAN, 0.1, reg=0.3)
     x = O.uniform(1003, 4096) * 2.0 - 1.0
     y_fly = hm.apply(x)
     hm.precompute_dense()
     y_st = hm.apply(x)
     assert _rel(y_st, y_fly) < 1e-14
+    om.aca(ok, 32)
+    assert _rel(y_st, om.hmvp(ok, x)) < TOL_FP64
+    b = om.blocks()[:om.ndense]
+    m1 = (b["set1_u"] - b["set1_l"] + 1).astype(np.int64)
+    m2 = (b["set2_u"] - b["set2_l"] + 1).astype(np.int64)
+    A = np.empty(int((m1 * m2).sum()))
+    memcpy_dtoh(A, hm.data.dA)
+    Xs = om.sorted_coords(0)
+    off = 0
+    for q in range(om.ndense):
+        l1, l2 = int(b["set1_l"][q]), int(b["set2_l"][q])
+        i = np.arange(l1, l1 + m1[q])
+        j = np.arange(l2, l2 + m2[q])
+        r2 = ((Xs[:, i][:, :, None] - Xs[:, j][:, None, :]) ** 2).sum(axis=0)
+        K = np.exp(-r2 / 0.02) + 0.3 * (i[:, None] == j[None, :])
+        Ab = A[off:off + m1[q] * m2[q]].reshape(m2[q], m1[q]).T   # column-major m1 x m2
+        np.testing.assert_allclose(Ab, K, rtol=1e-14, atol=1e-300)
+        off += m1[q] * m2[q]
 
 
 def test_not_precomputed_aca_matches_precomputed(gpu):
@@ -250,6 +273,10 @@ def test_not_precomputed_aca_matches_precomputed(gpu):
     hm.precompute_aca()
     y_pre = hm.apply(x)
     np.testing.assert_array_equal(y_fly, y_pre)
+    # the ACA recomputed inside the product (linear_algebra.cu:4359-4370) against
+    # the oracle's fixed-rank ACA + H-MVP
+    om.aca(ok, 12)
+    assert _rel(y_fly, om.hmvp(ok, x)) < TOL_FP64
 
 
 def test_without_batching_first_argmax(gpu):

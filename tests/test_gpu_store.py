@@ -41,7 +41,20 @@ def test_save_load_same_products(gpu, tmp_path, prec_name, same):
     kern.free()
     h2 = gpu.HMatrix.load(path)
     assert h2.work() == w
-    np.testing.assert_array_equal(h2.apply(Xr), y)
+    y2 = h2.apply(Xr)
+    np.testing.assert_array_equal(y2, y)
+    # the reloaded H-matrix against the oracle (FP64: its H-MVP; mixed: the BF16x3
+    # arithmetic bound relative to sum_j K_ij |x_j|, positive kernel)
+    om = O.HMatrix(X1, X2, 20, 1.0, 50, 128)
+    ok = O.kernel(O.K_EXPONENTIAL, 3, 0.5 if same else 0.0, 0.1)
+    om.aca(ok, 16)
+    Yo = om.hmvp(ok, Xr)
+    for c in range(R):
+        if prec_name == "fp64":
+            assert np.linalg.norm(y2[:, c] - Yo[:, c]) / np.linalg.norm(Yo[:, c]) < 1e-8
+        else:
+            mag = np.linalg.norm(om.hmvp(ok, np.abs(Xr[:, c])))
+            assert np.linalg.norm(y2[:, c] - Yo[:, c]) / mag < 4e-5
     np.testing.assert_array_equal(h2.blocks(O.WORK_ITEM_DTYPE).view(np.uint8),
                                   _blocks_of(gpu, X1, X2, h2.blocks(O.WORK_ITEM_DTYPE).dtype))
     # saving the loaded H-matrix gives the same bytes
@@ -57,6 +70,26 @@ def _blocks_of(gpu, X1, X2, dtype):
     b = hm.blocks(dtype).view(np.uint8)
     hm.destroy()
     return b
+
+
+def test_corrupted_header_is_refused(gpu, tmp_path):
+    """The checksum covers the header: a changed kernel length scale (a valid
+    value that the loader would otherwise accept) is refused."""
+    X = _points(8211, 2, 2500)
+    hm, kern = _built(gpu, X, X, 32, 1.5, 64, 16, O.K_GAUSSIAN, 0.1, 0.0, gpu.PREC_FP64)
+    path = tmp_path / "h.hmg"
+    hm.save(path)
+    raw = bytearray(path.read_bytes())
+    ls = np.float64(0.1).tobytes()
+    at = raw.find(ls, 0, 160)
+    assert at > 0
+    raw[at:at + 8] = np.float64(0.2).tobytes()
+    bad = tmp_path / "bad.hmg"
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(gpu.HMGError, match="checksum"):
+        gpu.HMatrix.load(bad)
+    hm.destroy()
+    kern.free()
 
 
 def test_corrupted_payload_is_refused(gpu, tmp_path):

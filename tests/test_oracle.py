@@ -130,16 +130,29 @@ def _fma(a, b, c):
     return float(Fraction(a) * Fraction(b) + Fraction(c))
 
 
-def _tree_brute(X1s, X2s, codes1, codes2, eta, max_level, c_leaf):
-    """Level-by-level traversal with exact boxes computed directly from the points."""
+def _tree_brute(X1s, X2s, codes1, codes2, eta, max_level, c_leaf, roots=(0, 0)):
+    """Level-by-level traversal with exact boxes computed directly from the points.
+    Nodes whose cluster levels differ split only the side with the lower level
+    into two children, whatever their admissibility (tree.cu:258-264, 395-444);
+    empty clusters (a one-point cluster split at its middle) are dropped."""
     def box(X, l, u):
         return X[:, l:u + 1].min(axis=1), X[:, l:u + 1].max(axis=1)
 
-    cur = [(0, X1s.shape[1] - 1, 0, X2s.shape[1] - 1, 0, 0)]
+    cur = [(0, X1s.shape[1] - 1, 0, X2s.shape[1] - 1, roots[0], roots[1])]
     leaves = []
     for level in range(max_level):
         nxt = []
         for (l1, u1, l2, u2, v1, v2) in cur:
+            if l1 > u1 or l2 > u2:
+                continue
+            if v1 != v2:
+                if v1 < v2:
+                    s1 = _split_brute(codes1, l1, u1)
+                    nxt += [(l1, s1, l2, u2, v1 + 1, v2), (s1 + 1, u1, l2, u2, v1 + 1, v2)]
+                else:
+                    s2 = _split_brute(codes2, l2, u2)
+                    nxt += [(l1, u1, l2, s2, v1, v2 + 1), (l1, u1, s2 + 1, u2, v1, v2 + 1)]
+                continue
             adm = _admissible(box(X1s, l1, u1), box(X2s, l2, u2), eta)
             if adm or (u1 - l1 + 1) <= c_leaf or (u2 - l2 + 1) <= c_leaf or level + 1 >= max_level:
                 leaves.append((l1, u1, l2, u2, O.WT_ACA if adm else O.WT_DENSE, v1, v2))
@@ -171,6 +184,29 @@ def test_tree_matches_brute_force(dim, bits, n1, n2, eta, c_leaf, seed):
               int(b["work_type"]), int(b["level_1"]), int(b["level_2"])) for b in got]
     assert got_t == ref
     assert hm.ndense == sum(1 for b in ref if b[4] == O.WT_DENSE)
+
+
+@pytest.mark.parametrize("dim,bits,n1,n2,eta,c_leaf,roots,seed", [
+    (2, 32, 300, 260, 1.2, 16, (0, 2), 5), (2, 32, 300, 260, 1.2, 16, (3, 1), 6),
+    (2, 32, 300, 260, 1.2, 16, (1, 0), 7), (2, 32, 300, 260, 1.2, 24, (0, 2), 10),
+    (3, 20, 320, 280, 1.2, 16, (1, 0), 5)])
+def test_tree_unequal_root_levels_matches_brute_force(dim, bits, n1, n2, eta, c_leaf, roots, seed):
+    """Root levels that differ: the traversal first splits the lower-level side in
+    two until the levels meet (tree.cu:258-264, 395-444), then proceeds as usual;
+    block list, work types and cluster levels against the brute-force traversal."""
+    X1 = O.uniform(seed, dim * n1).reshape(dim, n1)
+    X2 = O.uniform(seed + 100, dim * n2).reshape(dim, n2)
+    hm = O.HMatrix(X1, X2, bits, eta, 50, c_leaf, *roots)
+    diag = hm.diagnostics()
+    assert diag["box_differs"] == 0 and diag["undefined_table"] == 0
+    ref = _tree_brute(hm.sorted_coords(0), hm.sorted_coords(1), hm.codes(0), hm.codes(1), eta, 50,
+                      c_leaf, roots)
+    got = hm.blocks()
+    got_t = [(int(b["set1_l"]), int(b["set1_u"]), int(b["set2_l"]), int(b["set2_u"]),
+              int(b["work_type"]), int(b["level_1"]), int(b["level_2"])) for b in got]
+    assert got_t == ref
+    # every leaf was reached after the levels met
+    assert all(b[5] == b[6] for b in ref) and hm.naca > 0
 
 
 def test_tree_invariants_tiling_and_admissibility():
@@ -243,6 +279,81 @@ def _quirk_points(n):
     return X
 
 
+def _literal_level_boxes(X, ranges):
+    """Boxes of one level's clusters as the reference's lookup table forms them
+    (tree.cu:852-1147, the rule written out in the hmglib survey's appendix A.4),
+    restated independently with numpy: keys +t+1 at each distinct lower bound,
+    then -(t+1) at each distinct upper bound (a one-point cluster keeps the minus),
+    an inclusive prefix sum, +t+1 written back at each upper bound, maximal runs of
+    equal keys with key 0 dropped, the k-th run's min/max as table entry k, and
+    table entry t for the clusters starting at the t-th lower bound."""
+    n = X.shape[1]
+    L = sorted({l for l, u in ranges})
+    U = sorted({u for l, u in ranges})
+    assert len(L) == len(U)
+    key = np.zeros(n, dtype=np.int64)
+    for t in range(len(L)):
+        key[L[t]] = t + 1
+        key[U[t]] = -(t + 1)
+    key = np.cumsum(key)
+    for t in range(len(U)):
+        key[U[t]] = t + 1
+    starts = np.flatnonzero(np.r_[True, key[1:] != key[:-1]])
+    ends = np.r_[starts[1:], n]
+    table = [(X[:, a:b].min(axis=1), X[:, a:b].max(axis=1)) for a, b in zip(starts, ends) if key[a] != 0]
+    assert len(table) >= len(L)   # no entry read past the last run
+    where = {l: t for t, l in enumerate(L)}
+    return {(l, u): table[where[l]] for l, u in ranges}
+
+
+def _tree_brute_literal(X1s, X2s, codes1, codes2, eta, max_level, c_leaf):
+    """The brute-force traversal with every level's boxes from _literal_level_boxes
+    (equal root levels); returns the leaves with their boxes, DENSE first."""
+    cur = [(0, X1s.shape[1] - 1, 0, X2s.shape[1] - 1)]
+    leaves = []
+    for level in range(max_level):
+        cur = [c for c in cur if c[0] <= c[1] and c[2] <= c[3]]
+        if not cur:
+            break
+        b1 = _literal_level_boxes(X1s, [(c[0], c[1]) for c in cur])
+        b2 = _literal_level_boxes(X2s, [(c[2], c[3]) for c in cur])
+        nxt = []
+        for (l1, u1, l2, u2) in cur:
+            bx1, bx2 = b1[(l1, u1)], b2[(l2, u2)]
+            adm = _admissible(bx1, bx2, eta)
+            if adm or (u1 - l1 + 1) <= c_leaf or (u2 - l2 + 1) <= c_leaf or level + 1 >= max_level:
+                leaves.append((l1, u1, l2, u2, O.WT_ACA if adm else O.WT_DENSE, level, bx1, bx2))
+            else:
+                s1 = _split_brute(codes1, l1, u1)
+                s2 = _split_brute(codes2, l2, u2)
+                nxt += [(l1, s1, l2, s2), (l1, s1, s2 + 1, u2), (s1 + 1, u1, l2, s2), (s1 + 1, u1, s2 + 1, u2)]
+        cur = nxt
+    return [b for b in leaves if b[4] == O.WT_DENSE] + [b for b in leaves if b[4] == O.WT_ACA]
+
+
+@pytest.mark.parametrize("n,c_leaf,eta,seed", [(400, 2, 1.2, 91), (600, 1, 1.0, 92), (500, 3, 1.5, 93)])
+def test_quirk_literal_boxes_match_brute_force(n, c_leaf, eta, seed):
+    """Clustered points and tiny leaves make one-point clusters on many levels, so
+    the lookup table shifts boxes (the reference's size-1 quirk).  The whole block
+    list, boxes included (bit for bit), against the brute-force traversal that
+    forms every level's boxes with the literal key/scan rule."""
+    X = O.uniform(seed, 2 * n).reshape(2, n) ** 3
+    hm = O.HMatrix(X, X, 32, eta, 50, c_leaf)
+    d = hm.diagnostics()
+    assert d["singleton_levels"] >= 2 and d["box_differs"] > 0 and d["undefined_table"] == 0
+    ref = _tree_brute_literal(hm.sorted_coords(0), hm.sorted_coords(1), hm.codes(0), hm.codes(1), eta, 50,
+                              c_leaf)
+    got = hm.blocks()
+    assert len(got) == len(ref)
+    for w, r in zip(got, ref):
+        assert (int(w["set1_l"]), int(w["set1_u"]), int(w["set2_l"]), int(w["set2_u"]), int(w["work_type"]),
+                int(w["level_1"])) == r[:6]
+        np.testing.assert_array_equal(w["min1"][:2], r[6][0])
+        np.testing.assert_array_equal(w["max1"][:2], r[6][1])
+        np.testing.assert_array_equal(w["min2"][:2], r[7][0])
+        np.testing.assert_array_equal(w["max2"][:2], r[7][1])
+
+
 def test_quirk_hand_worked_example():
     """tree.cu:1000-1024 with a singleton cluster t=0 = [0,0] and t=1 = [1,N-1]:
     keys[0] = +1 then -1 (same thread); keys[1] = +2; keys[N-1] = -2; inclusive
@@ -265,6 +376,41 @@ def test_quirk_hand_worked_example():
             np.testing.assert_array_equal(w[hi_f][:2], expect[1])
     d = hm.diagnostics()
     assert d["singleton_levels"] >= 2 and d["box_differs"] > 0
+
+
+# ---------------------------------------------------------------- batch sizing
+@pytest.mark.parametrize("max_aca", [65536, 1 << 30, 3000, 200, 10])
+def test_batch_sizes_are_greedy_maximal_partitions(max_aca):
+    """Batch sizing of setup (hmglib.cu:211-219, linear_algebra.cu:4059-4288).
+    The dense cap is 1.05 x the dense entries, so the dense walk takes every DENSE
+    block in one batch and then records its terminating 0.  The ACA walk cuts the
+    ACA blocks, in block order, into batches whose sums of m1 stay within the cap,
+    each as long as possible (the next block would cross the cap), ending with a
+    recorded 0; a first block larger than the cap gives [0] (the reference then
+    builds no ACA block at all, hmglib.cu:232)."""
+    n = 4096
+    X = O.uniform(1001, 2 * n).reshape(2, n)
+    hm = O.HMatrix(X, X, 32, 1.5, 50, 64)
+    b = hm.blocks()
+    m1 = (b["set1_u"] - b["set1_l"] + 1).astype(np.int64)
+    m2 = (b["set2_u"] - b["set2_l"] + 1).astype(np.int64)
+    cap, dws, aws = hm.work_sizes(max_aca)
+    e_near = int((m1 * m2)[:hm.ndense].sum())
+    assert e_near == hm.work()[0]
+    assert cap == int(e_near * 1.05)
+    assert list(dws) == [hm.ndense, 0]
+    assert aws[-1] == 0 and np.all(aws[:-1] > 0)
+    m1a = m1[hm.ndense:]
+    if m1a[0] > max_aca:
+        assert list(aws) == [0]
+        return
+    assert aws.sum() == hm.naca
+    at = 0
+    for w in aws[:-1]:
+        assert m1a[at:at + w].sum() <= max_aca
+        if at + w < hm.naca:
+            assert m1a[at:at + w + 1].sum() > max_aca
+        at += w
 
 
 # ---------------------------------------------------------------- kernels
