@@ -156,6 +156,7 @@ SIGNATURES = {
     "hmg_set_gaussian_kernel_rhs": (ctypes.c_int, [_vp, _D]),
     # extensions
     "hmg_apply_h_matrix_mvp_multi": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _D]),
+    "hmg_apply_h_matrix_mvp_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _D]),
     "hmg_autotune_product": (ctypes.c_int, [_D, _vp, _vp, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int64)]),
     "hmg_dense_rows": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _D]),
     "hmg_get_work": (ctypes.c_int, [_D, _P(ctypes.c_int64)]),

@@ -353,6 +353,7 @@ struct Engine {
     DevBuf<double> Tp;                          // V^T x partials per row chunk
     Combine comb_stored, comb_fly;              // stored near field / matrix-free
     DevBuf<double> P, xs, T;
+    int64_t p_slots = 0;                        // FP64 partial-sum slots of P
 
     hipStream_t stream2 = nullptr;              // near field runs beside the low-rank part
     // tuning: LDS bytes reserved per near-field workgroup (caps its occupancy so
@@ -523,6 +524,24 @@ hmg_status dense_fill(Engine& eng, const Coords& c1, const Coords& c2, const Ker
 hmg_status hmvp(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
                 const uint64_t* order1, const uint64_t* order2, const double* x, double* y,
                 int nrhs, const double* U, const double* V, const int* keff, bool dense_stored);
+// The per-call buffers of the FP64 product: x in sorted order, V^T x partials per
+// point chunk, t in U order, and the row / column partial sums every row adds up.
+// Every element the product reads is written earlier in the same call.
+struct MvpBufs {
+    double* xs;
+    double* Tp;
+    double* T;
+    double* P;
+};
+// their element counts for the engine's current schedule
+struct MvpSizes {
+    int64_t xs, Tp, T, P;
+};
+MvpSizes mvp_sizes(const Engine& eng);
+// hmvp with caller-provided buffers (allocates nothing)
+hmg_status hmvp_with(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
+                     const uint64_t* order1, const uint64_t* order2, const double* x, double* y, int nrhs,
+                     const double* U, const double* V, const int* keff, bool dense_stored, const MvpBufs& b);
 hmg_status full_mvp(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
                     const uint64_t* order1, const uint64_t* order2, const double* x, double* y,
                     int nrhs);

@@ -470,45 +470,64 @@ namespace {
 
 template <int KIND, int DIM>
 void launch_near_fly(bool reg, unsigned grid, hipStream_t s, const Coords& c1, const Coords& c2,
-                     const KernelParams& kp, const Engine& eng, double* P)
+                     const KernelParams& kp, const Engine& eng, const double* xs, double* P)
 {
     double* Pnear = P + (int64_t)eng.n_lr_items * kTileRows;
     if (reg)
         near_fly_kernel<KIND, DIM, true><<<grid, kThreads, eng.near_lds_pad, s>>>(
-            c1, c2, kp, eng.fly_items.p, eng.n_fly_items, eng.fly_refs.p, eng.xs.p, Pnear, P);
+            c1, c2, kp, eng.fly_items.p, eng.n_fly_items, eng.fly_refs.p, xs, Pnear, P);
     else
         near_fly_kernel<KIND, DIM, false><<<grid, kThreads, eng.near_lds_pad, s>>>(
-            c1, c2, kp, eng.fly_items.p, eng.n_fly_items, eng.fly_refs.p, eng.xs.p, Pnear, P);
+            c1, c2, kp, eng.fly_items.p, eng.n_fly_items, eng.fly_refs.p, xs, Pnear, P);
 }
 
 template <int KIND>
-void launch_near_fly_dim(bool reg, unsigned grid, hipStream_t s, const Coords& c1,
-                         const Coords& c2, const KernelParams& kp, const Engine& eng, double* P)
+void launch_near_fly_dim(bool reg, unsigned grid, hipStream_t s, const Coords& c1, const Coords& c2,
+                         const KernelParams& kp, const Engine& eng, const double* xs, double* P)
 {
     if (kp.dim == 3)
-        launch_near_fly<KIND, 3>(reg, grid, s, c1, c2, kp, eng, P);
+        launch_near_fly<KIND, 3>(reg, grid, s, c1, c2, kp, eng, xs, P);
     else if (kp.dim == 2)
-        launch_near_fly<KIND, 2>(reg, grid, s, c1, c2, kp, eng, P);
+        launch_near_fly<KIND, 2>(reg, grid, s, c1, c2, kp, eng, xs, P);
     else
-        launch_near_fly<KIND, 0>(reg, grid, s, c1, c2, kp, eng, P);
+        launch_near_fly<KIND, 0>(reg, grid, s, c1, c2, kp, eng, xs, P);
 }
 
 }  // namespace
+
+MvpSizes mvp_sizes(const Engine& eng)
+{
+    MvpSizes z;
+    z.xs = std::max(eng.n[1], 1);
+    z.Tp = (int64_t)std::max(eng.n_lv_items, 1) * kTileRows;
+    z.T = std::max<int64_t>(eng.tsize, 1);
+    z.P = std::max<int64_t>(eng.p_slots, 1);
+    return z;
+}
 
 hmg_status hmvp(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
                 const uint64_t* order1, const uint64_t* order2, const double* x, double* y, int nrhs,
                 const double* U, const double* V, const int* keff, bool dense_stored)
 {
+    HMG_TRY(eng.xs.alloc(std::max(eng.n[1], 1)));
+    const MvpBufs b{eng.xs.p, eng.Tp.p, eng.T.p, eng.P.p};
+    return hmvp_with(eng, c1, c2, kp, order1, order2, x, y, nrhs, U, V, keff, dense_stored, b);
+}
+
+hmg_status hmvp_with(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
+                     const uint64_t* order1, const uint64_t* order2, const double* x, double* y, int nrhs,
+                     const double* U, const double* V, const int* keff, bool dense_stored, const MvpBufs& b)
+{
+    (void)keff;
     hipStream_t s = eng.stream, s2 = eng.serialize ? eng.stream : eng.stream2;
     const int n1 = eng.n[0], n2 = eng.n[1];
     const int na = eng.naca;
-    HMG_TRY(eng.xs.alloc(std::max(n2, 1)));
     const bool reg = kp.reg != 0.0;
     for (int c = 0; c < nrhs; c++) {
         const double* xc = x + (int64_t)c * n2;
         double* yc = y + (int64_t)c * n1;
         eng.prof.begin(PROF_GATHER, s);
-        gather_perm_kernel<<<grid_for(n2, kThreads), kThreads, 0, s>>>(xc, order2, eng.xs.p, n2);
+        gather_perm_kernel<<<grid_for(n2, kThreads), kThreads, 0, s>>>(xc, order2, b.xs, n2);
         HMG_HIP(hipGetLastError());
         eng.prof.end(PROF_GATHER, s);
         // near field on the second stream
@@ -521,20 +540,20 @@ hmg_status hmvp(Engine& eng, const Coords& c1, const Coords& c2, const KernelPar
             if (dense_stored) {
                 near_stored_kernel<<<g, kThreads, eng.near_lds_pad, s2>>>(
                     eng.near_items.p, eng.n_near_items, eng.near_blk.p, eng.blk.p, eng.aoff.p, eng.A.p,
-                    eng.xs.p, eng.P.p + (int64_t)eng.n_lr_items * kTileRows);
+                    b.xs, b.P + (int64_t)eng.n_lr_items * kTileRows);
             } else {
                 switch (kp.kind) {
                 case HMG_KERNEL_GAUSSIAN:
-                    launch_near_fly_dim<HMG_KERNEL_GAUSSIAN>(reg, g, s2, c1, c2, kp, eng, eng.P.p);
+                    launch_near_fly_dim<HMG_KERNEL_GAUSSIAN>(reg, g, s2, c1, c2, kp, eng, b.xs, b.P);
                     break;
                 case HMG_KERNEL_EXPONENTIAL:
-                    launch_near_fly_dim<HMG_KERNEL_EXPONENTIAL>(reg, g, s2, c1, c2, kp, eng, eng.P.p);
+                    launch_near_fly_dim<HMG_KERNEL_EXPONENTIAL>(reg, g, s2, c1, c2, kp, eng, b.xs, b.P);
                     break;
                 case HMG_KERNEL_GAUSSIAN_REF:
-                    launch_near_fly_dim<HMG_KERNEL_GAUSSIAN_REF>(reg, g, s2, c1, c2, kp, eng, eng.P.p);
+                    launch_near_fly_dim<HMG_KERNEL_GAUSSIAN_REF>(reg, g, s2, c1, c2, kp, eng, b.xs, b.P);
                     break;
                 default:
-                    launch_near_fly_dim<HMG_KERNEL_MATERN_REF>(reg, g, s2, c1, c2, kp, eng, eng.P.p);
+                    launch_near_fly_dim<HMG_KERNEL_MATERN_REF>(reg, g, s2, c1, c2, kp, eng, b.xs, b.P);
                     break;
                 }
             }
@@ -546,15 +565,15 @@ hmg_status hmvp(Engine& eng, const Coords& c1, const Coords& c2, const KernelPar
         if (na > 0 && eng.n_lr_items > 0) {
             eng.prof.begin(PROF_LR_VTX, s);
             lr_v_kernel<<<grid_for(eng.n_lv_items, kWaves), kThreads, 0, s>>>(
-                eng.lv_items.p, eng.n_lv_items, V, eng.xs.p, eng.Tp.p);
+                eng.lv_items.p, eng.n_lv_items, V, b.xs, b.Tp);
             HMG_HIP(hipGetLastError());
             lr_v_finalize_kernel<<<grid_for(eng.n_lv_slices, kWaves), kThreads, 0, s>>>(
-                eng.lv_slices.p, eng.n_lv_slices, eng.Tp.p, eng.tmap.p, eng.T.p);
+                eng.lv_slices.p, eng.n_lv_slices, b.Tp, eng.tmap.p, b.T);
             HMG_HIP(hipGetLastError());
             eng.prof.end(PROF_LR_VTX, s);
             eng.prof.begin(PROF_LR_U, s);
             lr_u_kernel<<<grid_for(eng.n_lr_items, kWaves), kThreads, 0, s>>>(
-                eng.lr_items.p, eng.n_lr_items, U, eng.T.p, eng.P.p);
+                eng.lr_items.p, eng.n_lr_items, U, b.T, b.P);
             HMG_HIP(hipGetLastError());
             eng.prof.end(PROF_LR_U, s);
         }
@@ -562,7 +581,7 @@ hmg_status hmvp(Engine& eng, const Coords& c1, const Coords& c2, const KernelPar
         eng.prof.begin(PROF_COMBINE, s);
         const Combine& cb = dense_stored ? eng.comb_stored : eng.comb_fly;
         combine_kernel<<<grid_for(n1, kThreads), kThreads, 0, s>>>(cb.ptr.p, cb.idx.p, cb.groups.p,
-                                                                   eng.P.p, order1, yc, n1);
+                                                                   b.P, order1, yc, n1);
         HMG_HIP(hipGetLastError());
         eng.prof.end(PROF_COMBINE, s);
     }

@@ -594,7 +594,8 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
     if (mixed) HMG_TRY(upload(eng.mxp, eng.h_mxp, s));
     if (!mixed) {
         HMG_TRY(eng.Tp.alloc((size_t)std::max(eng.n_lv_items, 1) * kSlice));
-        HMG_TRY(eng.P.alloc((size_t)(col0 + eng.n_col_slots)));
+        eng.p_slots = col0 + eng.n_col_slots;
+        HMG_TRY(eng.P.alloc((size_t)eng.p_slots));
         HMG_TRY(eng.T.alloc(std::max<int64_t>(eng.tsize, 1)));
     }
     HMG_HIP(hipStreamSynchronize(s));

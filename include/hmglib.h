@@ -146,6 +146,16 @@ hmg_status hmg_setup_batch(struct h_matrix_data* const* data, int count, int thr
 hmg_status hmg_precompute_aca_batch(struct h_matrix_data* const* data, int count, int threads);
 hmg_status hmg_apply_batch(struct h_matrix_data* const* data, const double* const* X, double* const* Y,
                            const int* nrhs, int count, int threads);
+/* Y = H X for an H-matrix with precomputed FP64 factors, allocating nothing (no
+ * counterpart in the reference; for callers that cannot allocate device memory:
+ * code under stream capture, serving loops that forbid allocation).  Same kernels
+ * and summation order as hmg_apply_h_matrix_mvp_multi, so the result is bitwise
+ * its result.  The per-call buffers live in a fixed device array of 32 MiB that
+ * the library holds on every device it runs on until the process ends; an
+ * H-matrix whose buffers need more is refused with HMG_ERR_INVALID, mixed-precision
+ * or missing factors with HMG_ERR_NOT_READY.  Calls on one device share the array:
+ * calls that overlap on the same device race. */
+hmg_status hmg_apply_h_matrix_mvp_noalloc(const double* X, double* Y, int nrhs, struct h_matrix_data* data);
 /* Autotuner of the product's launch shape (no counterpart in the reference): times
  * `reps` products (after one warm-up) for each candidate -- the near field's LDS
  * reservation of 0, 32, 48, 64 or 80 KB per workgroup with the near field and the
