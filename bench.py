@@ -160,9 +160,52 @@ def measured_traffic(hm, kname, traffic_name):
     return (k["read_GB"] + k["write_GB"]) * 1e9
 
 
+def host_info():
+    """CPU model, logical CPUs (and those this process may run on), physical cores
+    and RAM of the host, from /proc."""
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        info["usable_cpus"] = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    try:
+        cores, model = set(), None
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+        info["cpu_model"] = model
+        info["physical_cores"] = len(cores) or None
+        info["sockets"] = len({p for p, _ in cores}) or None
+    except Exception:
+        pass
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal:"):
+                    info["ram_GiB"] = round(int(line.split()[1]) / 2**20, 1)
+    except Exception:
+        pass
+    return info
+
+
 def cpu_baseline(X, x, cfg, stride):
     """The oracle (plain C/OpenMP port of the same algorithm) on a bounded sample:
-    every `stride`-th block of the full tree, time scaled by entries touched."""
+    every `stride`-th block of the full tree, time scaled by entries touched.  x
+    holds the workload's right-hand sides (n x R): the sampled product evaluates
+    each near-field entry once for all R, as the GPU pass does."""
     import oracle as O
     t0 = time.perf_counter()
     om = O.HMatrix(X, X, cfg["bits"], cfg["eta"], 50, cfg["c_leaf"])
@@ -171,6 +214,7 @@ def cpu_baseline(X, x, cfg, stride):
     t0 = time.perf_counter()
     e_aca_s = om.aca_sample(k, cfg["k"], stride)
     t_aca = time.perf_counter() - t0
+    R = 1 if x.ndim == 1 else x.shape[1]
     t0 = time.perf_counter()
     _, e_mvp_s = om.hmvp_sample(k, stride, x)
     t_mvp = time.perf_counter() - t0
@@ -185,12 +229,22 @@ def cpu_baseline(X, x, cfg, stride):
     scale_mvp = e_full / max(e_mvp_s, 1)
     scale_aca = e_lr_full / max(e_aca_s, 1)
     mvp_s = t_mvp * scale_mvp
-    return {"value": 1.0 / mvp_s, "unit": "H-MVP/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"oracle (C/OpenMP, -O3) on every {stride}th block of the full N={X.shape[1]} "
-                      f"tree; {t_mvp:.2f}s MVP x {scale_mvp:.1f} by entries touched, ACA build "
-                      f"{t_aca:.2f}s x {scale_aca:.1f}, tree {t_tree:.2f}s",
-            "mvp_s_est": mvp_s, "aca_s_est": t_aca * scale_aca, "tree_s": t_tree}
+    host = host_info()
+    return {"value": R / mvp_s, "unit": "H-MVP/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"oracle (C/OpenMP, -O3, FP64) on every {stride}th block of the full N={X.shape[1]} "
+                      f"tree with the workload's {R} right-hand side(s) per pass; {t_mvp:.2f}s product x "
+                      f"{scale_mvp:.1f} by entries touched = {mvp_s:.1f}s per {R}-RHS pass; ACA build "
+                      f"{t_aca:.2f}s x {scale_aca:.1f}; tree {t_tree:.2f}s; {O.num_threads()} OpenMP threads "
+                      f"on {host.get('cpu_model')} ({host.get('physical_cores')} physical cores in "
+                      f"{host.get('sockets')} socket(s), {host.get('usable_cpus')} CPUs usable, "
+                      f"{host.get('ram_GiB')} GiB RAM)",
+            "host": host, "mvp_s_est": mvp_s, "aca_s_est": t_aca * scale_aca, "tree_s": t_tree}
 
+
+RANK24_NOTE = ("the error against exact FP64 rows is the rank-24 H-approximation error at leaf 256, eta 1.0 "
+               "(the FP64 engine equals the oracle's H-MVP of the same tree and ranks to 1e-8 at these "
+               "parameters); the 1e-6 FP64 target is out of reach for rank <= 24 at this tree, the ACA "
+               "tolerance 1e-7 is met before rank 24 in almost no block")
 
 WORKLOADS = {
     # mixed precision, 32 right-hand sides, against the plain FP64 product (headline)
@@ -214,9 +268,7 @@ def serving_requests(seed_arrivals, seed_params, qps, duration):
     request a point cloud with N = floor(2^(12 + 6u)), uniform or an 8-blob
     Gaussian mixture (centres U[0,1]^d, sigma 0.05, clipped), 2D or 3D, exponential
     or Gaussian kernel (length scale 0.1), 1-16 right-hand sides; 70% applies
-    against a cached H-matrix, 30% builds.  Tree parameters follow the small and
-    the scale configuration: 2D leaf 64, eta 1.5, rank 32; 3D leaf 256, eta 1.0,
-    rank 24."""
+    against a cached H-matrix, 30% builds.  Tree and ACA parameters: serving_tree."""
     ra = np.random.default_rng(seed_arrivals)
     rp = np.random.default_rng(seed_params)
     t, out = 0.0, []
@@ -241,7 +293,15 @@ def cloud(rng, n, dim, blobs):
     return np.clip(centres[lab].T + 0.05 * rng.standard_normal((dim, n)), 0.0, 1.0)
 
 
-def serve_session(reqs, precision, budget, slots, log_tag, trace_path=None):
+def serving_tree(dim):
+    """Tree and ACA parameters of a served build: those of the small plain
+    configuration in 2D (leaf 64, eta 1.5, relative ACA tolerance 1e-6, rank cap 32)
+    and of the plain configuration at scale in 3D (leaf 256, eta 1.0, tolerance 1e-7,
+    rank cap 24).  Returns (bits, leaf, eta, rank cap, tolerance)."""
+    return (32, 64, 1.5, 32, 1e-6) if dim == 2 else (20, 256, 1.0, 24, 1e-7)
+
+
+def serve_session(reqs, precision, budget, slots, log_tag, trace_path=None, exact_boxes=True):
     """Replays `reqs` open-loop against a native server; returns latency stats.
     With trace_path, writes one JSON record per request (kind, size, times)."""
     import hmglib_amd
@@ -256,9 +316,11 @@ def serve_session(reqs, precision, budget, slots, log_tag, trace_path=None):
     srv = Server(budget_bytes=budget, slots=slots, max_rhs_per_pass=32, precision=precision,
                  product_slots=1)
     ids = []
-    mat_n = {}
+    mat_n, mat_req = {}, {}
+    sampled = []          # (request, its own output buffer): every 6th apply, checked after the run
     t0 = time.perf_counter()
     k_ring = 0
+    n_apply = 0
     for r in reqs:
         dt = r["t"] - (time.perf_counter() - t0)
         if dt > 0:
@@ -267,15 +329,23 @@ def serve_session(reqs, precision, budget, slots, log_tag, trace_path=None):
         if r["build"] or not resident:
             rng = np.random.default_rng(r["seed"])
             X = cloud(rng, r["n"], r["dim"], r["blobs"])
-            bits, leaf, eta, k = (32, 64, 1.5, 32) if r["dim"] == 2 else (20, 256, 1.0, 24)
-            rid, mid = srv.submit_build(X, X, bits, eta, 50, leaf, k, r["kind"], 0.0, 0.1)
+            bits, leaf, eta, k, eps = serving_tree(r["dim"])
+            rid, mid = srv.submit_build(X, X, bits, eta, 50, leaf, k, r["kind"], 0.0, 0.1, epsilon=eps,
+                                        aca_tolerance=2, exact_boxes=exact_boxes)
             mat_n[mid] = r["n"]
+            mat_req[mid] = r
             ids.append((rid, "build", r))
         else:
             m = resident[int(r["pick"] * len(resident))]
-            y = ring[k_ring % len(ring)]
-            k_ring += 1
+            if n_apply % 6 == 0 and len(sampled) < 16:
+                y = hip.DeviceArray.empty(mat_n[m] * r["nrhs"])
+            else:
+                y = ring[k_ring % len(ring)]
+                k_ring += 1
             rid = srv.submit_apply_ptr(m, Xpool.ptr, y.ptr, r["nrhs"])
+            if n_apply % 6 == 0 and len(sampled) < 16:
+                sampled.append((rid, y, m, r["nrhs"]))
+            n_apply += 1
             ids.append((rid, "apply", dict(r, n=mat_n[m])))
     srv.drain()
     wall = time.perf_counter() - t0
@@ -283,8 +353,10 @@ def serve_session(reqs, precision, budget, slots, log_tag, trace_path=None):
     svc = {"build": [], "apply": []}
     fails = 0
     trace = []
+    srv_status = {}
     for rid, kind, r in ids:
         info = srv.query(rid)
+        srv_status[rid] = info["status"]
         trace.append({"kind": kind, "n": r["n"], "dim": r["dim"], "nrhs": r["nrhs"], "status": info["status"],
                       "t_submit": info["t_submit"], "t_start": info["t_start"], "t_end": info["t_end"],
                       "merged": info["merged"]})
@@ -295,6 +367,20 @@ def serve_session(reqs, precision, budget, slots, log_tag, trace_path=None):
         svc[kind].append((info["t_end"] - info["t_start"]) * 1e3)
     st = srv.stats()
     srv.destroy()
+    # served results of the sampled applies against exact FP64 rows of their clouds
+    errs = []
+    xpool_h = np.random.default_rng(7).standard_normal((nmax * 16,))
+    for rid, y, m, nrhs in sampled:
+        if srv_status.get(rid, 0) != 0:
+            continue
+        q = mat_req[m]
+        n = q["n"]
+        X = cloud(np.random.default_rng(q["seed"]), n, q["dim"], q["blobs"])
+        bits, leaf, eta, k, eps = serving_tree(q["dim"])
+        Xc = np.asfortranarray(xpool_h[:n * nrhs].reshape(n, nrhs, order="F"))
+        Y = y.download().reshape(n, nrhs, order="F")
+        errs.append(sample_errors(X, bits, eta, leaf, k, q["kind"], Xc, Y, exact_boxes, 2, eps))
+        y.free()
     for b in ring:
         b.free()
     Xpool.free()
@@ -308,12 +394,49 @@ def serve_session(reqs, precision, budget, slots, log_tag, trace_path=None):
            "build_service_p50_ms": float(np.percentile(svc["build"], 50)) if svc["build"] else None,
            "apply_service_p50_ms": float(np.percentile(svc["apply"], 50)) if svc["apply"] else None,
            "builds": len(lat["build"]), "applies": len(lat["apply"]), "evictions": st["evictions"],
-           "passes": st["passes"], "shared_requests": st["shared"]}
+           "passes": st["passes"], "shared_requests": st["shared"],
+           "rel_err_exact_rows_max": max(e["exact_rows"] for e in errs) if errs else None,
+           "rel_err_vs_fp64_hmvp_max": max(e["vs_fp64_hmvp"] for e in errs) if errs else None,
+           "rel_err_sampled_applies": len(errs),
+           "rel_err_note": "every 6th apply (up to 16): its served result on 256 sampled rows per RHS "
+                           "against exact FP64 rows of its cloud (includes the H-approximation error of the "
+                           "tree parameters: leaf, eta, rank) and against the FP64 H-MVP of the same "
+                           "H-matrix built alone (the precision part)"}
     log(f"[bench] serving {log_tag}: {json.dumps(res)}")
     if trace_path:
         with open(trace_path, "w") as f:
             json.dump(trace, f)
     return res
+
+
+def sample_errors(X, bits, eta, leaf, k, kind, Xcols, Y, exact_boxes=True, tol_mode=0, eps=0.0, nrows=256,
+                  seed=6001):
+    """Errors of a served or batched result Y (n x R, the caller's order) on `nrows`
+    sampled rows, max over right-hand sides: against exact FP64 rows of the kernel
+    matrix of cloud X (hmg_dense_rows, checked against the oracle by the tests), and
+    against the FP64 H-MVP of an H-matrix built alone with the same parameters
+    (the precision part of the error)."""
+    import hmglib_amd
+    from hmglib_amd.hmatrix import HMatrix, Kernel
+    n = X.shape[1]
+    hm = HMatrix(X, X, bits, eta, 50, leaf, k, epsilon=eps if eps else 1e-6)
+    hm.set_option(hmglib_amd.OPT_EXACT_BOXES, int(exact_boxes))
+    hm.set_option(hmglib_amd.OPT_ACA_TOLERANCE, tol_mode)
+    hm.setup()
+    kern = Kernel(kind, 0.0, 0.1)
+    hm.set_kernel(kern)
+    hm.set_precision(hmglib_amd.PREC_FP64)
+    hm.precompute_aca()
+    rows = sample_rows(seed, n, min(nrows, n))
+    ex = hm.dense_rows(rows, Xcols).reshape(rows.size, -1)
+    h64 = hm.apply(Xcols).reshape(n, -1)[rows]
+    Ys = Y.reshape(n, -1)[rows]
+    hm.destroy()
+    kern.free()
+
+    def rel(a, b):
+        return max(float(np.linalg.norm(a[:, c] - b[:, c]) / np.linalg.norm(b[:, c])) for c in range(b.shape[1]))
+    return {"exact_rows": rel(Ys, ex), "vs_fp64_hmvp": rel(Ys, h64), "fp64_hmvp_exact_rows": rel(h64, ex)}
 
 
 def batch_clouds(n_clouds):
@@ -330,7 +453,7 @@ def batch_clouds(n_clouds):
     return out
 
 
-def batch_session(clouds, precision, slots, log_tag):
+def batch_session(clouds, precision, slots, log_tag, exact_boxes=True):
     """Every cloud's build, its product and its release submitted at once to a
     native server (Gaussian l = 0.1, leaf 64, eta 1.2, relative ACA tolerance 1e-5
     with rank cap 32); the clock runs until all have completed."""
@@ -345,7 +468,7 @@ def batch_session(clouds, precision, slots, log_tag):
     for c, y in zip(clouds, outs):
         bits = 32 if c["dim"] == 2 else 20
         rb, m = srv.submit_build(c["X"], c["X"], bits, 1.2, 50, 64, 32, 3, 0.0, 0.1, epsilon=1e-5,
-                                 aca_tolerance=2)
+                                 aca_tolerance=2, exact_boxes=exact_boxes)
         ra = srv.submit_apply_ptr(m, Xpool.ptr, y.ptr, c["nrhs"])
         srv.release(m)
         ids.append((rb, ra))
@@ -360,6 +483,15 @@ def batch_session(clouds, precision, slots, log_tag):
         bsvc.append((ib["t_end"] - ib["t_start"]) * 1e3)
         asvc.append((ia["t_end"] - ia["t_start"]) * 1e3)
     srv.destroy()
+    # every 32nd cloud's result against exact FP64 rows
+    errs = []
+    xpool_h = np.random.default_rng(9).standard_normal((nmax * 8,))
+    for i in range(0, len(clouds), 32):
+        c = clouds[i]
+        n, nr = c["n"], c["nrhs"]
+        Y = outs[i].download().reshape(n, nr, order="F")
+        Xc = np.asfortranarray(xpool_h[:n * nr].reshape(n, nr, order="F"))
+        errs.append(sample_errors(c["X"], 32 if c["dim"] == 2 else 20, 1.2, 64, 32, 3, Xc, Y, exact_boxes, 2, 1e-5))
     Xpool.free()
     for y in outs:
         y.free()
@@ -368,8 +500,73 @@ def batch_session(clouds, precision, slots, log_tag):
            "wall_s": wall, "clouds_per_s": len(clouds) / wall, "H-MVP_per_s_incl_build": rhs / wall,
            "build_service_p50_ms": float(np.percentile(bsvc, 50)) if bsvc else None,
            "apply_service_p50_ms": float(np.percentile(asvc, 50)) if asvc else None,
-           "apply_H-MVP_per_s_per_slot": rhs / (sum(asvc) * 1e-3) if asvc else None}
+           "apply_H-MVP_per_s_per_slot": rhs / (sum(asvc) * 1e-3) if asvc else None,
+           "rel_err_exact_rows_max": max(e["exact_rows"] for e in errs) if errs else None,
+           "rel_err_vs_fp64_hmvp_max": max(e["vs_fp64_hmvp"] for e in errs) if errs else None,
+           "rel_err_sampled_clouds": len(errs),
+           "rel_err_note": "every 32nd cloud: its result on 256 sampled rows per RHS against exact FP64 "
+                           "rows (includes the H-approximation error: tolerance ACA 1e-5, leaf 64, eta 1.2) "
+                           "and against the FP64 H-MVP of the cloud's H-matrix built alone"}
     log(f"[bench] batch {log_tag}: {json.dumps(res)}")
+    return res
+
+
+def batch_api_session(clouds, precision, log_tag, threads=8, exact_boxes=True):
+    """The same clouds through the batch entry points: hmg_setup_batch (every
+    cloud's setup on its own stream, `threads` host threads), then
+    hmg_precompute_aca_batch (the admissible blocks of all clouds of one dimension
+    in ONE ACA launch set), then hmg_apply_batch.  Times each stage."""
+    import hmglib_amd
+    from hmglib_amd import hip
+    from hmglib_amd.hmatrix import HMatrix, Kernel, apply_batch, precompute_aca_batch, setup_batch
+    nmax = max(c["n"] for c in clouds)
+    Xpool_h = np.random.default_rng(9).standard_normal((nmax * 8,))
+    t0 = time.perf_counter()
+    hms, kerns = [], []
+    for c in clouds:
+        hm = HMatrix(c["X"], c["X"], 32 if c["dim"] == 2 else 20, 1.2, 50, 64, 32, epsilon=1e-5)
+        hm.set_option(hmglib_amd.OPT_ACA_TOLERANCE, 2)
+        hm.set_option(hmglib_amd.OPT_EXACT_BOXES, int(exact_boxes))
+        hm.set_precision(precision)
+        hms.append(hm)
+    hip.synchronize()
+    t1 = time.perf_counter()
+    setup_batch(hms, threads)
+    hip.synchronize()
+    t2 = time.perf_counter()
+    kern = Kernel(3, 0.0, 0.1)
+    for hm in hms:
+        hm.set_kernel(kern)
+    precompute_aca_batch(hms, threads)
+    hip.synchronize()
+    t3 = time.perf_counter()
+    xds = [hip.DeviceArray(np.asfortranarray(Xpool_h[:c["n"] * c["nrhs"]])) for c in clouds]
+    yds = [hip.DeviceArray.empty(c["n"] * c["nrhs"]) for c in clouds]
+    hip.synchronize()
+    t4 = time.perf_counter()
+    apply_batch(hms, xds, yds, [c["nrhs"] for c in clouds], threads)
+    hip.synchronize()
+    t5 = time.perf_counter()
+    errs = []
+    for i in range(0, len(clouds), 32):
+        c = clouds[i]
+        n, nr = c["n"], c["nrhs"]
+        Y = yds[i].download().reshape(n, nr, order="F")
+        Xc = np.asfortranarray(Xpool_h[:n * nr].reshape(n, nr, order="F"))
+        errs.append(sample_errors(c["X"], 32 if c["dim"] == 2 else 20, 1.2, 64, 32, 3, Xc, Y, exact_boxes, 2, 1e-5))
+    for hm in hms:
+        hm.destroy()
+    kern.free()
+    for a in xds + yds:
+        a.free()
+    rhs = sum(c["nrhs"] for c in clouds)
+    wall = (t3 - t0) + (t5 - t4)
+    res = {"clouds": len(clouds), "rhs": rhs, "wall_s": wall, "clouds_per_s": len(clouds) / wall,
+           "init_s": t1 - t0, "setup_s": t2 - t1, "aca_s": t3 - t2, "apply_s": t5 - t4,
+           "rel_err_exact_rows_max": max(e["exact_rows"] for e in errs) if errs else None,
+           "rel_err_vs_fp64_hmvp_max": max(e["vs_fp64_hmvp"] for e in errs) if errs else None,
+           "rel_err_sampled_clouds": len(errs)}
+    log(f"[bench] batch api {log_tag}: {json.dumps(res)}")
     return res
 
 
@@ -401,11 +598,16 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override the point count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 leg of c3")
-    ap.add_argument("--plain-steps", type=int, default=10)
-    ap.add_argument("--cpu-stride", type=int, default=16)
+    ap.add_argument("--plain-steps", type=int, default=100,
+                    help="FP64 single-RHS products timed beside the mixed leg (the plain configuration "
+                         "times 100)")
+    ap.add_argument("--cpu-stride", type=int, default=8,
+                    help="the CPU baseline runs every n-th block of the full tree (8: ~15-25 s of CPU work "
+                         "at N=2^20 with 32 right-hand sides)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
-    ap.add_argument("--serve-qps", type=float, default=8.0)
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-qps", default="4,8,16,32",
+                    help="offered loads of the serving leg (requests/s, comma separated)")
+    ap.add_argument("--serve-seconds", type=float, default=12.0)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-trace", default="", help="prefix of per-request JSON traces of the serving leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the many-clouds batching leg")
@@ -502,7 +704,8 @@ def main():
         Xd = hip.DeviceArray(Xh)
         Yd = hip.DeviceArray.empty((n, R))
         # launch shape of the product (near-field LDS reservation, one or two
-        # streams) picked by the library's autotuner on these operands
+        # streams, low-rank products in workgroup or one-wavefront form) picked by
+        # the library's autotuner on these operands
         tuned = hm.autotune(Xd, Yd, R, reps=2)
         log(f"[bench] autotuned product: {tuned}")
         dt = timed_products(hm, Xd, Yd, R, args.steps, args.warmup, hip)
@@ -530,14 +733,30 @@ def main():
     served = None
     if mixed and not args.no_serve:
         log_free("the serving legs")
-        reqs = serving_requests(5001, 5002, args.serve_qps, args.serve_seconds)
-        served = {"qps": args.serve_qps, "seconds": args.serve_seconds, "budget_GB": 200,
+        loads = [float(q) for q in str(args.serve_qps).split(",") if q]
+        served = {"qps": loads, "seconds": args.serve_seconds, "budget_GB": 200,
                   "slots": args.serve_slots, "product_slots": 1,
                   "workload": "Poisson arrivals; N=2^(12+6u); uniform or 8-blob; 2D/3D; exp or Gaussian "
-                              "l=0.1; 1-16 RHS; 70% applies to cached H-matrices, 30% builds"}
-        for name, prec in (("fp64", hmglib_amd.PREC_FP64), ("mixed", hmglib_amd.PREC_MIXED)):
-            served[name] = serve_session(reqs, prec, 200e9, args.serve_slots, name,
-                                         f"{args.serve_trace}_{name}.json" if args.serve_trace else None)
+                              "l=0.1; 1-16 RHS; 70% applies to cached H-matrices, 30% builds",
+                  "tree": "2D leaf 64, eta 1.5, relative ACA tolerance 1e-6 capped at rank 32; 3D leaf 256, "
+                          "eta 1.0, tolerance 1e-7 capped at rank 24 (the plain configurations' parameters)",
+                  "cluster_boxes": "exact (HMG_OPT_EXACT_BOXES=1): the reference's lookup-table boxes shift "
+                                   "after one-point clusters, which 8-blob clouds produce, and then miss the "
+                                   "dense product by up to 0.6 (oracle and engine alike)",
+                  "sweep": []}
+        keys = ("throughput_rps", "p50_ms", "p99_ms", "apply_p50_ms", "apply_p99_ms", "build_p99_ms",
+                "failed", "rel_err_exact_rows_max", "rel_err_vs_fp64_hmvp_max")
+        for qps in loads:
+            reqs = serving_requests(5001, 5002, qps, args.serve_seconds)
+            point = {"qps": qps, "requests": len(reqs)}
+            for name, prec in (("fp64", hmglib_amd.PREC_FP64), ("mixed", hmglib_amd.PREC_MIXED)):
+                tag = f"{name} {qps:g} QPS"
+                r = serve_session(reqs, prec, 200e9, args.serve_slots, tag,
+                                  f"{args.serve_trace}_{name}_{qps:g}.json" if args.serve_trace else None)
+                point[name] = {k: r[k] for k in keys}
+                if qps == 8.0 or len(loads) == 1:
+                    served[name] = r
+            served["sweep"].append(point)
 
     batched = None
     if mixed and not args.no_batch:
@@ -546,15 +765,26 @@ def main():
         batched = {"workload": "512 clouds N=2^(10+6u), 2D/3D alternating, 8 Gaussian blobs sigma 0.05; "
                                "Gaussian l=0.1, leaf 64, eta 1.2, relative ACA tolerance 1e-5 (rank cap 32); "
                                "1-8 RHS; build, product and release of every cloud in flight at once",
+                   "cluster_boxes": "exact (HMG_OPT_EXACT_BOXES=1)",
                    "slots": args.batch_slots}
         for name, prec in (("fp64", hmglib_amd.PREC_FP64), ("mixed", hmglib_amd.PREC_MIXED)):
             batched[name] = batch_session(clouds, prec, args.batch_slots, name)
             log_free(f"what follows the {name} batching leg")
+        batched["batch_api"] = {"what": "hmg_setup_batch (8 host threads, a stream per cloud) + "
+                                        "hmg_precompute_aca_batch (one ACA launch set per dimension across "
+                                        "all clouds) + hmg_apply_batch; host-to-device copies of the "
+                                        "right-hand sides excluded"}
+        for name, prec in (("fp64", hmglib_amd.PREC_FP64), ("mixed", hmglib_amd.PREC_MIXED)):
+            batched["batch_api"][name] = batch_api_session(clouds, prec, name)
+            log_free(f"what follows the {name} batch-API leg")
 
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(X, x1, cfg, args.cpu_stride if n > 100000 else 1)
+            xb = x1
+            if mixed:
+                xb = np.asfortranarray(rng_normal(cfg["seed_X"], n * R).reshape(n, R, order="F"))
+            cpu = cpu_baseline(X, xb, cfg, args.cpu_stride if n > 100000 else 1)
             log(f"[bench] cpu baseline {cpu}")
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
@@ -571,7 +801,8 @@ def main():
         if plain is not None:
             config["plain_fp64"] = {"H-MVP/s": plain["value"], "ms_per_step": plain["ms_per_step"],
                                     "steps": plain["steps"], "nrhs": 1,
-                                    "rel_err_fp64_rows": plain["rel_err_fp64_rows"]}
+                                    "rel_err_fp64_rows": plain["rel_err_fp64_rows"],
+                                    "rel_err_note": RANK24_NOTE}
             config["mixed_over_plain"] = result["value"] / plain["value"]
             config["build_s"]["aca_fp64"] = plain["aca_s"]
         if served is not None:
@@ -582,7 +813,8 @@ def main():
         metric = ("H-MVP/s at N=2^20 3D, 32 RHS, mixed precision (rel.err<=1e-4 vs FP64), "
                   "exp(-r/0.1), rank-24 ACA")
     else:
-        config.update({"precision": "fp64", "rel_err_fp64_rows": plain["rel_err_fp64_rows"]})
+        config.update({"precision": "fp64", "rel_err_fp64_rows": plain["rel_err_fp64_rows"],
+                       "rel_err_note": RANK24_NOTE})
         config["build_s"]["aca_fp64"] = plain["aca_s"]
         value, ms, steps, dtype = plain["value"], plain["ms_per_step"], plain["steps"], "f64"
         metric = "H-MVP/s at N=2^20 3D (plain FP64, rank-24 ACA, exp(-r/0.1))"
@@ -602,7 +834,7 @@ def main():
         "config": config,
         "roofline": roof,
         "cpu_baseline": (None if cpu is None else
-                         {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}),
+                         {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample", "host")}),
     }
     print(json.dumps(out), flush=True)
 

@@ -26,6 +26,10 @@ OPT_ACA_ROW_SCAN = 5
 OPT_ACA_BIG_BLOCK = 6
 OPT_ACA_TOLERANCE = 7
 OPT_ACA_HUGE_BLOCK = 8
+OPT_LR_WORKGROUP = 9
+OPT_EXACT_BOXES = 10
+OPT_NEAR_WORKGROUP = 11
+OPT_FP64_BLOCK_RHS = 12
 
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -120,7 +124,7 @@ class BuildRequest(ctypes.Structure):
                 ("coords", c_double_p * 2), ("eta", ctypes.c_double), ("max_level", ctypes.c_int),
                 ("c_leaf", ctypes.c_int), ("k", ctypes.c_int), ("kernel_kind", ctypes.c_int),
                 ("regularization", ctypes.c_double), ("length_scale", ctypes.c_double),
-                ("epsilon", ctypes.c_double), ("aca_tolerance", ctypes.c_int)]
+                ("epsilon", ctypes.c_double), ("aca_tolerance", ctypes.c_int), ("exact_boxes", ctypes.c_int)]
 
 
 class RequestInfo(ctypes.Structure):

@@ -16,7 +16,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhmglib.so")
 SOURCES = ["prims.hip", "morton.hip", "tree.hip", "aca.hip", "schedule.hip", "mvp.hip", "mixed.hip",
            "api.hip", "server.hip", "store.hip", "batch.hip",
-           "noalloc.hip"]
+           "noalloc.hip", "mvp_multi.hip"]
 ARCH = os.environ.get("HMG_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -586,7 +586,8 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk,
            const work_item* __restrict__ items, const int* __restrict__ list,
            int nbig, int nsmall, const int* __restrict__ kb, const int64_t* __restrict__ uoff,
            const int* __restrict__ ldu, const int64_t* __restrict__ voff, const int* __restrict__ ldv,
-           T* __restrict__ U, T* __restrict__ V, int* __restrict__ keff, int tie_last, int scan, AcaTol tol)
+           T* __restrict__ U, T* __restrict__ V, T* const* __restrict__ ublk, T* const* __restrict__ vblk,
+           int* __restrict__ keff, int tie_last, int scan, AcaTol tol)
 {
     __shared__ double s_v[kAcaTeam * kScanRows];
     __shared__ int s_i[kAcaTeam];
@@ -600,14 +601,16 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk,
     if ((int)blockIdx.x < nbig) {
         const int a = list[blockIdx.x];
         const Team<kAcaTeam> team{s_v, s_i, s_x};
-        aca_block<T, kAcaTeam, KIND, TOL>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
+        aca_block<T, kAcaTeam, KIND, TOL>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a],
+                                          ublk ? ublk[a] : U + uoff[a], vblk ? vblk[a] : V + voff[a],
                                      keff + a, tie_last, scan, sU[w], sX[w], sBox[w], sVn[w], items[a], tol);
     } else {
         const int q = ((int)blockIdx.x - nbig) * kAcaTeam + w;
         if (q >= nsmall) return;
         const int a = list[nbig + q];
         const Team<1> team{nullptr, nullptr, nullptr};
-        aca_block<T, 1, KIND, TOL>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
+        aca_block<T, 1, KIND, TOL>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a],
+                                   ublk ? ublk[a] : U + uoff[a], vblk ? vblk[a] : V + voff[a],
                               keff + a, tie_last, scan, sU[w], sX[w], sBox[w], sVn[w], items[a], tol);
     }
 }
@@ -623,7 +626,8 @@ aca_huge_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ 
                 const work_item* __restrict__ items, const int* __restrict__ list,
                 const int* __restrict__ kb, const int64_t* __restrict__ uoff, const int* __restrict__ ldu,
                 const int64_t* __restrict__ voff, const int* __restrict__ ldv, T* __restrict__ U,
-                T* __restrict__ V, int* __restrict__ keff, int tie_last, int scan, AcaTol tol)
+                T* __restrict__ V, T* const* __restrict__ ublk, T* const* __restrict__ vblk,
+                int* __restrict__ keff, int tie_last, int scan, AcaTol tol)
 {
     __shared__ double s_v[kAcaHuge * 2 * kScanRows];
     __shared__ int s_i[kAcaHuge];
@@ -635,7 +639,8 @@ aca_huge_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ 
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int a = list[blockIdx.x];
     const Team<kAcaHuge> team{s_v, s_i, s_x};
-    aca_block<T, kAcaHuge, KIND, TOL>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a], U + uoff[a], V + voff[a],
+    aca_block<T, kAcaHuge, KIND, TOL>(team, c1, c2, kp, blk[a], kb[a], ldv[a], ldu[a],
+                                      ublk ? ublk[a] : U + uoff[a], vblk ? vblk[a] : V + voff[a],
                                       keff + a, tie_last, scan, sU[w], sX[w], sBox[w], sVn[w], items[a], tol);
 }
 
@@ -656,7 +661,8 @@ static int aca_class(const Engine& eng, int a)
 template <class T>
 static hmg_status aca_launch(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
                              int tie_last, const int* list, int nhuge, int nbig, int nsmall, const int64_t* uoff,
-                             const int64_t* voff, T* U, T* V, int* keff)
+                             const int64_t* voff, T* U, T* V, int* keff, T* const* ublk = nullptr,
+                             T* const* vblk = nullptr)
 {
     const int scan = eng.aca_row_scan ? 1 : 0;
     const int4* blk = eng.blk.p + eng.ndense;
@@ -671,11 +677,13 @@ static hmg_status aca_launch(Engine& eng, const Coords& c1, const Coords& c2, co
         if (tol.mode != 0) {
             HMG_DISPATCH_KIND(kp.kind, aca_huge_kernel<T, KIND, true><<<nhuge, kAcaHuge * kWave, 0, s2>>>(
                                            c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, eng.kb.p, uoff,
-                                           eng.ldu.p, voff, eng.ldv.p, U, V, keff, tie_last, scan, tol));
+                                           eng.ldu.p, voff, eng.ldv.p, U, V, ublk, vblk, keff, tie_last,
+                                           scan, tol));
         } else {
             HMG_DISPATCH_KIND(kp.kind, aca_huge_kernel<T, KIND, false><<<nhuge, kAcaHuge * kWave, 0, s2>>>(
                                            c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, eng.kb.p, uoff,
-                                           eng.ldu.p, voff, eng.ldv.p, U, V, keff, tie_last, scan, tol));
+                                           eng.ldu.p, voff, eng.ldv.p, U, V, ublk, vblk, keff, tie_last,
+                                           scan, tol));
         }
         HMG_HIP(hipGetLastError());
         if (fork) HMG_HIP(hipEventRecord(eng.ev_join, s2));
@@ -689,12 +697,12 @@ static hmg_status aca_launch(Engine& eng, const Coords& c1, const Coords& c2, co
     if (tol.mode != 0) {
         HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND, true><<<grid, kAcaTeam * kWave, 0, s>>>(
                                        c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, nbig, nsmall,
-                                       eng.kb.p, uoff, eng.ldu.p, voff, eng.ldv.p, U, V, keff, tie_last, scan,
+                                       eng.kb.p, uoff, eng.ldu.p, voff, eng.ldv.p, U, V, ublk, vblk, keff, tie_last, scan,
                                        tol));
     } else {
         HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND, false><<<grid, kAcaTeam * kWave, 0, s>>>(
                                        c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, nbig, nsmall,
-                                       eng.kb.p, uoff, eng.ldu.p, voff, eng.ldv.p, U, V, keff, tie_last, scan,
+                                       eng.kb.p, uoff, eng.ldu.p, voff, eng.ldv.p, U, V, ublk, vblk, keff, tie_last, scan,
                                        tol));
     }
     HMG_HIP(hipGetLastError());
@@ -703,7 +711,7 @@ static hmg_status aca_launch(Engine& eng, const Coords& c1, const Coords& c2, co
 }
 
 hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp, int k,
-                     int tie_last, double* U, double* V, int* keff)
+                     int tie_last, double* U, double* V, int* keff, double* const* ublk, double* const* vblk)
 {
     (void)k;
     const int na = eng.naca;
@@ -724,7 +732,7 @@ hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const Kern
     HMG_HIP(hipMemcpyAsync(eng.aca_list.p, order.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
     eng.prof.begin(PROF_ACA, s);
     HMG_TRY(aca_launch<double>(eng, c1, c2, kp, tie_last, eng.aca_list.p, cnt[0], cnt[1], cnt[2], eng.uoff.p,
-                               eng.voff.p, U, V, keff));
+                               eng.voff.p, U, V, keff, ublk, vblk));
     eng.prof.end(PROF_ACA, s);
     return HMG_OK;
 }

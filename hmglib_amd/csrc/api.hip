@@ -215,8 +215,11 @@ void release_build(Engine& e, h_matrix_data* d)
     e.P.release();
     e.xs.release();
     e.T.release();
+    for (DevBuf<double>* b : {&e.X64, &e.Tp64, &e.T64, &e.P64, &e.Ys64}) b->release();
     e.n_near_items = e.n_lr_items = 0;
     e.mx_items.release();
+    e.mx_wg.release();
+    e.h_mx_wg.clear();
     e.mx_refs.release();
     e.h_mx_items.clear();
     e.h_mx_refs.clear();
@@ -512,7 +515,7 @@ hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
     }
     // block-cluster tree (tree.cu:1429-1557) and DENSE-first partition
     TreeParams tp{data->eta, data->max_level, data->c_leaf, (int)data->root_level_set_1,
-                  (int)data->root_level_set_2};
+                  (int)data->root_level_set_2, e->exact_boxes};
     HMG_TRY(build_tree(*e, coords_of(data, 0), coords_of(data, 1), data->code_d[0], data->code_d[1], tp));
     data->mat_vec_data = (work_item**)malloc(sizeof(work_item*));
     *data->mat_vec_data = e->blocks.p;
@@ -660,8 +663,12 @@ hmg_status hmg_apply_h_matrix_mvp_multi(const double* X, double* Y, int nrhs, st
         U = tU.p;
         V = tV.p;
     }
-    HMG_TRY(hmvp(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[0], data->order[1], X, Y,
-                 nrhs, U, V, e->keff.p, pre_dense));
+    if (nrhs >= e->fp64_block_min && !pre_dense)
+        HMG_TRY(hmvp_multi(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[0], data->order[1], X, Y,
+                           nrhs, U, V));
+    else
+        HMG_TRY(hmvp(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[0], data->order[1], X, Y,
+                     nrhs, U, V, e->keff.p, pre_dense));
     HMG_HIP(hipStreamSynchronize(e->stream));
     tU.release();
     tV.release();
@@ -671,12 +678,13 @@ hmg_status hmg_apply_h_matrix_mvp_multi(const double* X, double* Y, int nrhs, st
 
 // Launch shapes of the product timed on the caller's operands (Y is overwritten);
 // the fastest is kept in the engine's options.  Candidates: the near field's LDS
-// reservation (0, 32, 48, 64, 80 KB: workgroups per CU beside the low-rank
-// kernels) with the two streams overlapped, and both parts on one stream.  The
-// product's values do not depend on the choice (every partial is summed in a
-// fixed order).
+// reservation (0 to 80 KB: workgroups per CU beside the low-rank kernels) with
+// the two streams overlapped, and both parts on one stream; with mixed-precision
+// factors each of these with the low-rank products and the near field each in
+// workgroup and in one-wavefront form.  The product's values do not depend on the choice
+// (every partial is summed in a fixed order).
 hmg_status hmg_autotune_product(struct h_matrix_data* data, const double* X, double* Y, int nrhs, int reps,
-                                int64_t out[3])
+                                int64_t out[4])
 {
     Engine* e;
     HMG_TRY(check_ready(data, &e));
@@ -691,21 +699,35 @@ hmg_status hmg_autotune_product(struct h_matrix_data* data, const double* X, dou
     struct Cand {
         int lds;
         bool serial;
+        bool wg;       // low-rank products in workgroup form
+        bool near_wg;  // near field in workgroup form
     };
-    const Cand cands[] = {{0, false}, {32 << 10, false}, {48 << 10, false}, {64 << 10, false}, {80 << 10, false},
-                          {0, true}};
-    const Cand keep{e->near_lds_pad, e->serialize};
+    const bool mixed = e->fact_prec == HMG_PREC_MIXED;
+    std::vector<Cand> cands;
+    for (const bool wg : {true, false}) {
+        for (const bool nwg : {true, false}) {
+            if (!mixed && !(wg && nwg)) continue;  // the FP64 product has one form of each
+            for (const int lds : {0, 16 << 10, 24 << 10, 36 << 10, 48 << 10, 64 << 10, 80 << 10})
+                cands.push_back(Cand{lds, false, wg, nwg});
+            cands.push_back(Cand{0, true, wg, nwg});
+        }
+    }
+    const Cand keep{e->near_lds_pad, e->serialize, e->lr_wg, e->near_wg};
+    auto set = [&](const Cand& c) {
+        e->near_lds_pad = c.lds;
+        e->serialize = c.serial;
+        e->lr_wg = c.wg;
+        e->near_wg = c.near_wg;
+    };
     Cand best_c = keep;
     double best = 1e300;
     for (const Cand& c : cands) {
-        e->near_lds_pad = c.lds;
-        e->serialize = c.serial;
+        set(c);
         hmg_status st = hmg_apply_h_matrix_mvp_multi(X, Y, nrhs, data);  // warm-up
         const auto t0 = std::chrono::steady_clock::now();
         for (int r = 0; st == HMG_OK && r < reps; r++) st = hmg_apply_h_matrix_mvp_multi(X, Y, nrhs, data);
         if (st != HMG_OK) {
-            e->near_lds_pad = keep.lds;
-            e->serialize = keep.serial;
+            set(keep);
             return st;
         }
         const double us =
@@ -715,11 +737,11 @@ hmg_status hmg_autotune_product(struct h_matrix_data* data, const double* X, dou
             best_c = c;
         }
     }
-    e->near_lds_pad = best_c.lds;
-    e->serialize = best_c.serial;
+    set(best_c);
     out[0] = best_c.lds;
     out[1] = best_c.serial ? 1 : 0;
     out[2] = (int64_t)best;
+    out[3] = (best_c.wg ? 1 : 0) | (best_c.near_wg ? 2 : 0);
     return HMG_OK;
 }
 
@@ -906,6 +928,8 @@ int64_t engine_bytes(const Engine& e)
     t += bytes_of(e.Tpan) + bytes_of(e.Tp32) + bytes_of(e.P32) + bytes_of(e.Xp) + bytes_of(e.Xt32) + bytes_of(e.mxp);
     t += bytes_of(e.Ys32) + bytes_of(e.inv1);
     t += bytes_of(e.cpan[0]) + bytes_of(e.cpan[1]);
+    t += bytes_of(e.X64) + bytes_of(e.Tp64) + bytes_of(e.T64) + bytes_of(e.P64) + bytes_of(e.Ys64);
+    t += bytes_of(e.vw_items) + bytes_of(e.uw_items) + bytes_of(e.mx_wg);
     for (const Combine* c : {&e.comb_stored, &e.comb_fly, &e.comb_mx})
         t += bytes_of(c->ptr) + bytes_of(c->idx) + bytes_of(c->groups);
     return t;
@@ -973,6 +997,13 @@ hmg_status hmg_memory(const struct h_matrix_data* data, int64_t out[2])
         add += n1 * kMxRhs * 4 + n1 * 4;                             // Ys32, inv1
         add += (g1 + g2) * e->dim * 8 * 4 + mx_items * (16 + 16);    // panels, items
     }
+    if (!mixed && e->X64.n == 0) {
+        // FP64 products of up to 32 right-hand sides per pass: sorted right-hand-side
+        // rows, V^T X chunk partials, T, row partials, sorted results, inverse order
+        const int64_t slots = (int64_t)e->n_lr_items * 64 + (int64_t)e->n_near_items * 128;
+        add += ((int64_t)e->n[0] + e->n[1] + e->tsize + slots + (int64_t)e->n_lv_items * 64) * kRbMax * 8 +
+               (int64_t)e->n[0] * 4;
+    }
     out[1] = add + add / 10 + (64 << 20);
     return HMG_OK;
 }
@@ -1028,6 +1059,22 @@ hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value)
             return HMG_ERR_INVALID;
         }
         e->aca_huge_entries = (int)value;
+        return HMG_OK;
+    case HMG_OPT_LR_WORKGROUP:
+        e->lr_wg = value != 0;
+        return HMG_OK;
+    case HMG_OPT_EXACT_BOXES:
+        e->exact_boxes = value != 0;
+        return HMG_OK;
+    case HMG_OPT_NEAR_WORKGROUP:
+        e->near_wg = value != 0;
+        return HMG_OK;
+    case HMG_OPT_FP64_BLOCK_RHS:
+        if (value < 1 || value > (int64_t)1 << 30) {
+            set_error("FP64 block threshold must be at least 1");
+            return HMG_ERR_INVALID;
+        }
+        e->fp64_block_min = (int)value;
         return HMG_OK;
     default:
         set_error("unknown option " + std::to_string(option));
@@ -1247,7 +1294,7 @@ int64_t hmg_struct_layout(int which, int field)
                                 OFF(hmg_build_request, c_leaf), OFF(hmg_build_request, k),
                                 OFF(hmg_build_request, kernel_kind), OFF(hmg_build_request, regularization),
                                 OFF(hmg_build_request, length_scale), OFF(hmg_build_request, epsilon),
-                                OFF(hmg_build_request, aca_tolerance)};
+                                OFF(hmg_build_request, aca_tolerance), OFF(hmg_build_request, exact_boxes)};
         if (field < 0) return (int64_t)sizeof(hmg_build_request);
         if (field < (int)(sizeof(offs) / sizeof(offs[0]))) return offs[field];
         return -1;

@@ -11,13 +11,14 @@
 //     so sorted indices coincide across the two sets exactly where they do inside
 //     the cloud (the regularization term stays where it belongs);
 //   - their ACA blocks, row and column ranges moved by o_c, their ranks, and
-//     their factor layout placed in one FP64 buffer (FP64: each H-matrix's own
-//     layout, moved; mixed precision: the scratch layout of the mixed ACA).
+//     where their factors go (FP64: a pointer per block into its H-matrix's own
+//     factor layout, written in place; mixed precision: the scratch layout of the
+//     mixed ACA in one FP64 buffer of at most kBatchScratch elements).
 // The single-H-matrix ACA kernels then run all of these blocks in one launch set
 // (the largest on 8-wavefront teams beside the rest), so large blocks of one
-// cloud run beside small blocks of the others.  Each H-matrix receives its ranks
-// and its factors from its slice: copied into its FP64 layout, or split into its
-// BF16 hi + lo panels.  Ranks, pivots and factors are those of precompute_aca on
+// cloud run beside small blocks of the others.  Each H-matrix receives its ranks,
+// and in mixed precision its factors split from its slice into its BF16 hi + lo
+// panels.  Ranks, pivots and factors are those of precompute_aca on
 // each H-matrix alone (same kernels, same per-block arithmetic).
 #include <algorithm>
 #include <atomic>
@@ -86,7 +87,7 @@ hmg_status run_group(h_matrix_data* const* data, const std::vector<int>& members
     const bool mixed = e0->precision == HMG_PREC_MIXED;
     const int dim = kp.dim;
     // offsets of the clouds' points, blocks and factors
-    std::vector<int64_t> pofs(nm + 1, 0), aofs(nm + 1, 0), fofs(nm + 1, 0), vofs(nm + 1, 0);
+    std::vector<int64_t> pofs(nm + 1, 0), aofs(nm + 1, 0), fofs(nm + 1, 0);
     for (int m = 0; m < nm; m++) {
         const Engine* e = engine_of(data[members[m]]);
         pofs[m + 1] = pofs[m] + std::max(e->n[0], e->n[1]);
@@ -95,9 +96,6 @@ hmg_status run_group(h_matrix_data* const* data, const std::vector<int>& members
             int64_t t = 0;
             for (int a = 0; a < e->naca; a++) t += scratch_of(*e, a);
             fofs[m + 1] = fofs[m] + t;
-        } else {
-            fofs[m + 1] = fofs[m] + e->h_uoff[e->naca] + 256;
-            vofs[m + 1] = vofs[m] + e->h_voff[e->naca] + 256;
         }
     }
     const int64_t npts = pofs[nm], na = aofs[nm];
@@ -137,7 +135,9 @@ hmg_status run_group(h_matrix_data* const* data, const std::vector<int>& members
     }
     v.h_blk.resize((size_t)na);
     std::vector<int> kb((size_t)na), ldu((size_t)na), ldv((size_t)na);
-    std::vector<int64_t> uoff((size_t)na), voff((size_t)na);
+    std::vector<int64_t> uoff((size_t)na, 0), voff((size_t)na, 0);
+    // FP64: each block's factors are written in place, in its H-matrix's own layout
+    std::vector<double*> ublk(mixed ? 0 : (size_t)na), vblk(mixed ? 0 : (size_t)na);
     HMG_TRY(v.blocks.alloc((size_t)na));
     for (int m = 0; m < nm; m++) {
         const h_matrix_data* d = data[members[m]];
@@ -166,8 +166,8 @@ hmg_status run_group(h_matrix_data* const* data, const std::vector<int>& members
                 ldv[q] = std::max(kb[q], 1);
                 run += scratch_of(*e, a);
             } else {
-                uoff[q] = fofs[m] + e->h_uoff[a];
-                voff[q] = vofs[m] + e->h_voff[a];
+                ublk[q] = e->U.p + e->h_uoff[a];
+                vblk[q] = e->V.p + e->h_voff[a];
                 ldu[q] = e->h_ldu[a];
                 ldv[q] = e->h_ldv[a];
             }
@@ -182,13 +182,20 @@ hmg_status run_group(h_matrix_data* const* data, const std::vector<int>& members
     HMG_TRY(upload(v.voff, voff, s));
     HMG_TRY(v.keff.alloc((size_t)na));
     HMG_HIP(hipMemsetAsync(v.keff.p, 0, sizeof(int) * na, s));
-    DevBuf<double> U, V;
-    HMG_TRY(U.alloc((size_t)std::max<int64_t>(fofs[nm], 1)));
-    if (!mixed) HMG_TRY(V.alloc((size_t)std::max<int64_t>(vofs[nm], 1)));
-    HMG_HIP(hipMemsetAsync(U.p, 0, sizeof(double) * U.n, s));
-    if (!mixed) HMG_HIP(hipMemsetAsync(V.p, 0, sizeof(double) * V.n, s));
-    // the ACA of every block of the group (last argmax: the batched rule)
-    HMG_TRY(aca_build(v, c[0], c[1], kp, 0, 1, U.p, mixed ? U.p : V.p, v.keff.p));
+    // the ACA of every block of the group (last argmax: the batched rule); mixed
+    // precision: into an FP64 scratch (bounded by kBatchScratch), then split into
+    // each H-matrix's panels
+    DevBuf<double> U;
+    DevBuf<double*> d_ublk, d_vblk;
+    if (mixed) {
+        HMG_TRY(U.alloc((size_t)std::max<int64_t>(fofs[nm], 1)));
+        HMG_HIP(hipMemsetAsync(U.p, 0, sizeof(double) * U.n, s));
+        HMG_TRY(aca_build(v, c[0], c[1], kp, 0, 1, U.p, U.p, v.keff.p));
+    } else {
+        HMG_TRY(upload(d_ublk, ublk, s));
+        HMG_TRY(upload(d_vblk, vblk, s));
+        HMG_TRY(aca_build(v, c[0], c[1], kp, 0, 1, nullptr, nullptr, v.keff.p, d_ublk.p, d_vblk.p));
+    }
     // ranks and factors to their H-matrices
     DevBuf<int> iota;
     if (mixed) {
@@ -202,15 +209,9 @@ hmg_status run_group(h_matrix_data* const* data, const std::vector<int>& members
         Engine* e = engine_of(data[members[m]]);
         if (e->naca == 0) continue;
         HMG_HIP(hipMemcpyAsync(e->keff.p, v.keff.p + aofs[m], sizeof(int) * e->naca, hipMemcpyDeviceToDevice, s));
-        if (mixed) {
+        if (mixed)
             HMG_TRY(aca_to_panels(*e, iota.p, e->naca, v.uoff.p + aofs[m], v.voff.p + aofs[m], U.p, e->Up.p,
                                   e->Vp.p, s));
-        } else {
-            HMG_HIP(hipMemcpyAsync(e->U.p, U.p + fofs[m], sizeof(double) * (fofs[m + 1] - fofs[m]),
-                                   hipMemcpyDeviceToDevice, s));
-            HMG_HIP(hipMemcpyAsync(e->V.p, V.p + vofs[m], sizeof(double) * (vofs[m + 1] - vofs[m]),
-                                   hipMemcpyDeviceToDevice, s));
-        }
     }
     HMG_HIP(hipStreamSynchronize(s));
     return HMG_OK;

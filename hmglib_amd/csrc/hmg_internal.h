@@ -176,6 +176,32 @@ struct LvItem {
     int tcol0;
     int tp;
 };
+// One wavefront's share of a low-rank workgroup item (mixed precision): W
+// wavefronts of a workgroup multiply their own A panel columns / rows by the
+// same B panel groups (the same point groups of X for V^T X, the same T slot
+// groups for U T), which the workgroup stages once through LDS.
+//   aoff: A panel element of the wavefront's first M row in B group b0 (hi half)
+//   ld:   A panel leading dimension (Kp of a V panel, m1p of a U panel)
+//   m:    valid M (columns of V^T X, rows of U T); 0 for an idle wavefront
+//   b0, nb: first B panel group and B groups (a multiple of 4; the same for all
+//           wavefronts of the workgroup)
+//   o0:   V^T X: first tmap column of the slice; U T: its partial-row slot item
+//   tp:   V^T X: partial slot (-1: the result goes to the T panels directly)
+struct WgItem {
+    int64_t aoff;
+    int ld;
+    int m;
+    int b0;
+    int nb;
+    int o0;
+    int tp;
+};
+// wavefronts per workgroup of the V^T X and U T workgroup kernels and of the
+// near field's workgroup form
+constexpr int kVwWaves = 4;
+constexpr int kUwWaves = 2;
+constexpr int kNwWaves = 2;
+
 // reduction of the row chunks of one column slice: partial slots
 // item0 .. item0+nchunks-1, slice columns map to T through tmap[tcol0 + l]
 struct LvSlice {
@@ -331,6 +357,14 @@ struct Engine {
     // groups and the end)
     int lv_nt_off[4] = {0, 0, 0, 0};
     int lr_nt_off[4] = {0, 0, 0, 0};
+    // the same items as workgroups that share their B operand (kVwWaves / kUwWaves
+    // consecutive entries per workgroup); lr_wg selects them
+    std::vector<WgItem> h_vw_items, h_uw_items;
+    DevBuf<WgItem> vw_items, uw_items;
+    bool lr_wg = false;
+    // cluster boxes of the tree: false = the reference's lookup table (a one-point
+    // cluster shifts the level's boxes), true = each cluster's exact min/max
+    bool exact_boxes = false;
     DevBuf<int4> near_items;                    // stored near field: {row0, nrows, b0, b1}
     DevBuf<int> near_blk;
     std::vector<int4> h_near_items;
@@ -354,6 +388,11 @@ struct Engine {
     Combine comb_stored, comb_fly;              // stored near field / matrix-free
     DevBuf<double> P, xs, T;
     int64_t p_slots = 0;                        // FP64 partial-sum slots of P
+    // FP64 products of blocks of right-hand sides (mvp_multi.hip): the pass's sorted
+    // right-hand-side rows, V^T X chunk partials, T, row partials, sorted results;
+    // products with at least fp64_block_min right-hand sides take this path
+    DevBuf<double> X64, Tp64, T64, P64, Ys64;
+    int fp64_block_min = 2;
 
     hipStream_t stream2 = nullptr;              // near field runs beside the low-rank part
     // tuning: LDS bytes reserved per near-field workgroup (caps its occupancy so
@@ -397,6 +436,11 @@ struct Engine {
     DevBuf<MxItem> mx_items;
     DevBuf<MxRef> mx_refs;
     int n_mx_items = 0;
+    // workgroup form of the near field: kNwWaves item indices per workgroup (the row
+    // slices of one column chunk, -1 = idle); near_wg selects it
+    std::vector<int> h_mx_wg;
+    DevBuf<int> mx_wg;
+    bool near_wg = false;
     int64_t mx_evals = 0;                       // kernel entries evaluated per pass (incl. masked)
     bool mx_sched = false;                      // near items built for the current tree
     Combine comb_mx;
@@ -409,6 +453,7 @@ struct Engine {
     DevBuf<float> Xt32;                         // a pass's right-hand sides as 32-wide FP32 rows
     DevBuf<float> cpan[2];
     bool cpan_ready = false;
+    float cpan_scale = 0.f;                     // the scale the coordinate panels were built with
 };
 
 Engine* engine_of(const h_matrix_data* d);
@@ -445,6 +490,7 @@ struct TreeParams {
     int max_level;
     int c_leaf;
     int root_level_1, root_level_2;
+    bool exact_boxes;  // cluster boxes from the points themselves (no lookup table)
 };
 // Builds the DENSE-first block list into eng.blocks / eng.blk / eng.h_blk.
 hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uint64_t* codes1,
@@ -487,8 +533,12 @@ void set_tolerance(Engine& e, const h_matrix_data* d);
 // ---- ACA (aca.hip) -------------------------------------------------------------
 // tie_last: 1 = last argmax (batched path), 0 = first argmax (unbatched path).
 // FP64 factors straight into the FP64 layout.
+// ublk / vblk (optional, device arrays of naca pointers): block a's factors start
+// at ublk[a] / vblk[a] instead of U + uoff[a] / V + voff[a] (factors that live in
+// other H-matrices' buffers).
 hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
-                     int k, int tie_last, double* U, double* V, int* keff);
+                     int k, int tie_last, double* U, double* V, int* keff, double* const* ublk = nullptr,
+                     double* const* vblk = nullptr);
 // Mixed precision: the same FP64 ACA, batch by batch through a bounded FP64
 // scratch, each batch's factors split into the BF16 hi + lo panels Up / Vp
 // (zeroed by the caller).
@@ -518,6 +568,16 @@ hmg_status aca_layout(Engine& eng, int k, int prec = HMG_PREC_FP64);
 hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
                       const uint64_t* order1, const uint64_t* order2, const double* X, double* Y,
                       int nrhs);
+
+// ---- FP64 products of blocks of right-hand sides (mvp_multi.hip) -------------------
+// Y = H X for nrhs right-hand sides with the FP64 factors: up to 32 right-hand
+// sides per pass, each factor element read once per pass, the near field
+// evaluated on the fly once per entry and pass.  A column's result does not depend
+// on the other columns of its pass.
+constexpr int kRbMax = 32;
+hmg_status hmvp_multi(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
+                      const uint64_t* order1, const uint64_t* order2, const double* X, double* Y, int nrhs,
+                      const double* U, const double* V);
 
 // ---- near field / MVP (mvp.hip) ----------------------------------------------------
 hmg_status dense_fill(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp);

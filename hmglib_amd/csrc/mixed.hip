@@ -29,6 +29,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;
@@ -39,13 +40,17 @@ inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b);
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
-// kernel parameters in the form the FP32 evaluation uses: k = 2^(e0 * s) with
-// s = r^2 (Gaussian kinds) or s = r (exponential); Matérn stays FP64
+// kernel parameters in the form the FP32 evaluation uses.  The coordinate panels
+// hold the points scaled by cscale, chosen so that the kernel is 2^(-s) with
+// s = r'^2 (Gaussian kinds: cscale = sqrt(c0 log2 e)) or s = r' (exponential:
+// cscale = c0 log2 e), r' the distance of the scaled points: no multiply per
+// entry, and the negation is an operand modifier of v_exp_f32.  Matérn stays FP64
+// on unscaled points.
 struct KernelParamsF {
     int kind;
     int dim;
     float reg;
-    float e0;
+    float cscale;
     double c1;
 };
 
@@ -57,19 +62,22 @@ KernelParamsF make_params_f(const KernelParams& kp)
     f.dim = kp.dim;
     f.reg = (float)kp.reg;
     f.c1 = kp.c1;
-    if (kp.kind == HMG_KERNEL_GAUSSIAN || kp.kind == HMG_KERNEL_EXPONENTIAL)
-        f.e0 = (float)(-kp.c0 * log2e);
+    if (kp.kind == HMG_KERNEL_GAUSSIAN)
+        f.cscale = (float)std::sqrt(kp.c0 * log2e);
+    else if (kp.kind == HMG_KERNEL_EXPONENTIAL)
+        f.cscale = (float)(kp.c0 * log2e);
+    else if (kp.kind == HMG_KERNEL_GAUSSIAN_REF)
+        f.cscale = (float)std::sqrt(log2e);  // reference Gaussian exp(-r^2)
     else
-        f.e0 = (float)(-log2e);  // reference Gaussian exp(-r^2)
+        f.cscale = 1.0f;
     return f;
 }
 
 template <int KIND>
 __device__ __forceinline__ float kval_f32(const KernelParamsF& kp, float r2)
 {
-    if (KIND == HMG_KERNEL_GAUSSIAN || KIND == HMG_KERNEL_GAUSSIAN_REF)
-        return __builtin_amdgcn_exp2f(r2 * kp.e0);
-    if (KIND == HMG_KERNEL_EXPONENTIAL) return __builtin_amdgcn_exp2f(__builtin_amdgcn_sqrtf(r2) * kp.e0);
+    if (KIND == HMG_KERNEL_GAUSSIAN || KIND == HMG_KERNEL_GAUSSIAN_REF) return __builtin_amdgcn_exp2f(-r2);
+    if (KIND == HMG_KERNEL_EXPONENTIAL) return __builtin_amdgcn_exp2f(-__builtin_amdgcn_sqrtf(r2));
     const double r = sqrt((double)r2);
     return (float)((yn(1, r + 1.0e-15) * r) / kp.c1);
 }
@@ -119,15 +127,16 @@ prep_x_kernel(const float* __restrict__ Xt, const uint64_t* __restrict__ order2,
     Xp[gbase + kMxRhs * kGroup] = l;
 }
 
-// coordinate panels: cp[(g*dim + d)*8 + e] = x_d[8g + e] as FP32 (0 past n)
+// coordinate panels: cp[(g*dim + d)*8 + e] = scale * x_d[8g + e] as FP32 (0 past n)
 __global__ void __launch_bounds__(kThreads)
-coord_panel_kernel(Coords c, int dim, int ngroups, float* __restrict__ cp)
+coord_panel_kernel(Coords c, int dim, int ngroups, double scale, float* __restrict__ cp)
 {
     const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (t >= (int64_t)ngroups * kGroup) return;
     const int p = (int)t;
     const int g = p / kGroup, e = p % kGroup;
-    for (int d = 0; d < dim; d++) cp[((int64_t)g * dim + d) * kGroup + e] = p < c.n ? (float)c.x[d][p] : 0.f;
+    for (int d = 0; d < dim; d++)
+        cp[((int64_t)g * dim + d) * kGroup + e] = p < c.n ? (float)(c.x[d][p] * scale) : 0.f;
 }
 
 // ---- near field ----------------------------------------------------------------------
@@ -152,8 +161,16 @@ __device__ __forceinline__ f32x16 near_tile(const KernelParamsF& kp, int dim, co
         if (REG && row == col0 + e) v += kp.reg;
         kv[e] = v;
     }
+    // hi = kv rounded to BF16 (packed pairs); its FP32 value is read back from the
+    // packed words (a shift or a mask per value), lo = kv - hi rounded to BF16
     const bf16x8 ah = __builtin_convertvector(kv, bf16x8);
-    const f32x8 hf = __builtin_convertvector(ah, f32x8);
+    const u32x4 hw = __builtin_bit_cast(u32x4, ah);
+    f32x8 hf;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        hf[2 * p] = __builtin_bit_cast(float, hw[p] << 16);
+        hf[2 * p + 1] = __builtin_bit_cast(float, hw[p] & 0xffff0000u);
+    }
     const bf16x8 al = __builtin_convertvector(kv - hf, bf16x8);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
@@ -227,6 +244,117 @@ near_mx_kernel(const float* __restrict__ cp1, const float* __restrict__ cp2, Ker
         }
     }
     float* __restrict__ out = P + (int64_t)q * kMxRowsItem * kMxRhs + c32;
+#pragma unroll
+    for (int t = 0; t < kMxRowTiles; t++) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int rr = t * 32 + (r >> 2) * 8 + h * 4 + (r & 3);
+            if (rr < it.nrows) out[(int64_t)rr * kMxRhs] = acc[t][r];
+        }
+    }
+}
+
+// Workgroup form of the near field: kNwWaves wavefronts run the row slices of one
+// column chunk (the same refs, so the same column groups in the same order); the
+// workgroup stages each step's two X panel groups (2 KB: one 16-byte
+// global_load_lds per lane) through LDS, double-buffered, so each group is read
+// once per workgroup instead of once per row slice.  Per lane the arithmetic is
+// that of near_mx_kernel, in the same order (bitwise the same partials).
+// wg[kNwWaves * blockIdx.x + w] = wavefront w's item (-1: idle, it only helps stage).
+constexpr int kNwStage = 2 * 2 * kMxRhs;  // bf16x8 entries of one step (two groups, hi + lo)
+
+template <int KIND, int DIM, bool REG>
+__global__ void __launch_bounds__(kNwWaves * kWave)
+near_mx_wg_kernel(const float* __restrict__ cp1, const float* __restrict__ cp2, KernelParamsF kp,
+                  const MxItem* __restrict__ items, const int* __restrict__ wg, const MxRef* __restrict__ refs,
+                  const bf16x8* __restrict__ Xp, float* __restrict__ P)
+{
+    __shared__ bf16x8 sX[2][kNwStage];
+    constexpr int DM = DIM > 0 ? DIM : MAX_POINT_DIMENSION;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int w = wave_id();
+    const int qlead = wg[(int64_t)blockIdx.x * kNwWaves];
+    const int qme = wg[(int64_t)blockIdx.x * kNwWaves + w];
+    const bool active = qme >= 0;
+    const MxItem lead = items[qlead];
+    const MxItem it = items[active ? qme : qlead];
+    const int dim = DIM > 0 ? DIM : kp.dim;
+    const int c32 = lane & 31, h = lane >> 5;
+    const int ntile = active ? (it.nrows + 31) >> 5 : 0;  // wave-uniform
+    float xr[kMxRowTiles][DM];
+    int row[kMxRowTiles];
+#pragma unroll
+    for (int t = 0; t < kMxRowTiles; t++) {
+        const int rr = t * 32 + c32 < it.nrows ? t * 32 + c32 : 0;  // rows past the item: its first row
+        row[t] = it.row0 + rr;
+#pragma unroll
+        for (int d = 0; d < DM; d++)
+            xr[t][d] = (DIM > 0 || d < dim) ? cp1[((int64_t)(row[t] >> 3) * dim + d) * kGroup + (row[t] & 7)] : 0.f;
+    }
+    f32x16 acc[kMxRowTiles];
+#pragma unroll
+    for (int t = 0; t < kMxRowTiles; t++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[t][r] = 0.f;
+    // steps: (ref rf, first group g) over the chunk's refs, two groups at a time
+    int rf = lead.ref0;
+    if (rf >= lead.ref1) return;  // uniform over the workgroup: no barrier is skipped
+    int g = refs[rf].g0;
+    auto stage = [&](int buf, int gs) {
+        __builtin_amdgcn_global_load_lds((const void*)(Xp + (int64_t)gs * (2 * kMxRhs) + tid),
+                                         (__attribute__((address_space(3))) void*)(sX[buf] + w * kWave), 16, 0,
+                                         0);
+    };
+    stage(0, g);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    int cur = 0;
+    for (;;) {
+        // the next step
+        int rfn = rf, gn = g + 2;
+        if (gn >= refs[rf].g1) {
+            rfn = rf + 1;
+            gn = rfn < lead.ref1 ? refs[rfn].g0 : 0;
+        }
+        const bool more = rfn < lead.ref1;
+        if (more) stage(cur ^ 1, gn);
+        if (ntile > 0) {
+            const MxRef f = refs[rf];
+            const int gg = g + h;
+            bf16x8 bh = sX[cur][h * 2 * kMxRhs + c32];
+            bf16x8 bl = sX[cur][h * 2 * kMxRhs + kMxRhs + c32];
+            const int col0 = gg * kGroup;
+            if (col0 < f.lo || col0 + kGroup > f.hi) {
+#pragma unroll
+                for (int e = 0; e < kGroup; e++) {
+                    if (col0 + e < f.lo || col0 + e >= f.hi) {
+                        bh[e] = (__bf16)0.f;
+                        bl[e] = (__bf16)0.f;
+                    }
+                }
+            }
+            f32x8 xc[DM];
+#pragma unroll
+            for (int d = 0; d < DM; d++) {
+                if (DIM > 0 || d < dim)
+                    xc[d] = *(const f32x8*)(cp2 + ((int64_t)gg * dim + d) * kGroup);
+                else
+                    xc[d] = f32x8{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            }
+#pragma unroll
+            for (int t = 0; t < kMxRowTiles; t++)
+                if (t < ntile) acc[t] = near_tile<KIND, DIM, REG>(kp, dim, xr[t], xc, row[t], col0, bh, bl, acc[t]);
+        }
+        if (more) __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (!more) break;
+        rf = rfn;
+        g = gn;
+        cur ^= 1;
+    }
+    if (!active) return;
+    float* __restrict__ out = P + (int64_t)qme * kMxRowsItem * kMxRhs + c32;
 #pragma unroll
     for (int t = 0; t < kMxRowTiles; t++) {
 #pragma unroll
@@ -403,6 +531,157 @@ lr_ub_kernel(const LrItem* __restrict__ items, int nitems, const __bf16* __restr
     }
 }
 
+// ---- low-rank products, workgroup form ------------------------------------------
+// W wavefronts per workgroup, each with its own WgItem (its own A columns or rows,
+// 1, 2 or 4 interleaved 32-wide tiles) against the B panel groups [b0, b0 + nb)
+// that all of them share.  The workgroup stages B through LDS kWgGroups groups
+// (kWgGroups / 2 reduction steps) at a time, double-buffered, so each B element is
+// loaded once per workgroup instead of once per wavefront; A streams from HBM one
+// reduction step ahead of the MFMAs, as in the one-wavefront kernels above.
+constexpr int kWgGroups = 8;
+constexpr int kWgEntries = kWgGroups * 2 * kMxRhs;  // bf16x8 entries of one staged block (8 KB)
+
+template <int NT>
+__device__ __forceinline__ void wg_load_a(bf16x8 (&hi)[NT], bf16x8 (&lo)[NT], const __bf16* __restrict__ a,
+                                          int64_t a_lo)
+{
+#pragma unroll
+    for (int e = 0; e < NT; e++) {
+        hi[e] = *(const bf16x8*)(a + 256 * e);
+        lo[e] = *(const bf16x8*)(a + 256 * e + a_lo);
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void wg_mfma(f32x16 (&acc)[NT], const bf16x8 (&hi)[NT], const bf16x8 (&lo)[NT],
+                                        bf16x8 bh, bf16x8 bl)
+{
+#pragma unroll
+    for (int e = 0; e < NT; e++) {
+        acc[e] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[e], bh, acc[e], 0, 0, 0);
+        acc[e] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[e], bl, acc[e], 0, 0, 0);
+        acc[e] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo[e], bh, acc[e], 0, 0, 0);
+    }
+}
+
+// B groups [g, g + kWgGroups) of Bpan (bf16x8 entries, 2 * kMxRhs = one wavefront's
+// 64 lanes per group) straight into the LDS block sb (global_load_lds, 16 bytes per
+// lane: the data never passes through registers).  Groups at or past bgroups
+// re-read the last group (no reduction step uses them).  Complete after a vmcnt(0)
+// wait.
+template <int W>
+__device__ __forceinline__ void wg_stage_b(bf16x8* sb, const bf16x8* __restrict__ Bpan, int64_t g,
+                                           int64_t bgroups, int w, int lane)
+{
+#pragma unroll
+    for (int u = 0; u < kWgGroups / W; u++) {
+        const int gl = w + u * W;  // group of this wavefront in the block
+        const int64_t gg = min(g + gl, bgroups - 1);
+        __builtin_amdgcn_global_load_lds((const void*)(Bpan + gg * (2 * kMxRhs) + lane),
+                                         (__attribute__((address_space(3))) void*)(sb + gl * (2 * kMxRhs)), 16,
+                                         0, 0);
+    }
+}
+
+// one wavefront of a workgroup item: NT tiles (NT = 0: no item, staging only)
+template <int NT, int W, bool VB>
+__device__ __forceinline__ void wg_wave(const WgItem& it, int lane, int tid, const __bf16* __restrict__ Apan,
+                                        const bf16x8* __restrict__ Bpan, int64_t bgroups,
+                                        bf16x8 (*sB)[kWgEntries], const int* __restrict__ tmap,
+                                        __bf16* __restrict__ Tpan, float* __restrict__ out)
+{
+    const int c32 = lane & 31, h = lane >> 5;
+    const int w = tid / kWave;
+    const int nsteps = it.nb / 2;                      // even (nb is a multiple of 4)
+    const int nblk = (it.nb + kWgGroups - 1) / kWgGroups;
+    const int64_t gstride = 16 * (int64_t)it.ld;
+    const __bf16* __restrict__ a = Apan + it.aoff + h * gstride + c32 * 8;
+    const int64_t a_step = 2 * gstride, a_lo = 8 * (int64_t)it.ld;
+    f32x16 acc[NT > 0 ? NT : 1];
+#pragma unroll
+    for (int e = 0; e < (NT > 0 ? NT : 1); e++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[e][r] = 0.f;
+    bf16x8 ph[NT > 0 ? NT : 1], pl[NT > 0 ? NT : 1], qh[NT > 0 ? NT : 1], ql[NT > 0 ? NT : 1];
+    if constexpr (NT > 0) wg_load_a<NT>(ph, pl, a, a_lo);
+    for (int blk = 0; blk < nblk; blk++) {
+        const int cur = blk & 1;
+        const bool more = blk + 1 < nblk;
+        // the next block into the other buffer (last read before the previous barrier)
+        if (more) wg_stage_b<W>(sB[cur ^ 1], Bpan, (int64_t)it.b0 + (int64_t)(blk + 1) * kWgGroups, bgroups, w, lane);
+        if constexpr (NT > 0) {
+            const int s0 = blk * (kWgGroups / 2);
+            const int s1 = min(nsteps, s0 + kWgGroups / 2);
+            const bf16x8* __restrict__ sb = sB[cur];
+            for (int s = s0; s < s1; s += 2) {
+                wg_load_a<NT>(qh, ql, a + (s + 1) * a_step, a_lo);
+                __builtin_amdgcn_sched_barrier(0);
+                {
+                    const int gi = 2 * (s - s0) + h;
+                    wg_mfma<NT>(acc, ph, pl, sb[gi * 2 * kMxRhs + c32], sb[gi * 2 * kMxRhs + kMxRhs + c32]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                wg_load_a<NT>(ph, pl, a + min(s + 2, nsteps - 1) * a_step, a_lo);
+                __builtin_amdgcn_sched_barrier(0);
+                {
+                    const int gi = 2 * (s + 1 - s0) + h;
+                    wg_mfma<NT>(acc, qh, ql, sb[gi * 2 * kMxRhs + c32], sb[gi * 2 * kMxRhs + kMxRhs + c32]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        if (more) __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
+    if constexpr (NT > 0) {
+#pragma unroll
+        for (int e = 0; e < NT; e++) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int mm = 32 * e + (r >> 2) * 8 + h * 4 + (r & 3);
+                if (mm < it.m) {
+                    if (VB) {
+                        if (it.tp < 0)
+                            store_split(Tpan, tpan_at(tmap[it.o0 + mm], c32), acc[e][r]);
+                        else
+                            out[((int64_t)it.tp * kMxVCols + mm) * kMxRhs + c32] = acc[e][r];
+                    } else {
+                        out[((int64_t)it.o0 * kMxLrRows + mm) * kMxRhs + c32] = acc[e][r];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// V^T X (VB) or U T (!VB) over workgroup items: items[W*blockIdx.x + w] is wavefront
+// w's share; all W share b0 and nb.  VB: out = the chunk partials Tp (or T panels
+// directly); !VB: out = the partial-row slots P.
+template <int W, bool VB>
+__global__ void __launch_bounds__(W * kWave) __attribute__((amdgpu_waves_per_eu(3)))
+lr_wg_kernel(const WgItem* __restrict__ items, const __bf16* __restrict__ Apan, const bf16x8* __restrict__ Bpan,
+             int64_t bgroups, const int* __restrict__ tmap, __bf16* __restrict__ Tpan, float* __restrict__ out)
+{
+    __shared__ bf16x8 sB[2][kWgEntries];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int w = wave_id();
+    const WgItem it = items[(int64_t)blockIdx.x * W + w];
+    // the first block of B, staged by the whole workgroup
+    wg_stage_b<W>(sB[0], Bpan, it.b0, bgroups, w, lane);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int nt = it.m <= 0 ? 0 : it.m <= 32 ? 1 : it.m <= 64 ? 2 : 4;  // wave-uniform
+    if (nt == 4)
+        wg_wave<4, W, VB>(it, lane, tid, Apan, Bpan, bgroups, sB, tmap, Tpan, out);
+    else if (nt == 2)
+        wg_wave<2, W, VB>(it, lane, tid, Apan, Bpan, bgroups, sB, tmap, Tpan, out);
+    else if (nt == 1)
+        wg_wave<1, W, VB>(it, lane, tid, Apan, Bpan, bgroups, sB, tmap, Tpan, out);
+    else
+        wg_wave<0, W, VB>(it, lane, tid, Apan, Bpan, bgroups, sB, tmap, Tpan, out);
+}
+
 // Ys[i*32 + c] = sum of sorted row i's partials, in list order (deterministic)
 __global__ void __launch_bounds__(kThreads)
 combine_mx_kernel(const int* __restrict__ ptr, const int* __restrict__ idx, const float* __restrict__ P,
@@ -467,6 +746,17 @@ void launch_near_mx(bool reg, unsigned grid, hipStream_t s, const Engine& eng, c
     // the LDS reservation caps near-field workgroups per CU so the low-rank
     // kernels on the other stream keep register room beside them
     const size_t lds = eng.serialize ? 0 : (size_t)eng.near_lds_pad;
+    if (eng.near_wg) {
+        const unsigned gw = (unsigned)(eng.h_mx_wg.size() / kNwWaves);
+        if (gw == 0) return;
+        if (reg)
+            near_mx_wg_kernel<KIND, DIM, true><<<gw, kNwWaves * kWave, lds, s>>>(
+                eng.cpan[0].p, eng.cpan[1].p, kp, eng.mx_items.p, eng.mx_wg.p, eng.mx_refs.p, xp, Pn);
+        else
+            near_mx_wg_kernel<KIND, DIM, false><<<gw, kNwWaves * kWave, lds, s>>>(
+                eng.cpan[0].p, eng.cpan[1].p, kp, eng.mx_items.p, eng.mx_wg.p, eng.mx_refs.p, xp, Pn);
+        return;
+    }
     if (reg)
         near_mx_kernel<KIND, DIM, true><<<grid, kThreads, lds, s>>>(
             eng.cpan[0].p, eng.cpan[1].p, kp, eng.mx_items.p, eng.n_mx_items, eng.mx_refs.p, xp, Pn);
@@ -505,14 +795,18 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
     const int n1 = eng.n[0], n2 = eng.n[1];
     const int dim = kp.dim;
     const int64_t g1 = groups_of(n1), g2 = groups_of(n2);
-    if (!eng.cpan_ready) {
+    const KernelParamsF kpf = make_params_f(kp);
+    if (!eng.cpan_ready || eng.cpan_scale != kpf.cscale) {
         HMG_TRY(eng.cpan[0].alloc(g1 * dim * kGroup));
         HMG_TRY(eng.cpan[1].alloc(g2 * dim * kGroup));
-        coord_panel_kernel<<<grid_for(g1 * kGroup, kThreads), kThreads, 0, s>>>(c1, dim, (int)g1, eng.cpan[0].p);
+        coord_panel_kernel<<<grid_for(g1 * kGroup, kThreads), kThreads, 0, s>>>(c1, dim, (int)g1, kpf.cscale,
+                                                                                eng.cpan[0].p);
         HMG_HIP(hipGetLastError());
-        coord_panel_kernel<<<grid_for(g2 * kGroup, kThreads), kThreads, 0, s>>>(c2, dim, (int)g2, eng.cpan[1].p);
+        coord_panel_kernel<<<grid_for(g2 * kGroup, kThreads), kThreads, 0, s>>>(c2, dim, (int)g2, kpf.cscale,
+                                                                                eng.cpan[1].p);
         HMG_HIP(hipGetLastError());
         eng.cpan_ready = true;
+        eng.cpan_scale = kpf.cscale;
     }
     const int n2p = (int)(g2 * kGroup);
     const int64_t near0 = (int64_t)eng.n_lr_items * kMxLrRows;
@@ -520,8 +814,9 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
     HMG_TRY(eng.Xp.alloc((size_t)g2 * 2 * kMxRhs * kGroup));
     HMG_TRY(eng.Xt32.alloc((size_t)n2 * kMxRhs));
     // T panels: slots no V column maps to (each row cluster's padding) stay zero
+    const int64_t tgroups = eng.tsize / kGroup + 2;
     if (!eng.tpan_ready) {
-        const size_t n = (size_t)(eng.tsize / kGroup + 2) * 2 * kMxRhs * kGroup;
+        const size_t n = (size_t)tgroups * 2 * kMxRhs * kGroup;
         HMG_TRY(eng.Tpan.alloc(n));
         HMG_HIP(hipMemsetAsync(eng.Tpan.p, 0, sizeof(__bf16) * n, s));
         eng.tpan_ready = true;
@@ -535,7 +830,6 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
         HMG_HIP(hipGetLastError());
         eng.inv1_ready = true;
     }
-    const KernelParamsF kpf = make_params_f(kp);
     const bool reg = kp.reg != 0.0;
     const __bf16* U = eng.Up.p;
     const __bf16* V = eng.Vp.p;
@@ -572,7 +866,30 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
             eng.prof.end(PROF_NEAR, s2);
         }
         HMG_HIP(hipEventRecord(eng.ev_join, s2));
-        if (eng.naca > 0 && eng.n_lr_items > 0) {
+        if (eng.naca > 0 && eng.n_lr_items > 0 && eng.lr_wg) {
+            // workgroup form: B staged once per workgroup through LDS
+            eng.prof.begin(PROF_LR_VTX, s);
+            const unsigned gv = (unsigned)(eng.h_vw_items.size() / kVwWaves);
+            if (gv > 0) {
+                lr_wg_kernel<kVwWaves, true><<<gv, kVwWaves * kWave, 0, s>>>(
+                    eng.vw_items.p, V, (const bf16x8*)eng.Xp.p, g2, eng.tmap.p, eng.Tpan.p, eng.Tp32.p);
+                HMG_HIP(hipGetLastError());
+            }
+            if (eng.n_lv_slices > 0) {
+                lr_vb_finalize_kernel<<<eng.n_lv_slices, kThreads, 0, s>>>(eng.lv_slices.p, eng.Tp32.p,
+                                                                           eng.tmap.p, eng.Tpan.p);
+                HMG_HIP(hipGetLastError());
+            }
+            eng.prof.end(PROF_LR_VTX, s);
+            eng.prof.begin(PROF_LR_U, s);
+            const unsigned gu = (unsigned)(eng.h_uw_items.size() / kUwWaves);
+            if (gu > 0) {
+                lr_wg_kernel<kUwWaves, false><<<gu, kUwWaves * kWave, 0, s>>>(
+                    eng.uw_items.p, U, (const bf16x8*)eng.Tpan.p, tgroups, nullptr, nullptr, eng.P32.p);
+                HMG_HIP(hipGetLastError());
+            }
+            eng.prof.end(PROF_LR_U, s);
+        } else if (eng.naca > 0 && eng.n_lr_items > 0) {
             eng.prof.begin(PROF_LR_VTX, s);
             // items grouped by interleaved tile count: [4 | 2 | 1]
             for (int g = 0; g < 3; g++) {

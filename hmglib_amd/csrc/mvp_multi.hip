@@ -1,0 +1,354 @@
+// mvp_multi.hip -- FP64 H-matrix-vector products for blocks of right-hand sides.
+//
+// The FP64 product of mvp.hip streams every factor element once per right-hand
+// side.  Here up to kRbMax right-hand sides share one pass (reference:
+// linear_algebra.cu:4292-4379, 1429-1591, 2817-3137; the same decomposition):
+//   gather     X (caller order, column-major) -> Xs[p * RB + c], sorted points,
+//              one RB-wide row per point (64 points per workgroup through LDS)
+//   lr_v       Tp = V_D^T Xs per (column cluster, 64 columns, <= 1024 points):
+//              lane = column, RB accumulators; the point's Xs row is uniform over
+//              the wavefront (scalar loads)
+//   lr_u       P = U_C T_C per 64 rows of a row cluster: lane = row
+//   near       the near field evaluated on the fly, row form: lane = row, the
+//              column uniform over the wavefront, each entry evaluated once for
+//              all RB right-hand sides
+//   combine    every row adds its partials in the fixed order of the row's list,
+//              then the sorted rows go back to the caller's order through LDS
+// Every right-hand side's arithmetic is independent of RB and of the other
+// columns in the pass (one FMA chain per column, in point / rank / column order),
+// so a column's result is bitwise the same in every pass width.
+#include "hmg_internal.h"
+#include "kernel_eval.h"
+
+namespace hmg {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+constexpr int kRowTile = 64;
+
+// Xs[p*RB + c] = X[order2[p] + (c0 + c) * n2] (0 for c >= nr)
+template <int RB>
+__global__ void __launch_bounds__(kThreads)
+gather_rows_kernel(const double* __restrict__ X, const uint64_t* __restrict__ order2, int n2, int c0, int nr,
+                   double* __restrict__ Xs)
+{
+    __shared__ double tile[RB][kRowTile + 1];
+    const int p0 = blockIdx.x * kRowTile;
+    for (int e = threadIdx.x; e < kRowTile * RB; e += kThreads) {
+        const int c = e / kRowTile, pl = e % kRowTile;
+        const int p = p0 + pl;
+        tile[c][pl] = (p < n2 && c < nr) ? X[(int64_t)order2[p] + (int64_t)(c0 + c) * n2] : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kRowTile * RB; e += kThreads) {
+        const int pl = e / RB, c = e % RB;
+        const int p = p0 + pl;
+        if (p < n2) Xs[(int64_t)p * RB + c] = tile[c][pl];
+    }
+}
+
+// Tp[(q*64 + lane)*RB + c] = sum_{j < nrows} V[vbase + lane + j*ld] * Xs[(j0 + j)*RB + c]
+template <int RB>
+__global__ void __launch_bounds__(kThreads)
+lr_v_multi_kernel(const LvItem* __restrict__ items, int nitems, const double* __restrict__ V,
+                  const double* __restrict__ Xs, double* __restrict__ Tp)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = blockIdx.x * kWaves + wave_id();
+    if (q >= nitems) return;
+    const LvItem it = items[q];
+    const double* __restrict__ v = V + it.vbase + (lane < it.ncols ? lane : 0);
+    const double* __restrict__ xb = Xs + (int64_t)it.j0 * RB;
+    const int64_t ld = it.ld;
+    double acc[RB];
+#pragma unroll
+    for (int c = 0; c < RB; c++) acc[c] = 0.0;
+    int j = 0;
+    for (; j + 4 <= it.nrows; j += 4) {
+        const double v0 = v[(j + 0) * ld], v1 = v[(j + 1) * ld], v2 = v[(j + 2) * ld], v3 = v[(j + 3) * ld];
+        const double* __restrict__ x = xb + (int64_t)j * RB;
+#pragma unroll
+        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(v0, x[c], acc[c]);
+#pragma unroll
+        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(v1, x[RB + c], acc[c]);
+#pragma unroll
+        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(v2, x[2 * RB + c], acc[c]);
+#pragma unroll
+        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(v3, x[3 * RB + c], acc[c]);
+    }
+    for (; j < it.nrows; j++) {
+        const double vj = v[j * ld];
+        const double* __restrict__ x = xb + (int64_t)j * RB;
+#pragma unroll
+        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(vj, x[c], acc[c]);
+    }
+    double* __restrict__ out = Tp + ((int64_t)q * kTileRows + lane) * RB;
+#pragma unroll
+    for (int c = 0; c < RB; c++) out[c] = acc[c];
+}
+
+// T[tmap[tcol0 + lane]*RB + c] = sum of the slice's point chunks, in chunk order
+template <int RB>
+__global__ void __launch_bounds__(kThreads)
+lr_v_finalize_multi_kernel(const LvSlice* __restrict__ slices, int nslices, const double* __restrict__ Tp,
+                           const int* __restrict__ tmap, double* __restrict__ T)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = blockIdx.x * kWaves + wave_id();
+    if (q >= nslices) return;
+    const LvSlice sl = slices[q];
+    if (lane >= sl.ncols) return;
+    double s[RB];
+#pragma unroll
+    for (int c = 0; c < RB; c++) s[c] = 0.0;
+    for (int ch = 0; ch < sl.nchunks; ch++) {
+        const double* __restrict__ p = Tp + ((int64_t)(sl.item0 + ch) * kTileRows + lane) * RB;
+#pragma unroll
+        for (int c = 0; c < RB; c++) s[c] += p[c];
+    }
+    double* __restrict__ t = T + (int64_t)tmap[sl.tcol0 + lane] * RB;
+#pragma unroll
+    for (int c = 0; c < RB; c++) t[c] = s[c];
+}
+
+// P[(q*64 + lane)*RB + c] = sum_l U[uoff + lane + l*ld] * T[(toff + l)*RB + c]
+template <int RB>
+__global__ void __launch_bounds__(kThreads)
+lr_u_multi_kernel(const LrItem* __restrict__ items, int nitems, const double* __restrict__ U,
+                  const double* __restrict__ T, double* __restrict__ P)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = blockIdx.x * kWaves + wave_id();
+    if (q >= nitems) return;
+    const LrItem it = items[q];
+    const bool act = lane < it.nrows;
+    const double* __restrict__ u = U + it.uoff + (act ? lane : 0);
+    const double* __restrict__ tb = T + it.toff * RB;
+    const int64_t ld = it.ld;
+    double acc[RB];
+#pragma unroll
+    for (int c = 0; c < RB; c++) acc[c] = 0.0;
+    int l = 0;
+    for (; l + 4 <= it.K; l += 4) {
+        const double u0 = u[(l + 0) * ld], u1 = u[(l + 1) * ld], u2 = u[(l + 2) * ld], u3 = u[(l + 3) * ld];
+        const double* __restrict__ t = tb + (int64_t)l * RB;
+#pragma unroll
+        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(u0, t[c], acc[c]);
+#pragma unroll
+        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(u1, t[RB + c], acc[c]);
+#pragma unroll
+        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(u2, t[2 * RB + c], acc[c]);
+#pragma unroll
+        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(u3, t[3 * RB + c], acc[c]);
+    }
+    for (; l < it.K; l++) {
+        const double ul = u[l * ld];
+        const double* __restrict__ t = tb + (int64_t)l * RB;
+#pragma unroll
+        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(ul, t[c], acc[c]);
+    }
+    if (!act) return;
+    double* __restrict__ out = P + ((int64_t)q * kTileRows + lane) * RB;
+#pragma unroll
+    for (int c = 0; c < RB; c++) out[c] = acc[c];
+}
+
+// Near field on the fly, row form, over the stored-near-field schedule
+// (items {row0, nrows <= 128, b0, b1} over near_blk): lane = row (two passes of 64
+// rows), the item's dense blocks' columns in block order, each entry evaluated
+// once and applied to all RB right-hand sides.  P[(q*128 + r)*RB + c].
+template <int KIND, int DIM, int RB>
+__global__ void __launch_bounds__(kThreads)
+near_multi_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ items, int nitems,
+                  const int* __restrict__ near_blk, const int4* __restrict__ blk, const double* __restrict__ Xs,
+                  double* __restrict__ P)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = blockIdx.x * kWaves + wave_id();
+    if (q >= nitems) return;
+    const int4 it = items[q];
+    const int dim = DIM > 0 ? DIM : kp.dim;
+    for (int half = 0; half * kWave < it.y; half++) {
+        const bool act = lane + half * kWave < it.y;
+        const int row = it.x + (act ? lane + half * kWave : 0);
+        double xi[MAX_POINT_DIMENSION];
+#pragma unroll
+        for (int d = 0; d < MAX_POINT_DIMENSION; d++) xi[d] = d < dim ? c1.x[d][row] : 0.0;
+        double acc[RB];
+#pragma unroll
+        for (int c = 0; c < RB; c++) acc[c] = 0.0;
+        for (int bi = it.z; bi < it.w; bi++) {
+            const int4 b = blk[near_blk[bi]];
+            for (int col = b.z; col <= b.w; col++) {
+                double r2 = 0.0;
+#pragma unroll
+                for (int d = 0; d < MAX_POINT_DIMENSION; d++) {
+                    if (d < dim) {
+                        const double t = xi[d] - c2.x[d][col];
+                        r2 = __fma_rn(t, t, r2);
+                    }
+                }
+                double kv = kernel_of_r2_k<KIND>(kp, r2);
+                if (row == col) kv += kp.reg;
+                const double* __restrict__ x = Xs + (int64_t)col * RB;
+#pragma unroll
+                for (int c = 0; c < RB; c++) acc[c] = __fma_rn(kv, x[c], acc[c]);
+            }
+        }
+        if (act) {
+            double* __restrict__ out = P + ((int64_t)q * 2 * kTileRows + lane + half * kWave) * RB;
+#pragma unroll
+            for (int c = 0; c < RB; c++) out[c] = acc[c];
+        }
+    }
+}
+
+// Ys[i*RB + c] = sum of sorted row i's partials P[v*RB + c], in list order
+template <int RB>
+__global__ void __launch_bounds__(kThreads)
+combine_multi_kernel(const int* __restrict__ ptr, const int* __restrict__ idx, const double* __restrict__ P,
+                     double* __restrict__ Ys, int n1)
+{
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int i = (int)(t / RB), c = (int)(t % RB);
+    if (i >= n1) return;
+    double s = 0.0;
+    for (int q = ptr[i]; q < ptr[i + 1]; q++) s += P[(int64_t)idx[q] * RB + c];
+    Ys[t] = s;
+}
+
+// Y[p + (c0 + c)*n1] = Ys[inv1[p]*RB + c] for c < nr, through LDS (64 rows per workgroup)
+template <int RB>
+__global__ void __launch_bounds__(kThreads)
+permute_out_multi_kernel(const double* __restrict__ Ys, const int* __restrict__ inv1, double* __restrict__ Y,
+                         int n1, int c0, int nr)
+{
+    __shared__ double tile[RB][kRowTile + 1];
+    const int p0 = blockIdx.x * kRowTile;
+    for (int e = threadIdx.x; e < kRowTile * RB; e += kThreads) {
+        const int pl = e / RB, c = e % RB;
+        const int p = p0 + pl;
+        tile[c][pl] = p < n1 ? Ys[(int64_t)inv1[p] * RB + c] : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kRowTile * RB; e += kThreads) {
+        const int c = e / kRowTile, pl = e % kRowTile;
+        const int p = p0 + pl;
+        if (c < nr && p < n1) Y[p + (int64_t)(c0 + c) * n1] = tile[c][pl];
+    }
+}
+
+__global__ void inv_rows_kernel(const uint64_t* __restrict__ order, int* __restrict__ inv, int n)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) inv[order[i]] = i;
+}
+
+template <int KIND, int DIM, int RB>
+void launch_near_multi(hipStream_t s, const Coords& c1, const Coords& c2, const KernelParams& kp,
+                       const Engine& eng, double* Pn)
+{
+    near_multi_kernel<KIND, DIM, RB><<<grid_for(eng.n_near_items, kWaves), kThreads, eng.near_lds_pad, s>>>(
+        c1, c2, kp, eng.near_items.p, eng.n_near_items, eng.near_blk.p, eng.blk.p, eng.X64.p, Pn);
+}
+
+template <int RB>
+hmg_status pass(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp, const uint64_t* order2,
+                const double* X, double* Y, int c0, int nr, const double* U, const double* V)
+{
+    hipStream_t s = eng.stream, s2 = eng.serialize ? eng.stream : eng.stream2;
+    const int n1 = eng.n[0], n2 = eng.n[1];
+    eng.prof.begin(PROF_GATHER, s);
+    gather_rows_kernel<RB><<<grid_for(n2, kRowTile), kThreads, 0, s>>>(X, order2, n2, c0, nr, eng.X64.p);
+    HMG_HIP(hipGetLastError());
+    eng.prof.end(PROF_GATHER, s);
+    HMG_HIP(hipEventRecord(eng.ev_fork, s));
+    HMG_HIP(hipStreamWaitEvent(s2, eng.ev_fork, 0));
+    double* Pn = eng.P64.p + (int64_t)eng.n_lr_items * kTileRows * RB;
+    if (eng.n_near_items > 0) {
+        eng.prof.begin(PROF_NEAR, s2);
+        HMG_DISPATCH_KIND(kp.kind, {
+            if (kp.dim == 3)
+                launch_near_multi<KIND, 3, RB>(s2, c1, c2, kp, eng, Pn);
+            else if (kp.dim == 2)
+                launch_near_multi<KIND, 2, RB>(s2, c1, c2, kp, eng, Pn);
+            else
+                launch_near_multi<KIND, 0, RB>(s2, c1, c2, kp, eng, Pn);
+        });
+        HMG_HIP(hipGetLastError());
+        eng.prof.end(PROF_NEAR, s2);
+    }
+    HMG_HIP(hipEventRecord(eng.ev_join, s2));
+    if (eng.naca > 0 && eng.n_lr_items > 0) {
+        eng.prof.begin(PROF_LR_VTX, s);
+        lr_v_multi_kernel<RB><<<grid_for(eng.n_lv_items, kWaves), kThreads, 0, s>>>(eng.lv_items.p, eng.n_lv_items,
+                                                                                  V, eng.X64.p, eng.Tp64.p);
+        HMG_HIP(hipGetLastError());
+        lr_v_finalize_multi_kernel<RB><<<grid_for(eng.n_lv_slices, kWaves), kThreads, 0, s>>>(
+            eng.lv_slices.p, eng.n_lv_slices, eng.Tp64.p, eng.tmap.p, eng.T64.p);
+        HMG_HIP(hipGetLastError());
+        eng.prof.end(PROF_LR_VTX, s);
+        eng.prof.begin(PROF_LR_U, s);
+        lr_u_multi_kernel<RB><<<grid_for(eng.n_lr_items, kWaves), kThreads, 0, s>>>(eng.lr_items.p, eng.n_lr_items,
+                                                                                  U, eng.T64.p, eng.P64.p);
+        HMG_HIP(hipGetLastError());
+        eng.prof.end(PROF_LR_U, s);
+    }
+    HMG_HIP(hipStreamWaitEvent(s, eng.ev_join, 0));
+    eng.prof.begin(PROF_COMBINE, s);
+    combine_multi_kernel<RB><<<grid_for((int64_t)n1 * RB, kThreads), kThreads, 0, s>>>(
+        eng.comb_stored.ptr.p, eng.comb_stored.idx.p, eng.P64.p, eng.Ys64.p, n1);
+    HMG_HIP(hipGetLastError());
+    permute_out_multi_kernel<RB><<<grid_for(n1, kRowTile), kThreads, 0, s>>>(eng.Ys64.p, eng.inv1.p, Y, n1, c0, nr);
+    HMG_HIP(hipGetLastError());
+    eng.prof.end(PROF_COMBINE, s);
+    return HMG_OK;
+}
+
+}  // namespace
+
+hmg_status hmvp_multi(Engine& eng, const Coords& c1, const Coords& c2, const KernelParams& kp,
+                      const uint64_t* order1, const uint64_t* order2, const double* X, double* Y, int nrhs,
+                      const double* U, const double* V)
+{
+    if (eng.layout_prec != HMG_PREC_FP64) {
+        set_error("FP64 block product without an FP64 layout");
+        return HMG_ERR_NOT_READY;
+    }
+    hipStream_t s = eng.stream;
+    const int n1 = eng.n[0], n2 = eng.n[1];
+    const int rb = nrhs >= kRbMax ? kRbMax : nrhs > 8 ? 16 : nrhs > 4 ? 8 : 4;
+    // partial-sum slots of the stored-near-field schedule (the combine lists')
+    const int64_t slots = (int64_t)eng.n_lr_items * kTileRows + (int64_t)eng.n_near_items * 2 * kTileRows;
+    HMG_TRY(eng.X64.alloc((size_t)std::max(n2, 1) * rb));
+    HMG_TRY(eng.Tp64.alloc((size_t)std::max(eng.n_lv_items, 1) * kTileRows * rb));
+    HMG_TRY(eng.T64.alloc((size_t)std::max<int64_t>(eng.tsize, 1) * rb));
+    HMG_TRY(eng.P64.alloc((size_t)std::max<int64_t>(slots, 1) * rb));
+    HMG_TRY(eng.Ys64.alloc((size_t)std::max(n1, 1) * rb));
+    if (!eng.inv1_ready) {
+        HMG_TRY(eng.inv1.alloc(std::max(n1, 1)));
+        inv_rows_kernel<<<grid_for(n1, kThreads), kThreads, 0, s>>>(order1, eng.inv1.p, n1);
+        HMG_HIP(hipGetLastError());
+        eng.inv1_ready = true;
+    }
+    for (int c0 = 0; c0 < nrhs; c0 += rb) {
+        const int nr = std::min(rb, nrhs - c0);
+        hmg_status st;
+        if (rb == 32)
+            st = pass<32>(eng, c1, c2, kp, order2, X, Y, c0, nr, U, V);
+        else if (rb == 16)
+            st = pass<16>(eng, c1, c2, kp, order2, X, Y, c0, nr, U, V);
+        else if (rb == 8)
+            st = pass<8>(eng, c1, c2, kp, order2, X, Y, c0, nr, U, V);
+        else
+            st = pass<4>(eng, c1, c2, kp, order2, X, Y, c0, nr, U, V);
+        HMG_TRY(st);
+    }
+    return HMG_OK;
+}
+
+}  // namespace hmg

@@ -146,6 +146,7 @@ void build_mixed_near(Engine& eng)
 {
     eng.h_mx_items.clear();
     eng.h_mx_refs.clear();
+    eng.h_mx_wg.clear();
     eng.mx_evals = 0;
     const int ng = (int)eng.h_dgrp_ptr.size() - 1;
     for (int g = 0; g < ng; g++) {
@@ -189,11 +190,17 @@ void build_mixed_near(Engine& eng)
             const int ref0 = (int)eng.h_mx_refs.size();
             eng.h_mx_refs.insert(eng.h_mx_refs.end(), pieces.begin() + p0, pieces.begin() + p1);
             const int ref1 = (int)eng.h_mx_refs.size();
+            const int q0 = (int)eng.h_mx_items.size();
             for (int r0 = 0; r0 < m1; r0 += kMxRows) {
                 const int nr = std::min(kMxRows, m1 - r0);
                 eng.h_mx_items.push_back(MxItem{l1 + r0, nr, ref0, ref1});
                 eng.mx_evals += round_up(nr, 32) * cols;  // whole 32-row tiles are evaluated
             }
+            // workgroup form: the row slices of this chunk kNwWaves at a time (they
+            // share its column groups); -1 = an idle wavefront
+            const int q1 = (int)eng.h_mx_items.size();
+            for (int q = q0; q < q1; q += kNwWaves)
+                for (int w = 0; w < kNwWaves; w++) eng.h_mx_wg.push_back(q + w < q1 ? q + w : -1);
             p0 = p1;
         }
     }
@@ -421,6 +428,8 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
         // BF16 hi + lo panels (layout described with MxPanel).  U panels and T slots
         // per row cluster; T slots of a cluster padded to 32 (zero slots)
         eng.h_mxp.assign(std::max(na, 1), MxPanel{});
+        eng.h_vw_items.clear();
+        eng.h_uw_items.clear();
         int64_t urun = 0, trun = 0;
         const int ng = (int)eng.h_agrp_ptr.size() - 1;
         for (int g = 0; g < ng; g++) {
@@ -485,6 +494,7 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
             }
             vrun += (int64_t)ngp * 16 * Kp;
             const int nch = (ngp + vgroups - 1) / vgroups;
+            const size_t first_item = vitems.size();
             for (int s0 = 0; s0 < K; s0 += kMxVCols) {
                 const int nc = std::min(kMxVCols, K - s0);
                 const bool direct = nch == 1;
@@ -498,6 +508,21 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
                 if (!direct) {
                     slices.push_back(LvSlice{tp0, nch, nc, (int)(col + s0)});
                     ntp += nch;
+                }
+            }
+            // workgroup items: the slices of one point chunk, kVwWaves at a time
+            const int nsl = (K + kMxVCols - 1) / kMxVCols;
+            for (int c = 0; c < nch; c++) {
+                for (int q0 = 0; q0 < nsl; q0 += kVwWaves) {
+                    const LvItem& lead = vitems[first_item + (size_t)q0 * nch + c];
+                    for (int w = 0; w < kVwWaves; w++) {
+                        if (q0 + w < nsl) {
+                            const LvItem& it = vitems[first_item + (size_t)(q0 + w) * nch + c];
+                            eng.h_vw_items.push_back(WgItem{it.vbase, it.ld, it.ncols, it.j0, it.nrows, it.tcol0, it.tp});
+                        } else {
+                            eng.h_vw_items.push_back(WgItem{0, lead.ld, 0, lead.j0, lead.nrows, 0, -1});
+                        }
+                    }
                 }
             }
             col += K;
@@ -516,6 +541,38 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
         }
         eng.lr_nt_off[3] = (int)grouped.size();
         items.swap(grouped);
+        // U T workgroup items: the row slices of one row cluster (they share its T
+        // slots), kUwWaves at a time; partial-row slots as the grouped items'
+        {
+            std::unordered_map<int64_t, std::vector<int>> by_t;
+            std::vector<int64_t> t_order;
+            for (int q = 0; q < (int)items.size(); q++) {
+                auto it = by_t.find(items[q].toff);
+                if (it == by_t.end()) {
+                    t_order.push_back(items[q].toff);
+                    by_t[items[q].toff].push_back(q);
+                } else {
+                    it->second.push_back(q);
+                }
+            }
+            std::sort(t_order.begin(), t_order.end());
+            for (int64_t t : t_order) {
+                std::vector<int>& qs = by_t[t];
+                std::sort(qs.begin(), qs.end(), [&](int a, int b) { return items[a].row0 < items[b].row0; });
+                for (size_t q0 = 0; q0 < qs.size(); q0 += kUwWaves) {
+                    const LrItem& lead = items[qs[q0]];
+                    for (int w = 0; w < kUwWaves; w++) {
+                        if (q0 + w < qs.size()) {
+                            const LrItem& it = items[qs[q0 + w]];
+                            eng.h_uw_items.push_back(
+                                WgItem{it.uoff, it.ld, it.nrows, (int)it.toff, it.K / 8, qs[q0 + w], -1});
+                        } else {
+                            eng.h_uw_items.push_back(WgItem{0, lead.ld, 0, (int)lead.toff, lead.K / 8, 0, -1});
+                        }
+                    }
+                }
+            }
+        }
         std::vector<LvItem> vgrouped;
         vgrouped.reserve(vitems.size());
         gi = 0;
@@ -562,6 +619,7 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
         if (!eng.mx_sched) {
             build_mixed_near(eng);
             HMG_TRY(upload(eng.mx_items, eng.h_mx_items, s));
+            HMG_TRY(upload(eng.mx_wg, eng.h_mx_wg, s));
             HMG_TRY(upload(eng.mx_refs, eng.h_mx_refs, s));
             eng.mx_sched = true;
         }
@@ -591,7 +649,11 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
     HMG_TRY(upload(eng.lv_items, vitems, s));
     HMG_TRY(upload(eng.lv_slices, slices, s));
     HMG_TRY(upload(eng.tmap, tmap, s));
-    if (mixed) HMG_TRY(upload(eng.mxp, eng.h_mxp, s));
+    if (mixed) {
+        HMG_TRY(upload(eng.mxp, eng.h_mxp, s));
+        HMG_TRY(upload(eng.vw_items, eng.h_vw_items, s));
+        HMG_TRY(upload(eng.uw_items, eng.h_uw_items, s));
+    }
     if (!mixed) {
         HMG_TRY(eng.Tp.alloc((size_t)std::max(eng.n_lv_items, 1) * kSlice));
         eng.p_slots = col0 + eng.n_col_slots;

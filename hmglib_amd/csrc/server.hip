@@ -249,6 +249,10 @@ hmg_status run_build(hmg_server* s, Matrix* m, Request* r, hipStream_t slot, std
         d.max_batched_aca_size = 1 << 30;
         st = hmg_set_option(&d, HMG_OPT_PRECISION, s->cfg.precision);
         if (st == HMG_OK) st = hmg_set_option(&d, HMG_OPT_ACA_TOLERANCE, b.aca_tolerance);
+        if (st == HMG_OK) st = hmg_set_option(&d, HMG_OPT_EXACT_BOXES, b.exact_boxes);
+        // every FP64 pass through the block product: a request's result is then
+        // bitwise the same whatever other requests share its pass
+        if (st == HMG_OK) st = hmg_set_option(&d, HMG_OPT_FP64_BLOCK_RHS, 1);
     }
     if (st == HMG_OK) st = hmg_setup_h_matrix(&d);
     if (st == HMG_OK) {

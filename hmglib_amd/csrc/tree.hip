@@ -370,6 +370,16 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
             HMG_TRY(scan_i32(flags.p, rank.p, n, false, dT, s));
             compact_kernel<<<grid_for(n, kT), kT, 0, s>>>(flags.p, ub.p, rank.p, n, Lc.p, Uc.p);
             HMG_HIP(hipGetLastError());
+            if (tp.exact_boxes) {
+                // table entry t = the t-th distinct cluster [Lc[t], Uc[t]] itself
+                run_minmax_kernel<<<(unsigned)ncur, kT, 0, s>>>(*cs[set], dim, Lc.p, Uc.p, dT, dT, tmin.p,
+                                                                tmax.p, dDiag);
+                HMG_HIP(hipGetLastError());
+                assign_box_kernel<<<grid_for(ncur, kT), kT, 0, s>>>(cur.p, ncur, set, dim, rank.p, tmin.p,
+                                                                    tmax.p);
+                HMG_HIP(hipGetLastError());
+                continue;
+            }
             HMG_HIP(hipMemsetAsync(keys.p, 0, sizeof(int) * n, s));
             keys_set_kernel<<<grid_for(ncur, kT), kT, 0, s>>>(Lc.p, Uc.p, dT, keys.p, dDiag);
             HMG_HIP(hipGetLastError());

@@ -131,10 +131,11 @@ class HMatrix:
     def autotune(self, xd, yd, nrhs=1, reps=2):
         """hmg_autotune_product on device arrays: keeps the fastest launch shape of
         the product and returns it (yd is overwritten)."""
-        out = (ctypes.c_int64 * 3)()
+        out = (ctypes.c_int64 * 4)()
         _abi.check(self.L.hmg_autotune_product(ctypes.byref(self.data), xd.ptr, yd.ptr, nrhs, reps, out),
                    "hmg_autotune_product")
-        return {"near_lds_bytes": int(out[0]), "one_stream": bool(out[1]), "us_per_product": int(out[2])}
+        return {"near_lds_bytes": int(out[0]), "one_stream": bool(out[1]), "us_per_product": int(out[2]),
+                "lr_workgroup": bool(out[3] & 1), "near_workgroup": bool(out[3] & 2)}
 
     # ---- products on host arrays ---------------------------------------------
     def apply(self, x):

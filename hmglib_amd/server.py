@@ -24,10 +24,10 @@ class Server:
         self._keep = {}   # host / device buffers that must outlive their requests
 
     def submit_build(self, X1, X2, bits, eta, max_level, c_leaf, k, kind, regularization=0.0,
-                     length_scale=1.0, epsilon=0.0, aca_tolerance=0):
+                     length_scale=1.0, epsilon=0.0, aca_tolerance=0, exact_boxes=False):
         """X1, X2: (dim, n) point arrays (X2 is X1 for one point set).  Returns
         (request id, matrix id).  aca_tolerance: HMG_OPT_ACA_TOLERANCE mode with
-        epsilon."""
+        epsilon; exact_boxes: HMG_OPT_EXACT_BOXES."""
         X1 = np.ascontiguousarray(X1, dtype=np.float64)
         same = X2 is X1
         X2 = X1 if same else np.ascontiguousarray(X2, dtype=np.float64)
@@ -40,6 +40,7 @@ class Server:
         b.eta, b.max_level, b.c_leaf, b.k = eta, max_level, c_leaf, k
         b.kernel_kind, b.regularization, b.length_scale = kind, regularization, length_scale
         b.epsilon, b.aca_tolerance = epsilon, aca_tolerance
+        b.exact_boxes = int(bool(exact_boxes))
         r = ctypes.c_int64()
         m = ctypes.c_int64()
         _abi.check(self.L.hmg_server_submit_build(self.h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(m)),

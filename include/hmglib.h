@@ -149,8 +149,9 @@ hmg_status hmg_apply_batch(struct h_matrix_data* const* data, const double* cons
 /* Y = H X for an H-matrix with precomputed FP64 factors, allocating nothing (no
  * counterpart in the reference; for callers that cannot allocate device memory:
  * code under stream capture, serving loops that forbid allocation).  Same kernels
- * and summation order as hmg_apply_h_matrix_mvp_multi, so the result is bitwise
- * its result.  The per-call buffers live in a fixed device array of 32 MiB that
+ * and summation order as hmg_apply_h_matrix_mvp_multi one right-hand side at a
+ * time (HMG_OPT_FP64_BLOCK_RHS above nrhs), so the result is bitwise that result
+ * (and within rounding of the block path's).  The per-call buffers live in a fixed device array of 32 MiB that
  * the library holds on every device it runs on until the process ends; an
  * H-matrix whose buffers need more is refused with HMG_ERR_INVALID, mixed-precision
  * or missing factors with HMG_ERR_NOT_READY.  Calls on one device share the array:
@@ -158,14 +159,17 @@ hmg_status hmg_apply_batch(struct h_matrix_data* const* data, const double* cons
 hmg_status hmg_apply_h_matrix_mvp_noalloc(const double* X, double* Y, int nrhs, struct h_matrix_data* data);
 /* Autotuner of the product's launch shape (no counterpart in the reference): times
  * `reps` products (after one warm-up) for each candidate -- the near field's LDS
- * reservation of 0, 32, 48, 64 or 80 KB per workgroup with the near field and the
- * low-rank kernels on two streams, or everything on one stream -- on the caller's
- * device arrays X, Y (Y is overwritten; results do not depend on the choice) and
- * keeps the fastest (HMG_OPT_NEAR_LDS_BYTES, HMG_OPT_SERIALIZE).  out[0] = LDS
- * bytes, out[1] = 1 for one stream, out[2] = microseconds per product.  Needs
- * precompute_aca. */
+ * reservation of 0, 16, 24, 36, 48, 64 or 80 KB per workgroup with the near field
+ * and the low-rank kernels on two streams, or everything on one stream; with
+ * mixed-precision factors each with the low-rank products and the near field in
+ * workgroup or one-wavefront form -- on the caller's device arrays X, Y (Y is
+ * overwritten; results do not depend on the choice) and keeps the fastest
+ * (HMG_OPT_NEAR_LDS_BYTES, HMG_OPT_SERIALIZE, HMG_OPT_LR_WORKGROUP,
+ * HMG_OPT_NEAR_WORKGROUP).  out[0] = LDS bytes, out[1] = 1 for one stream, out[2] =
+ * microseconds per product, out[3] = 1 (low-rank workgroup form) | 2 (near-field
+ * workgroup form).  Needs precompute_aca. */
 hmg_status hmg_autotune_product(struct h_matrix_data* data, const double* X, double* Y, int nrhs, int reps,
-                                int64_t out[3]);
+                                int64_t out[4]);
 /* dense product for nrhs right-hand sides on sampled ORIGINAL rows (checker helper
  * at scale): Yrows[q + c*nrows] = sum_j K(rows[q], j) X[j + c*n2]. rows is a device
  * int64 array. */
@@ -221,7 +225,29 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
  *   ||u_r|| ||v_r|| <= epsilon (1 - eta) / (1 + epsilon) ||U V^T||_F (never for
  *   eta >= 1).  2: the plain relative criterion, checked at every step:
  *   ||u_r|| ||v_r|| <= epsilon ||U V^T||_F.  Uses data->epsilon and data->eta at
- *   precompute_aca; ranks stay capped at k and the factor layout keeps k columns. */
+ *   precompute_aca; ranks stay capped at k and the factor layout keeps k columns.
+ * HMG_OPT_LR_WORKGROUP [0]: mixed precision: 1 runs V^T X and U T as workgroups
+ *   of wavefronts that share their X (T) panel groups through LDS (each group
+ *   loaded once per workgroup); 0: one wavefront per item, each loading its own.
+ *   The products are bitwise the same (the autotuner picks the faster).
+ * HMG_OPT_EXACT_BOXES [0]: cluster bounding boxes of the tree.  0: the
+ *   reference's lookup table (tree.cu:852-1147), bit for bit, including its
+ *   behaviour on a level that holds a one-point cluster, where the boxes of the
+ *   following clusters are shifted and blocks that are not separated can be taken
+ *   as admissible (large approximation errors on clustered clouds).  1: every
+ *   cluster's box is the exact min/max of its points; the block list is then the
+ *   reference's wherever no level holds a one-point cluster.  Takes effect at the
+ *   next setup_h_matrix.
+ * HMG_OPT_NEAR_WORKGROUP [0]: mixed precision: 1 runs the near field as
+ *   workgroups of two wavefronts (two row slices of one column chunk) that share
+ *   each step's X panel groups through LDS; 0: one wavefront per row slice.  The
+ *   products are bitwise the same (the autotuner picks the faster).
+ * HMG_OPT_FP64_BLOCK_RHS [2]: FP64 products with at least this many right-hand
+ *   sides run in passes of up to 32 (each factor element read once per pass, the
+ *   near field evaluated once per entry and pass in row form) instead of one
+ *   right-hand side at a time (symmetric near field).  A right-hand side's result
+ *   is bitwise the same in every pass width; between the two paths it agrees to
+ *   rounding.  1 sends every FP64 product through the block path. */
 #define HMG_OPT_NEAR_SYMMETRIC 1
 #define HMG_OPT_NEAR_LDS_BYTES 2
 #define HMG_OPT_SERIALIZE 3
@@ -230,6 +256,10 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
 #define HMG_OPT_ACA_BIG_BLOCK 6
 #define HMG_OPT_ACA_TOLERANCE 7
 #define HMG_OPT_ACA_HUGE_BLOCK 8
+#define HMG_OPT_LR_WORKGROUP 9
+#define HMG_OPT_EXACT_BOXES 10
+#define HMG_OPT_NEAR_WORKGROUP 11
+#define HMG_OPT_FP64_BLOCK_RHS 12
 #define HMG_PREC_FP64 0
 #define HMG_PREC_MIXED 1
 hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value);

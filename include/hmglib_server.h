@@ -59,6 +59,7 @@ typedef struct {
 	double length_scale;
 	double epsilon;             /* ACA tolerance (used when aca_tolerance != 0) */
 	int aca_tolerance;          /* HMG_OPT_ACA_TOLERANCE mode: 0 fixed rank k, 1, 2 */
+	int exact_boxes;            /* HMG_OPT_EXACT_BOXES: 0 the reference's cluster boxes, 1 exact */
 } hmg_build_request;
 
 typedef struct {

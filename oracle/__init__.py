@@ -78,6 +78,10 @@ def lib():
         L.orc_sort.argtypes = [P(ctypes.c_uint64), ctypes.c_int64, P(ctypes.c_uint64)]
         L.orc_find_split.restype = ctypes.c_int
         L.orc_find_split.argtypes = [P(ctypes.c_uint64), ctypes.c_int, ctypes.c_int]
+        L.orc_build_ex.restype = ctypes.c_void_p
+        L.orc_build_ex.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_build.restype = ctypes.c_void_p
         L.orc_build.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double,
@@ -119,7 +123,7 @@ def lib():
         L.orc_hmvp.argtypes = [ctypes.c_void_p, P(Kernel), ctypes.c_int, P(ctypes.c_double),
                                P(ctypes.c_double)]
         L.orc_hmvp_sample.restype = ctypes.c_int64
-        L.orc_hmvp_sample.argtypes = [ctypes.c_void_p, P(Kernel), ctypes.c_int,
+        L.orc_hmvp_sample.argtypes = [ctypes.c_void_p, P(Kernel), ctypes.c_int, ctypes.c_int,
                                       P(ctypes.c_double), P(ctypes.c_double)]
         L.orc_dense_mvp.argtypes = [ctypes.c_void_p, P(Kernel), ctypes.c_int, P(ctypes.c_double),
                                     P(ctypes.c_double)]
@@ -224,15 +228,16 @@ def kernel(kind, dim, regularization=0.0, length_scale=1.0):
 class HMatrix:
     """Oracle H-matrix: build (Morton, sort, tree, partition), ACA, products."""
 
-    def __init__(self, X1, X2, bits, eta, max_level, c_leaf, root_level_1=0, root_level_2=0):
+    def __init__(self, X1, X2, bits, eta, max_level, c_leaf, root_level_1=0, root_level_2=0,
+                 exact_boxes=False):
         X1 = np.ascontiguousarray(X1, dtype=np.float64)
         X2 = np.ascontiguousarray(X2, dtype=np.float64)
         assert X1.shape[0] == X2.shape[0]
         self.dim = X1.shape[0]
         self.n1 = X1.shape[1]
         self.n2 = X2.shape[1]
-        self._h = lib().orc_build(self.n1, self.n2, self.dim, bits, _dp(X1), _dp(X2), eta,
-                                  max_level, c_leaf, root_level_1, root_level_2)
+        self._h = lib().orc_build_ex(self.n1, self.n2, self.dim, bits, _dp(X1), _dp(X2), eta,
+                                     max_level, c_leaf, root_level_1, root_level_2, int(bool(exact_boxes)))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -328,10 +333,13 @@ class HMatrix:
         return y[:, 0] if R == 1 else y
 
     def hmvp_sample(self, kern, stride, x):
-        x = np.ascontiguousarray(x, dtype=np.float64)
-        y = np.zeros(self.n1)
-        e = lib().orc_hmvp_sample(self._h, ctypes.byref(kern), stride, _dp(x), _dp(y))
-        return y, e
+        """H-MVP over every stride-th block (R = columns of x); returns (y, entries
+        touched) -- times a bounded CPU sample."""
+        x = np.asfortranarray(np.asarray(x, dtype=np.float64).reshape(self.n2, -1))
+        R = x.shape[1]
+        y = np.zeros((self.n1, R), order="F")
+        e = lib().orc_hmvp_sample(self._h, ctypes.byref(kern), stride, R, _dp(x), _dp(y))
+        return (y[:, 0] if R == 1 else y), e
 
     def dense_mvp(self, kern, x):
         x = np.asfortranarray(np.asarray(x, dtype=np.float64).reshape(self.n2, -1))

@@ -183,6 +183,7 @@ struct orc_hm {
     int nblocks, ndense, naca;
     int levels;
     int64_t diag[4];
+    int exact_boxes;       /* 1: cluster boxes from the points themselves (no lookup table) */
     /* ACA factors (ACA blocks only, index a = block - ndense) */
     int* kb;
     int* keff;
@@ -321,7 +322,7 @@ static void level_boxes(orc_hm* hm, orc_work_item* nodes, int nn, int set)
         for (int d = 0; d < dim; d++) {
             bmin[d] = tmin[(int64_t)rank * dim + d];
             bmax[d] = tmax[(int64_t)rank * dim + d];
-            /* exact box for the diagnostic */
+            /* exact box: the diagnostic, or the box itself in exact mode */
             if (l >= 0 && u < N && l <= u) {
                 double elo = X[(int64_t)d * N + l], ehi = elo;
                 for (int p = l + 1; p <= u; p++) {
@@ -330,6 +331,10 @@ static void level_boxes(orc_hm* hm, orc_work_item* nodes, int nn, int set)
                     if (ehi < v) ehi = v;
                 }
                 if (elo != bmin[d] || ehi != bmax[d]) differs = 1;
+                if (hm->exact_boxes) {
+                    bmin[d] = elo;
+                    bmax[d] = ehi;
+                }
             }
         }
         nodes[i].dim = dim;
@@ -440,7 +445,14 @@ static void traverse(orc_hm* hm, int root_level_1, int root_level_2)
 orc_hm* orc_build(int n1, int n2, int dim, int bits, const double* X1, const double* X2,
                   double eta, int max_level, int c_leaf, int root_level_1, int root_level_2)
 {
+    return orc_build_ex(n1, n2, dim, bits, X1, X2, eta, max_level, c_leaf, root_level_1, root_level_2, 0);
+}
+
+orc_hm* orc_build_ex(int n1, int n2, int dim, int bits, const double* X1, const double* X2, double eta,
+                     int max_level, int c_leaf, int root_level_1, int root_level_2, int exact_boxes)
+{
     orc_hm* hm = (orc_hm*)calloc(1, sizeof(orc_hm));
+    hm->exact_boxes = exact_boxes;
     hm->n[0] = n1; hm->n[1] = n2;
     hm->dim = dim; hm->bits = bits;
     hm->eta = eta; hm->max_level = max_level; hm->c_leaf = c_leaf;
@@ -757,6 +769,10 @@ static int64_t hmvp_impl(const orc_hm* hm, const orc_kernel* kern, int R, const 
     double* xs = (double*)malloc(sizeof(double) * n2 * R);
     for (int c = 0; c < R; c++)
         for (int64_t p = 0; p < n2; p++) xs[c * n2 + p] = x[c * n2 + (int64_t)hm->order[1][p]];
+    /* the near field reads the right-hand sides of one point together */
+    double* xr = (double*)malloc(sizeof(double) * n2 * R);
+    for (int64_t p = 0; p < n2; p++)
+        for (int c = 0; c < R; c++) xr[p * R + c] = xs[c * n2 + p];
     int nth = 1;
 #ifdef _OPENMP
     nth = omp_get_max_threads();
@@ -771,6 +787,7 @@ static int64_t hmvp_impl(const orc_hm* hm, const orc_kernel* kern, int R, const 
 #endif
         double* yl = ys + (int64_t)tid * n1 * R;
         double* tmp = (double*)malloc(sizeof(double) * (hm->kmax > 0 ? hm->kmax : 1) * R);
+        double* acc = (double*)malloc(sizeof(double) * R);
 #pragma omp for schedule(dynamic, 8)
         for (int b = 0; b < hm->nblocks; b++) {
             const orc_work_item* w = &hm->blocks[b];
@@ -778,13 +795,16 @@ static int64_t hmvp_impl(const orc_hm* hm, const orc_kernel* kern, int R, const 
             const int m1 = w->set1_u - l1 + 1, m2 = w->set2_u - l2 + 1;
             if (w->work_type == ORC_WT_DENSE) {
                 if (stride > 1 && (b % stride) != 0) continue;
+                /* each entry evaluated once and applied to every right-hand side;
+                 * every (row, rhs) sum still runs over the columns in order */
                 for (int i = 0; i < m1; i++) {
-                    for (int c = 0; c < R; c++) {
-                        double acc = 0.0;
-                        for (int j = 0; j < m2; j++)
-                            acc = fma(kernel_eval(hm, kern, l1 + i, l2 + j), xs[c * n2 + l2 + j], acc);
-                        yl[c * n1 + l1 + i] += acc;
+                    for (int c = 0; c < R; c++) acc[c] = 0.0;
+                    for (int j = 0; j < m2; j++) {
+                        const double kv = kernel_eval(hm, kern, l1 + i, l2 + j);
+                        const double* xj = xr + (int64_t)(l2 + j) * R;
+                        for (int c = 0; c < R; c++) acc[c] = fma(kv, xj[c], acc[c]);
                     }
+                    for (int c = 0; c < R; c++) yl[c * n1 + l1 + i] += acc[c];
                 }
                 touched += (int64_t)m1 * m2;
             } else {
@@ -809,6 +829,7 @@ static int64_t hmvp_impl(const orc_hm* hm, const orc_kernel* kern, int R, const 
             }
         }
         free(tmp);
+        free(acc);
     }
     for (int c = 0; c < R; c++)
         for (int64_t s = 0; s < n1; s++) {
@@ -817,6 +838,7 @@ static int64_t hmvp_impl(const orc_hm* hm, const orc_kernel* kern, int R, const 
             y[c * n1 + (int64_t)hm->order[0][s]] = acc;
         }
     free(ys);
+    free(xr);
     free(xs);
     return touched;
 }
@@ -826,9 +848,9 @@ void orc_hmvp(const orc_hm* hm, const orc_kernel* kern, int R, const double* x, 
     hmvp_impl(hm, kern, R, x, y, 1);
 }
 
-int64_t orc_hmvp_sample(const orc_hm* hm, const orc_kernel* kern, int stride, const double* x, double* y)
+int64_t orc_hmvp_sample(const orc_hm* hm, const orc_kernel* kern, int stride, int R, const double* x, double* y)
 {
-    return hmvp_impl(hm, kern, 1, x, y, stride);
+    return hmvp_impl(hm, kern, R, x, y, stride);
 }
 
 /* apply_full_mvp (hmglib.cu:351-401): dense product in sorted space */

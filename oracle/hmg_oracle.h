@@ -87,6 +87,12 @@ int orc_find_split(const uint64_t* codes, int first, int last);
  * partition (hmglib.cu:125-225).  X1/X2 SoA in original order. */
 orc_hm* orc_build(int n1, int n2, int dim, int bits, const double* X1, const double* X2,
                   double eta, int max_level, int c_leaf, int root_level_1, int root_level_2);
+/* orc_build with exact_boxes = 1: every cluster's box is the exact min/max of its
+ * points (no lookup table, so a one-point cluster shifts nothing; the engine's
+ * HMG_OPT_EXACT_BOXES).  Not the reference's behaviour where a level holds a
+ * one-point cluster (tree.cu:1000-1024); identical to it elsewhere. */
+orc_hm* orc_build_ex(int n1, int n2, int dim, int bits, const double* X1, const double* X2, double eta,
+                     int max_level, int c_leaf, int root_level_1, int root_level_2, int exact_boxes);
 void orc_free(orc_hm* hm);
 
 int orc_block_count(const orc_hm* hm);
@@ -135,7 +141,7 @@ void orc_get_aca(const orc_hm* hm, int* kb, int* keff, int64_t* uoff, int64_t* v
 void orc_hmvp(const orc_hm* hm, const orc_kernel* kern, int R, const double* x, double* y);
 /* H-MVP restricted to a subset of ACA/DENSE blocks (stride sampling) -- used only to
  * time a bounded CPU sample. Returns the number of matrix entries touched. */
-int64_t orc_hmvp_sample(const orc_hm* hm, const orc_kernel* kern, int stride,
+int64_t orc_hmvp_sample(const orc_hm* hm, const orc_kernel* kern, int stride, int R,
                         const double* x, double* y);
 int64_t orc_aca_sample(orc_hm* hm, const orc_kernel* kern, int k, int stride);
 /* Dense FP64 product in sorted space, then un-permuted (hmglib.cu:351-401). */

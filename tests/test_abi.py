@@ -85,7 +85,7 @@ def test_invalid_arguments_are_rejected_without_touching_the_gpu():
     assert L.hmg_apply_full_mvp(None, None, ctypes.byref(z)) == _abi.HMG_ERR_NOT_READY
     out = (ctypes.c_int64 * 6)()
     assert L.hmg_get_work(ctypes.byref(z), out) == _abi.HMG_ERR_NOT_READY
-    tuned = (ctypes.c_int64 * 3)()
+    tuned = (ctypes.c_int64 * 4)()
     assert L.hmg_autotune_product(ctypes.byref(z), None, None, 1, 1, tuned) == _abi.HMG_ERR_NOT_READY
     assert L.hmg_destroy_h_matrix_data(ctypes.byref(z)) == _abi.HMG_OK
     assert not L.hmg_kernel_new(99, 0.0, 1.0)

@@ -152,8 +152,9 @@ C5_CASES = [
 ]
 
 
+@pytest.mark.parametrize("exact", [False, True], ids=["reference_boxes", "exact_boxes"])
 @pytest.mark.parametrize("prec_name", ["mixed", "fp64"])
-def test_c5_served_results_against_oracle(gpu, prec_name):
+def test_c5_served_results_against_oracle(gpu, prec_name, exact):
     from hmglib_amd import hip
     from hmglib_amd.server import Server
     prec = gpu.PREC_MIXED if prec_name == "mixed" else gpu.PREC_FP64
@@ -164,7 +165,7 @@ def test_c5_served_results_against_oracle(gpu, prec_name):
             rng = np.random.default_rng(seed)
             X = _blob_cloud(rng, n, dim) if blobs else rng.random((dim, n))
             bits, leaf, eta, k = (32, 64, 1.5, 32) if dim == 2 else (20, 256, 1.0, 24)
-            rb, m = srv.submit_build(X, X, bits, eta, 50, leaf, k, kind, 0.0, 0.1)
+            rb, m = srv.submit_build(X, X, bits, eta, 50, leaf, k, kind, 0.0, 0.1, exact_boxes=exact)
             mats.append((X, bits, leaf, eta, k, kind, dim, n, rb, m))
         rng = np.random.default_rng(55)
         reqs = []
@@ -180,7 +181,7 @@ def test_c5_served_results_against_oracle(gpu, prec_name):
             assert srv.wait(rb)["status"] == gpu.HMG_OK
         oracles = []
         for X, bits, leaf, eta, k, kind, dim, n, rb, m in mats:
-            om = O.HMatrix(X, X, bits, eta, 50, leaf)
+            om = O.HMatrix(X, X, bits, eta, 50, leaf, exact_boxes=exact)
             ok = O.kernel(kind, dim, 0.0, 0.1)
             om.aca(ok, k)
             oracles.append((om, ok))
@@ -197,6 +198,11 @@ def test_c5_served_results_against_oracle(gpu, prec_name):
                     # the BF16x3 arithmetic bound, relative to sum_j K_ij |x_j| (positive kernels)
                     mag = np.linalg.norm(om.hmvp(ok, np.abs(Xh[:, c])))
                     assert np.linalg.norm(Y[:, c] - Yo[:, c]) / mag < 4e-5
+            if exact:
+                # exact boxes: the served result is within 1e-4 of the dense product
+                Yd = om.dense_mvp(ok, Xh[:, :2]).reshape(n, -1)
+                for c in range(Yd.shape[1]):
+                    assert _rel(Y[:, c], Yd[:, c]) < 1e-4
         assert srv.stats()["failed"] == 0
     finally:
         srv.destroy()
@@ -252,3 +258,44 @@ def test_products_with_one_point_clusters(gpu, prec_name, case):
             assert np.linalg.norm(Y[:, c] - Yo[:, c]) / mag < 4e-5
         assert _rel(Yo[:, c], yd) < 1e-6
         assert _rel(Y[:, c], yd) < (1e-6 if prec_name == "fp64" else 1e-4)
+
+
+# ------------------------------------------------------------ exact cluster boxes
+@pytest.mark.parametrize("prec_name", ["fp64", "mixed"])
+def test_exact_boxes_option_against_oracle(gpu, prec_name):
+    """HMG_OPT_EXACT_BOXES on 8-blob clouds whose levels hold one-point clusters
+    (where the reference's lookup table shifts boxes): the block list, boxes
+    included, bit-exact against the oracle's exact-box mode, which the CPU tests pin
+    with a brute-force traversal; the product against the oracle's H-MVP and the
+    dense product (the literal boxes admit unseparated blocks on these clouds and
+    miss the dense product by up to 0.6)."""
+    prec = gpu.PREC_MIXED if prec_name == "mixed" else gpu.PREC_FP64
+    for seed, n, dim in ((5204, 5204, 2), (4562, 4562, 3)):
+        rng = np.random.default_rng(seed)
+        X = _blob_cloud(rng, n, dim)
+        bits, leaf, eta, k, kind = (32, 16, 1.5, 32, O.K_EXPONENTIAL) if dim == 2 else (20, 32, 1.0, 24, O.K_GAUSSIAN)
+        lit = O.HMatrix(X, X, bits, eta, 50, leaf)
+        assert lit.diagnostics()["box_differs"] > 0
+        om = O.HMatrix(X, X, bits, eta, 50, leaf, exact_boxes=True)
+        hm = gpu.HMatrix(X, X, bits, eta, 50, leaf, k)
+        hm.set_option(gpu.OPT_EXACT_BOXES, 1)
+        hm.setup()
+        g, o = hm.blocks(O.WORK_ITEM_DTYPE), om.blocks()
+        assert len(g) == len(o)
+        for f in ("set1_l", "set1_u", "set2_l", "set2_u", "work_type", "level_1", "min1", "max1", "min2", "max2"):
+            np.testing.assert_array_equal(g[f], o[f])
+        kern = gpu.Kernel(kind, 0.0, 0.1)
+        hm.set_kernel(kern)
+        hm.set_precision(prec)
+        hm.precompute_aca()
+        ok = O.kernel(kind, dim, 0.0, 0.1)
+        om.aca(ok, k)
+        x = O.normal(seed + 1, n)
+        y = hm.apply(x)
+        y_o = om.hmvp(ok, x)
+        y_d = om.dense_mvp(ok, x)
+        assert _rel(y_o, y_d) < 1e-4
+        assert _rel(y, y_o) < (1e-6 if prec_name == "fp64" else 1e-4)
+        assert _rel(y, y_d) < (1e-4 if prec_name == "fp64" else 2e-4)
+        hm.destroy()
+        kern.free()

@@ -215,8 +215,11 @@ def test_autotune_keeps_the_product(gpu, prec_name):
     hm.apply_dev(xd, yd, R)
     y0 = yd.download()
     t = hm.autotune(xd, yd, R, reps=1)
-    assert t["near_lds_bytes"] in (0, 32768, 49152, 65536, 81920) and t["us_per_product"] > 0
+    assert t["near_lds_bytes"] in (0, 16384, 24576, 36864, 49152, 65536, 81920) and t["us_per_product"] > 0
     assert not (t["one_stream"] and t["near_lds_bytes"])
+    if prec_name == "fp64":
+        # the FP64 product has one form of each; the options are left on
+        assert t["lr_workgroup"] and t["near_workgroup"]
     hm.apply_dev(xd, yd, R)
     np.testing.assert_array_equal(yd.download(), y0)
     xd.free()
@@ -252,3 +255,41 @@ def test_switching_precision_back_to_fp64(gpu):
     hmx.precompute_aca()
     hmx.apply_without_batching(x)
     np.testing.assert_array_equal(hmx.apply(x), y_mx)
+
+
+@pytest.mark.parametrize("case", ["c1_2d_gauss", "chunked_2d_eta4", "c3_like_3d_exp"])
+def test_lr_workgroup_form_bitwise_equal(gpu, case):
+    """The workgroup forms of V^T X, U T and the near field (X / T panel groups
+    staged once per workgroup through LDS) run the same arithmetic per output as the
+    one-wavefront kernels: the products are bitwise equal.  The eta-4 case has
+    column clusters of more than 4096 points (several point chunks per slice:
+    partial sums and their reduction)."""
+    if case == "c1_2d_gauss":
+        n, dim, eta, leaf, k, kind, ls = 4096, 2, 1.5, 64, 32, O.K_GAUSSIAN, 0.1
+    elif case == "chunked_2d_eta4":
+        n, dim, eta, leaf, k, kind, ls = 50000, 2, 4.0, 64, 16, O.K_EXPONENTIAL, 0.2
+    else:
+        n, dim, eta, leaf, k, kind, ls = 40000, 3, 1.0, 256, 24, O.K_EXPONENTIAL, 0.1
+    X = _points(1201, dim, n)
+    hm, kern = _build(gpu, X, X, 32 if dim == 2 else 20, eta, leaf, k, kind, ls)
+    if case == "chunked_2d_eta4":
+        b = hm.blocks(O.WORK_ITEM_DTYPE)
+        aca = b[b["work_type"] == O.WT_ACA]
+        assert (aca["set2_u"] - aca["set2_l"] + 1).max() > 4096
+    R = 32
+    Xr = O.normal(1203, n * R).reshape(n, R, order="F")
+    forms = {}
+    for lr_wg, near_wg in ((1, 1), (0, 0), (1, 0), (0, 1)):
+        hm.set_option(gpu.OPT_LR_WORKGROUP, lr_wg)
+        hm.set_option(gpu.OPT_NEAR_WORKGROUP, near_wg)
+        forms[(lr_wg, near_wg)] = hm.apply(Xr)
+    Y_wg = forms[(1, 1)]
+    for Y in forms.values():
+        np.testing.assert_array_equal(Y, Y_wg)
+    # and both are the H-matrix's product (FP64 H-MVP of the same tree and ranks)
+    hm64, k64 = _build(gpu, X, X, 32 if dim == 2 else 20, eta, leaf, k, kind, ls, prec=gpu.PREC_FP64)
+    _check_columns(Y_wg, hm64.apply(Xr), TOL_MIXED_VS_FP64)
+    hm.destroy()
+    kern.free()
+    hm64.destroy()
+    k64.free()

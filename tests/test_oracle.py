@@ -354,6 +354,38 @@ def test_quirk_literal_boxes_match_brute_force(n, c_leaf, eta, seed):
         np.testing.assert_array_equal(w["max2"][:2], r[7][1])
 
 
+@pytest.mark.parametrize("n,c_leaf,eta,seed", [(400, 2, 1.2, 91), (600, 1, 1.0, 92), (500, 3, 1.5, 93)])
+def test_exact_boxes_mode_matches_brute_force(n, c_leaf, eta, seed):
+    """The exact-box mode (a cluster's box is the min/max of its own points, no
+    lookup table) on the clouds where the literal mode shifts boxes: the whole
+    block list, boxes included, against the brute-force traversal with exact boxes;
+    every ACA block admissible and every DENSE block small or inadmissible under
+    exact boxes; and the block list differs from the literal mode's."""
+    X = O.uniform(seed, 2 * n).reshape(2, n) ** 3
+    hm = O.HMatrix(X, X, 32, eta, 50, c_leaf, exact_boxes=True)
+    lit = O.HMatrix(X, X, 32, eta, 50, c_leaf)
+    assert lit.diagnostics()["box_differs"] > 0
+    ref = _tree_brute(hm.sorted_coords(0), hm.sorted_coords(1), hm.codes(0), hm.codes(1), eta, 50, c_leaf)
+    got = hm.blocks()
+    got_t = [(int(b["set1_l"]), int(b["set1_u"]), int(b["set2_l"]), int(b["set2_u"]),
+              int(b["work_type"]), int(b["level_1"]), int(b["level_2"])) for b in got]
+    assert got_t == ref
+    Xs = hm.sorted_coords(0)
+    for w in got:
+        l1, u1, l2, u2 = (int(w[f]) for f in ("set1_l", "set1_u", "set2_l", "set2_u"))
+        b1 = (Xs[:, l1:u1 + 1].min(1), Xs[:, l1:u1 + 1].max(1))
+        b2 = (Xs[:, l2:u2 + 1].min(1), Xs[:, l2:u2 + 1].max(1))
+        np.testing.assert_array_equal(w["min1"][:2], b1[0])
+        np.testing.assert_array_equal(w["max1"][:2], b1[1])
+        np.testing.assert_array_equal(w["min2"][:2], b2[0])
+        np.testing.assert_array_equal(w["max2"][:2], b2[1])
+        if w["work_type"] == O.WT_ACA:
+            assert _admissible(b1, b2, eta)
+    lb = lit.blocks()
+    assert len(lb) != len(got) or not np.array_equal(lb["work_type"], got["work_type"]) or \
+        not np.array_equal(lb["min1"], got["min1"])
+
+
 def test_quirk_hand_worked_example():
     """tree.cu:1000-1024 with a singleton cluster t=0 = [0,0] and t=1 = [1,N-1]:
     keys[0] = +1 then -1 (same thread); keys[1] = +2; keys[N-1] = -2; inclusive

@@ -55,8 +55,15 @@ def test_noalloc_product_equals_allocating_product_twice(gpu):
         assert st == gpu.HMG_OK, Y
         # the no-alloc product took no buffer of its own from the engine
         assert hm.memory()[0] == held
+        # bitwise the allocating entry point's one-right-hand-side-at-a-time path ...
+        hm.set_option(gpu.OPT_FP64_BLOCK_RHS, 1 << 20)
         Yref = hm.apply(X)
         np.testing.assert_array_equal(Y, Yref)
+        # ... and within rounding of its block path (the default for several right-hand sides)
+        hm.set_option(gpu.OPT_FP64_BLOCK_RHS, 2)
+        Yblk = hm.apply(X)
+        for c in range(nrhs):
+            assert np.linalg.norm(Y[:, c] - Yblk[:, c]) / np.linalg.norm(Yblk[:, c]) < 1e-12
         # and the product is the H-matrix's (oracle H-MVP of the same tree and ranks)
         om = O.HMatrix(O.uniform(seed, dim * n).reshape(dim, n), O.uniform(seed, dim * n).reshape(dim, n),
                        32 if dim == 2 else 20, eta, 50, c_leaf)
