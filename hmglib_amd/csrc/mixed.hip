@@ -214,34 +214,48 @@ near_mx_kernel(const float* __restrict__ cp1, const float* __restrict__ cp2, Ker
     for (int t = 0; t < kMxRowTiles; t++)
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[t][r] = 0.f;
-    for (int rf = it.ref0; rf < it.ref1; rf++) {
+    // steps (ref rf, first group g) two groups at a time; the next step's X panel
+    // values load while this step's entries are evaluated (past the last step the
+    // loads repeat its groups)
+    int rf = it.ref0;
+    int g = refs[rf].g0;
+    bf16x8 nbh = Xp[((int64_t)(g + h) * 2 + 0) * kMxRhs + c32];
+    bf16x8 nbl = Xp[((int64_t)(g + h) * 2 + 1) * kMxRhs + c32];
+    for (;;) {
         const MxRef f = refs[rf];
-        for (int g = f.g0; g < f.g1; g += 2) {
-            const int gg = g + h;
-            bf16x8 bh = Xp[((int64_t)gg * 2 + 0) * kMxRhs + c32];
-            bf16x8 bl = Xp[((int64_t)gg * 2 + 1) * kMxRhs + c32];
-            const int col0 = gg * kGroup;
-            if (col0 < f.lo || col0 + kGroup > f.hi) {
+        const int gg = g + h;
+        bf16x8 bh = nbh, bl = nbl;
+        int rfn = rf, gn = g + 2;
+        if (gn >= f.g1) {
+            rfn = rf + 1;
+            gn = rfn < it.ref1 ? refs[rfn].g0 : g;
+        }
+        nbh = Xp[((int64_t)(gn + h) * 2 + 0) * kMxRhs + c32];
+        nbl = Xp[((int64_t)(gn + h) * 2 + 1) * kMxRhs + c32];
+        const int col0 = gg * kGroup;
+        if (col0 < f.lo || col0 + kGroup > f.hi) {
 #pragma unroll
-                for (int e = 0; e < kGroup; e++) {
-                    if (col0 + e < f.lo || col0 + e >= f.hi) {
-                        bh[e] = (__bf16)0.f;
-                        bl[e] = (__bf16)0.f;
-                    }
+            for (int e = 0; e < kGroup; e++) {
+                if (col0 + e < f.lo || col0 + e >= f.hi) {
+                    bh[e] = (__bf16)0.f;
+                    bl[e] = (__bf16)0.f;
                 }
             }
-            f32x8 xc[DM];
-#pragma unroll
-            for (int d = 0; d < DM; d++) {
-                if (DIM > 0 || d < dim)
-                    xc[d] = *(const f32x8*)(cp2 + ((int64_t)gg * dim + d) * kGroup);
-                else
-                    xc[d] = f32x8{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            }
-#pragma unroll
-            for (int t = 0; t < kMxRowTiles; t++)
-                if (t < ntile) acc[t] = near_tile<KIND, DIM, REG>(kp, dim, xr[t], xc, row[t], col0, bh, bl, acc[t]);
         }
+        f32x8 xc[DM];
+#pragma unroll
+        for (int d = 0; d < DM; d++) {
+            if (DIM > 0 || d < dim)
+                xc[d] = *(const f32x8*)(cp2 + ((int64_t)gg * dim + d) * kGroup);
+            else
+                xc[d] = f32x8{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+        for (int t = 0; t < kMxRowTiles; t++)
+            if (t < ntile) acc[t] = near_tile<KIND, DIM, REG>(kp, dim, xr[t], xc, row[t], col0, bh, bl, acc[t]);
+        if (rfn >= it.ref1) break;
+        rf = rfn;
+        g = gn;
     }
     float* __restrict__ out = P + (int64_t)q * kMxRowsItem * kMxRhs + c32;
 #pragma unroll

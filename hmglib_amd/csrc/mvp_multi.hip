@@ -5,18 +5,17 @@
 // linear_algebra.cu:4292-4379, 1429-1591, 2817-3137; the same decomposition):
 //   gather     X (caller order, column-major) -> Xs[p * RB + c], sorted points,
 //              one RB-wide row per point (64 points per workgroup through LDS)
-//   lr_v       Tp = V_D^T Xs per (column cluster, 64 columns, <= 1024 points):
-//              lane = column, RB accumulators; the point's Xs row is uniform over
-//              the wavefront (scalar loads)
-//   lr_u       P = U_C T_C per 64 rows of a row cluster: lane = row
-//   near       the near field evaluated on the fly, row form: lane = row, the
-//              column uniform over the wavefront, each entry evaluated once for
-//              all RB right-hand sides
+//   lr_v       Tp = V_D^T Xs per (column cluster, 64 columns, <= 1024 points)
+//   lr_u       P = U_C T_C per 64 rows of a row cluster
+//   near       the near field evaluated on the fly, each entry once for all the
+//              pass's right-hand sides
+//              all three as FP64 MFMA GEMMs (v_mfma_f64_16x16x4_f64, right-hand
+//              sides as the N dimension, 16 or 32 per pass)
 //   combine    every row adds its partials in the fixed order of the row's list,
 //              then the sorted rows go back to the caller's order through LDS
-// Every right-hand side's arithmetic is independent of RB and of the other
-// columns in the pass (one FMA chain per column, in point / rank / column order),
-// so a column's result is bitwise the same in every pass width.
+// An MFMA's output column depends on its own B column only, and the reduction
+// steps run in a fixed order, so a right-hand side's result is bitwise the same
+// in every pass width and whatever other columns share the pass.
 #include "hmg_internal.h"
 #include "kernel_eval.h"
 
@@ -51,44 +50,66 @@ gather_rows_kernel(const double* __restrict__ X, const uint64_t* __restrict__ or
     }
 }
 
-// Tp[(q*64 + lane)*RB + c] = sum_{j < nrows} V[vbase + lane + j*ld] * Xs[(j0 + j)*RB + c]
+// FP64 MFMA v_mfma_f64_16x16x4_f64 (wave64): for lane l, i = l % 16, g = l / 16:
+//   A[i][g] (16 x 4), B[g][i] (4 x 16), D[4r + g][i] in result register r < 4 (16 x 16)
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma64(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Tp[(q*64 + col)*RB + c] = sum_{j < nrows} V[vbase + col + j*ld] * Xs[(j0 + j)*RB + c]:
+// M = the item's 64 columns (four 16-column tiles), N = right-hand sides (RB / 16
+// tiles), K = points, four per step.  Points past nrows enter as zeros.
 template <int RB>
 __global__ void __launch_bounds__(kThreads)
 lr_v_multi_kernel(const LvItem* __restrict__ items, int nitems, const double* __restrict__ V,
                   const double* __restrict__ Xs, double* __restrict__ Tp)
 {
+    constexpr int NB = RB / 16;
     const int lane = threadIdx.x & (kWave - 1);
     const int q = blockIdx.x * kWaves + wave_id();
     if (q >= nitems) return;
     const LvItem it = items[q];
-    const double* __restrict__ v = V + it.vbase + (lane < it.ncols ? lane : 0);
-    const double* __restrict__ xb = Xs + (int64_t)it.j0 * RB;
+    const int i16 = lane & 15, g = lane >> 4;
     const int64_t ld = it.ld;
-    double acc[RB];
+    const double* __restrict__ v = V + it.vbase + i16;
+    const double* __restrict__ xb = Xs + (int64_t)it.j0 * RB + i16;
+    d4 acc[4][NB];
 #pragma unroll
-    for (int c = 0; c < RB; c++) acc[c] = 0.0;
-    int j = 0;
-    for (; j + 4 <= it.nrows; j += 4) {
-        const double v0 = v[(j + 0) * ld], v1 = v[(j + 1) * ld], v2 = v[(j + 2) * ld], v3 = v[(j + 3) * ld];
-        const double* __restrict__ x = xb + (int64_t)j * RB;
+    for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(v0, x[c], acc[c]);
+        for (int h = 0; h < NB; h++) acc[t][h] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int j0 = 0; j0 < it.nrows; j0 += 4) {
+        const int pt = j0 + g;
+        const bool ok = pt < it.nrows;
+        const int pc = ok ? pt : it.nrows - 1;  // loads stay on the item's rows
+        double a[4], b[NB];
 #pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(v1, x[RB + c], acc[c]);
+        for (int t = 0; t < 4; t++) {
+            const double x = v[pc * ld + 16 * t];
+            a[t] = ok ? x : 0.0;
+        }
 #pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(v2, x[2 * RB + c], acc[c]);
+        for (int h = 0; h < NB; h++) {
+            const double x = xb[(int64_t)pc * RB + 16 * h];
+            b[h] = ok ? x : 0.0;
+        }
 #pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(v3, x[3 * RB + c], acc[c]);
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int h = 0; h < NB; h++) acc[t][h] = mfma64(a[t], b[h], acc[t][h]);
     }
-    for (; j < it.nrows; j++) {
-        const double vj = v[j * ld];
-        const double* __restrict__ x = xb + (int64_t)j * RB;
 #pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(vj, x[c], acc[c]);
-    }
-    double* __restrict__ out = Tp + ((int64_t)q * kTileRows + lane) * RB;
+    for (int t = 0; t < 4; t++)
 #pragma unroll
-    for (int c = 0; c < RB; c++) out[c] = acc[c];
+        for (int h = 0; h < NB; h++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int col = 16 * t + 4 * r + g;
+                Tp[((int64_t)q * kTileRows + col) * RB + 16 * h + i16] = acc[t][h][r];
+            }
 }
 
 // T[tmap[tcol0 + lane]*RB + c] = sum of the slice's point chunks, in chunk order
@@ -115,95 +136,141 @@ lr_v_finalize_multi_kernel(const LvSlice* __restrict__ slices, int nslices, cons
     for (int c = 0; c < RB; c++) t[c] = s[c];
 }
 
-// P[(q*64 + lane)*RB + c] = sum_l U[uoff + lane + l*ld] * T[(toff + l)*RB + c]
+// P[(q*64 + row)*RB + c] = sum_l U[uoff + row + l*ld] * T[(toff + l)*RB + c]:
+// M = the item's 64 rows, N = right-hand sides, K = rank columns, four per step
 template <int RB>
 __global__ void __launch_bounds__(kThreads)
 lr_u_multi_kernel(const LrItem* __restrict__ items, int nitems, const double* __restrict__ U,
                   const double* __restrict__ T, double* __restrict__ P)
 {
+    constexpr int NB = RB / 16;
     const int lane = threadIdx.x & (kWave - 1);
     const int q = blockIdx.x * kWaves + wave_id();
     if (q >= nitems) return;
     const LrItem it = items[q];
-    const bool act = lane < it.nrows;
-    const double* __restrict__ u = U + it.uoff + (act ? lane : 0);
-    const double* __restrict__ tb = T + it.toff * RB;
+    const int i16 = lane & 15, g = lane >> 4;
     const int64_t ld = it.ld;
-    double acc[RB];
+    const double* __restrict__ u = U + it.uoff + i16;
+    const double* __restrict__ tb = T + it.toff * RB + i16;
+    d4 acc[4][NB];
 #pragma unroll
-    for (int c = 0; c < RB; c++) acc[c] = 0.0;
-    int l = 0;
-    for (; l + 4 <= it.K; l += 4) {
-        const double u0 = u[(l + 0) * ld], u1 = u[(l + 1) * ld], u2 = u[(l + 2) * ld], u3 = u[(l + 3) * ld];
-        const double* __restrict__ t = tb + (int64_t)l * RB;
+    for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(u0, t[c], acc[c]);
+        for (int h = 0; h < NB; h++) acc[t][h] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int l0 = 0; l0 < it.K; l0 += 4) {
+        const int l = l0 + g;
+        const bool ok = l < it.K;
+        const int lc = ok ? l : it.K - 1;  // loads stay on the cluster's rank columns
+        double a[4], b[NB];
 #pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(u1, t[RB + c], acc[c]);
+        for (int t = 0; t < 4; t++) {
+            const double x = u[lc * ld + 16 * t];
+            a[t] = ok ? x : 0.0;
+        }
 #pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(u2, t[2 * RB + c], acc[c]);
+        for (int h = 0; h < NB; h++) {
+            const double x = tb[(int64_t)lc * RB + 16 * h];
+            b[h] = ok ? x : 0.0;
+        }
 #pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(u3, t[3 * RB + c], acc[c]);
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int h = 0; h < NB; h++) acc[t][h] = mfma64(a[t], b[h], acc[t][h]);
     }
-    for (; l < it.K; l++) {
-        const double ul = u[l * ld];
-        const double* __restrict__ t = tb + (int64_t)l * RB;
 #pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = __fma_rn(ul, t[c], acc[c]);
-    }
-    if (!act) return;
-    double* __restrict__ out = P + ((int64_t)q * kTileRows + lane) * RB;
+    for (int t = 0; t < 4; t++)
 #pragma unroll
-    for (int c = 0; c < RB; c++) out[c] = acc[c];
+        for (int r = 0; r < 4; r++) {
+            const int row = 16 * t + 4 * r + g;
+            if (row < it.nrows) {
+#pragma unroll
+                for (int h = 0; h < NB; h++) P[((int64_t)q * kTileRows + row) * RB + 16 * h + i16] = acc[t][h][r];
+            }
+        }
 }
 
-// Near field on the fly, row form, over the stored-near-field schedule
-// (items {row0, nrows <= 128, b0, b1} over near_blk): lane = row (two passes of 64
-// rows), the item's dense blocks' columns in block order, each entry evaluated
-// once and applied to all RB right-hand sides.  P[(q*128 + r)*RB + c].
+// Near field on the fly over the stored-near-field schedule (items {row0, nrows <=
+// 128, b0, b1} over near_blk), as FP64 MFMA GEMMs: M = 64 rows per pass (four
+// 16-row tiles; two passes for 128 rows), N = right-hand sides, K = the item's
+// dense blocks' columns in block order, four per step.  Lane l evaluates the
+// entries it feeds as A operand: rows 16t + l % 16 against column l / 16 of the
+// step (each entry evaluated once for all right-hand sides).  Columns past a
+// block's end enter as zeros.  P[(q*128 + r)*RB + c].
 template <int KIND, int DIM, int RB>
 __global__ void __launch_bounds__(kThreads)
 near_multi_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ items, int nitems,
                   const int* __restrict__ near_blk, const int4* __restrict__ blk, const double* __restrict__ Xs,
                   double* __restrict__ P)
 {
+    constexpr int NB = RB / 16;
     const int lane = threadIdx.x & (kWave - 1);
     const int q = blockIdx.x * kWaves + wave_id();
     if (q >= nitems) return;
     const int4 it = items[q];
     const int dim = DIM > 0 ? DIM : kp.dim;
+    const int i16 = lane & 15, g = lane >> 4;
     for (int half = 0; half * kWave < it.y; half++) {
-        const bool act = lane + half * kWave < it.y;
-        const int row = it.x + (act ? lane + half * kWave : 0);
-        double xi[MAX_POINT_DIMENSION];
+        int row[4];
+        double xi[4][MAX_POINT_DIMENSION];
 #pragma unroll
-        for (int d = 0; d < MAX_POINT_DIMENSION; d++) xi[d] = d < dim ? c1.x[d][row] : 0.0;
-        double acc[RB];
+        for (int t = 0; t < 4; t++) {
+            const int rr = half * kWave + 16 * t + i16;
+            row[t] = it.x + (rr < it.y ? rr : 0);  // rows past the item: its first row
 #pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = 0.0;
+            for (int d = 0; d < MAX_POINT_DIMENSION; d++) xi[t][d] = d < dim ? c1.x[d][row[t]] : 0.0;
+        }
+        d4 acc[4][NB];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int h = 0; h < NB; h++) acc[t][h] = d4{0.0, 0.0, 0.0, 0.0};
         for (int bi = it.z; bi < it.w; bi++) {
             const int4 b = blk[near_blk[bi]];
-            for (int col = b.z; col <= b.w; col++) {
-                double r2 = 0.0;
+            for (int j0 = b.z; j0 <= b.w; j0 += 4) {
+                const int col = j0 + g;
+                const bool ok = col <= b.w;
+                const int cc = ok ? col : b.w;
+                double xj[MAX_POINT_DIMENSION];
 #pragma unroll
-                for (int d = 0; d < MAX_POINT_DIMENSION; d++) {
-                    if (d < dim) {
-                        const double t = xi[d] - c2.x[d][col];
-                        r2 = __fma_rn(t, t, r2);
-                    }
+                for (int d = 0; d < MAX_POINT_DIMENSION; d++) xj[d] = d < dim ? c2.x[d][cc] : 0.0;
+                double bx[NB];
+#pragma unroll
+                for (int h = 0; h < NB; h++) {
+                    const double x = Xs[(int64_t)cc * RB + 16 * h + i16];
+                    bx[h] = ok ? x : 0.0;
                 }
-                double kv = kernel_of_r2_k<KIND>(kp, r2);
-                if (row == col) kv += kp.reg;
-                const double* __restrict__ x = Xs + (int64_t)col * RB;
+                double a[4];
 #pragma unroll
-                for (int c = 0; c < RB; c++) acc[c] = __fma_rn(kv, x[c], acc[c]);
+                for (int t = 0; t < 4; t++) {
+                    double r2 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < MAX_POINT_DIMENSION; d++) {
+                        if (d < dim) {
+                            const double dd = xi[t][d] - xj[d];
+                            r2 = __fma_rn(dd, dd, r2);
+                        }
+                    }
+                    double kv = kernel_of_r2_k<KIND>(kp, r2);
+                    if (row[t] == cc) kv += kp.reg;
+                    a[t] = ok ? kv : 0.0;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+#pragma unroll
+                    for (int h = 0; h < NB; h++) acc[t][h] = mfma64(a[t], bx[h], acc[t][h]);
             }
         }
-        if (act) {
-            double* __restrict__ out = P + ((int64_t)q * 2 * kTileRows + lane + half * kWave) * RB;
+        double* __restrict__ out = P + ((int64_t)q * 2 * kTileRows + half * kWave) * RB;
 #pragma unroll
-            for (int c = 0; c < RB; c++) out[c] = acc[c];
-        }
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rr = 16 * t + 4 * r + g;
+                if (half * kWave + rr < it.y) {
+#pragma unroll
+                    for (int h = 0; h < NB; h++) out[(int64_t)rr * RB + 16 * h + i16] = acc[t][h][r];
+                }
+            }
     }
 }
 
@@ -321,7 +388,7 @@ hmg_status hmvp_multi(Engine& eng, const Coords& c1, const Coords& c2, const Ker
     }
     hipStream_t s = eng.stream;
     const int n1 = eng.n[0], n2 = eng.n[1];
-    const int rb = nrhs >= kRbMax ? kRbMax : nrhs > 8 ? 16 : nrhs > 4 ? 8 : 4;
+    const int rb = nrhs > 16 ? kRbMax : 16;
     // partial-sum slots of the stored-near-field schedule (the combine lists')
     const int64_t slots = (int64_t)eng.n_lr_items * kTileRows + (int64_t)eng.n_near_items * 2 * kTileRows;
     HMG_TRY(eng.X64.alloc((size_t)std::max(n2, 1) * rb));
@@ -340,12 +407,8 @@ hmg_status hmvp_multi(Engine& eng, const Coords& c1, const Coords& c2, const Ker
         hmg_status st;
         if (rb == 32)
             st = pass<32>(eng, c1, c2, kp, order2, X, Y, c0, nr, U, V);
-        else if (rb == 16)
-            st = pass<16>(eng, c1, c2, kp, order2, X, Y, c0, nr, U, V);
-        else if (rb == 8)
-            st = pass<8>(eng, c1, c2, kp, order2, X, Y, c0, nr, U, V);
         else
-            st = pass<4>(eng, c1, c2, kp, order2, X, Y, c0, nr, U, V);
+            st = pass<16>(eng, c1, c2, kp, order2, X, Y, c0, nr, U, V);
         HMG_TRY(st);
     }
     return HMG_OK;
