@@ -30,6 +30,8 @@ OPT_LR_WORKGROUP = 9
 OPT_EXACT_BOXES = 10
 OPT_NEAR_WORKGROUP = 11
 OPT_FP64_BLOCK_RHS = 12
+OPT_LR_PRIORITY = 13
+OPT_MIXED_ACA_PARTS = 14
 
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -169,6 +171,7 @@ SIGNATURES = {
     "hmg_set_option": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_int64]),
     "hmg_get_profile": (ctypes.c_int, [_D, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64)]),
     "hmg_copy_probe": (ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
+    "hmg_release_cached_memory": (None, []),
     "hmg_memory": (ctypes.c_int, [_D, _P(ctypes.c_int64)]),
     "hmg_setup_batch": (ctypes.c_int, [_P(_D), ctypes.c_int, ctypes.c_int]),
     "hmg_precompute_aca_batch": (ctypes.c_int, [_P(_D), ctypes.c_int, ctypes.c_int]),

@@ -803,9 +803,10 @@ int64_t aca_mixed_scratch(const Engine& eng)
 {
     int64_t total = 0;
     for (int a = 0; a < eng.naca; a++) total += block_scratch(eng, a);
-    // all at once up to 2^31 elements (16 GiB), else a quarter of the factors at a
-    // time (each batch ends with its slowest block)
-    const int64_t cap = std::max<int64_t>(total / 4, (int64_t)1 << 31);
+    // 1/parts of the factors at a time, at least 2^27 elements (1 GiB): every batch
+    // mixes large and small blocks, so more batches cost little GPU time, while a
+    // large fresh allocation costs seconds (mapping tens of GB on MI355X)
+    const int64_t cap = std::max<int64_t>(total / std::max(eng.aca_mixed_parts, 1), (int64_t)1 << 27);
     return std::min(total, cap);
 }
 

@@ -10,6 +10,8 @@
 #include <cstring>
 #include <cstddef>
 
+#include <mutex>
+
 #include "hmg_internal.h"
 #include "../../include/hmglib_server.h"
 
@@ -20,6 +22,90 @@ thread_local std::string g_error;
 }
 
 thread_local hipStream_t g_alloc_stream = nullptr;
+
+namespace {
+struct LargeBlock {
+    int dev;
+    void* p;
+    size_t bytes;
+};
+std::mutex g_large_mu;
+std::vector<LargeBlock> g_large;  // released large buffers, still mapped
+}  // namespace
+
+hipError_t large_alloc(void** p, size_t bytes, size_t* held)
+{
+    *p = nullptr;
+    *held = 0;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    {
+        // best fit among the cached buffers of this device no larger than twice the
+        // request
+        std::lock_guard<std::mutex> lk(g_large_mu);
+        size_t best = SIZE_MAX, at = g_large.size();
+        for (size_t i = 0; i < g_large.size(); i++) {
+            const LargeBlock& b = g_large[i];
+            if (b.dev == dev && b.bytes >= bytes && b.bytes / 2 <= bytes && b.bytes < best) {
+                best = b.bytes;
+                at = i;
+            }
+        }
+        if (at < g_large.size()) {
+            *p = g_large[at].p;
+            *held = g_large[at].bytes;
+            g_large.erase(g_large.begin() + (ptrdiff_t)at);
+            return hipSuccess;
+        }
+    }
+    e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        large_trim(false);
+        e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return e;
+    }
+    *held = bytes;
+    return hipSuccess;
+}
+
+void large_free(void* p, size_t held)
+{
+    if (!p) return;
+    int dev = 0;
+    // the buffer may be handed to work on another stream next: wait for the device
+    // as hipFree does
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        (void)hipFree(p);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_large_mu);
+    g_large.push_back(LargeBlock{dev, p, held});
+}
+
+void large_trim(bool every)
+{
+    int dev = 0;
+    if (!every && hipGetDevice(&dev) != hipSuccess) return;
+    std::vector<LargeBlock> out;
+    {
+        std::lock_guard<std::mutex> lk(g_large_mu);
+        std::vector<LargeBlock> keep;
+        for (const LargeBlock& b : g_large) (every || b.dev == dev ? out : keep).push_back(b);
+        g_large.swap(keep);
+    }
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (const LargeBlock& b : out) {
+        if (b.dev != cur) (void)hipSetDevice(b.dev);
+        (void)hipFree(b.p);
+        if (b.dev != cur) (void)hipSetDevice(cur);
+    }
+}
 
 void set_error(const std::string& msg) { g_error = msg; }
 
@@ -303,6 +389,8 @@ extern "C" {
 
 const char* hmg_last_error(void) { return g_error.c_str(); }
 
+void hmg_release_cached_memory(void) { large_trim(true); }
+
 // ------------------------------------------------------------------ init
 hmg_status hmg_init_h_matrix_data(struct h_matrix_data* data, int point_count[2], int dim, int bits)
 {
@@ -396,11 +484,10 @@ hmg_status engine_init(h_matrix_data* data, int point_count[2], int dim, int bit
         point_set ps{data->coords_device[i], dim, n, data->max_per_dim_d[i], data->min_per_dim_d[i],
                      data->point_ids_d[i]};
         HMG_HIP(hipMallocAsync((void**)&data->points_d[i], sizeof(point_set), e->stream));
-        HMG_HIP(hipMemcpyAsync(data->points_d[i], &ps, sizeof(ps), hipMemcpyHostToDevice, e->stream));
+        HMG_TRY(upload_bytes(data->points_d[i], &ps, sizeof(ps), e->stream));
         morton_code mc{data->code_d[i], n, dim, bits, nullptr, nullptr};
         HMG_HIP(hipMallocAsync((void**)&data->morton_d[i], sizeof(morton_code), e->stream));
-        HMG_HIP(hipMemcpyAsync(data->morton_d[i], &mc, sizeof(mc), hipMemcpyHostToDevice, e->stream));
-        HMG_HIP(hipStreamSynchronize(e->stream));
+        HMG_TRY(upload_bytes(data->morton_d[i], &mc, sizeof(mc), e->stream));
     }
     return HMG_OK;
 }
@@ -912,7 +999,8 @@ namespace {
 template <class T>
 int64_t bytes_of(const DevBuf<T>& b)
 {
-    return (int64_t)(b.p ? b.n : 0) * (int64_t)sizeof(T);
+    if (!b.p) return 0;
+    return b.held ? (int64_t)b.held : (int64_t)b.n * (int64_t)sizeof(T);
 }
 
 int64_t engine_bytes(const Engine& e)
@@ -1075,6 +1163,16 @@ hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value)
             return HMG_ERR_INVALID;
         }
         e->fp64_block_min = (int)value;
+        return HMG_OK;
+    case HMG_OPT_LR_PRIORITY:
+        e->lr_prio = value != 0;
+        return HMG_OK;
+    case HMG_OPT_MIXED_ACA_PARTS:
+        if (value < 1 || value > 4096) {
+            set_error("mixed ACA parts must be in 1..4096");
+            return HMG_ERR_INVALID;
+        }
+        e->aca_mixed_parts = (int)value;
         return HMG_OK;
     default:
         set_error("unknown option " + std::to_string(option));

@@ -78,18 +78,35 @@ struct AllocScope {
     AllocScope& operator=(const AllocScope&) = delete;
 };
 
-// allocations of at least this many bytes bypass the stream-ordered pool
+// Allocations of at least kLargeAlloc bytes bypass the stream-ordered pool: they
+// come from hipMalloc and, when released, wait for the device (as hipFree would)
+// and stay mapped in a process-wide cache that later large allocations of up to
+// twice their size on the same device reuse.  On MI355X a fresh hipMalloc of device
+// memory used before costs about 28 ms per GB (tens of GB: seconds); a reused
+// buffer costs nothing.  An allocation that fails returns the cached buffers of
+// its device and tries again; hmg_release_cached_memory returns them all.
 constexpr size_t kLargeAlloc = (size_t)8 << 30;
+hipError_t large_alloc(void** p, size_t bytes, size_t* held);
+void large_free(void* p, size_t held);
+// returns the cached large buffers of the current device (all devices: every = true)
+void large_trim(bool every);
 
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
     hipStream_t s = nullptr;  // allocation stream (nullptr: hipMalloc)
+    size_t held = 0;          // bytes of a large buffer (from the large-buffer cache)
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), s(o.s) { o.p = nullptr; o.n = 0; o.s = nullptr; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), s(o.s), held(o.held)
+    {
+        o.p = nullptr;
+        o.n = 0;
+        o.s = nullptr;
+        o.held = 0;
+    }
     DevBuf& operator=(DevBuf&& o) noexcept
     {
         if (this != &o) {
@@ -97,9 +114,11 @@ struct DevBuf {
             p = o.p;
             n = o.n;
             s = o.s;
+            held = o.held;
             o.p = nullptr;
             o.n = 0;
             o.s = nullptr;
+            o.held = 0;
         }
         return *this;
     }
@@ -110,14 +129,25 @@ struct DevBuf {
         if (count <= n && p) return HMG_OK;
         release();
         if (count == 0) count = 1;
-        // stream-ordered below kLargeAlloc bytes; above it plain hipMalloc / hipFree
-        // (the stream-ordered pool maps a 98 GB buffer in 1.8 s and returns it in
-        // 1.1 s on MI355X, hipMalloc / hipFree in well under 0.1 s)
-        hipStream_t st = count * sizeof(T) >= kLargeAlloc ? nullptr : g_alloc_stream;
-        hipError_t e = st ? hipMallocAsync((void**)&p, count * sizeof(T), st) : hipMalloc(&p, count * sizeof(T));
+        const size_t bytes = count * sizeof(T);
+        hipError_t e;
+        hipStream_t st = nullptr;
+        if (bytes >= kLargeAlloc) {
+            e = large_alloc((void**)&p, bytes, &held);
+        } else {
+            st = g_alloc_stream;
+            e = st ? hipMallocAsync((void**)&p, bytes, st) : hipMalloc(&p, bytes);
+            if (e != hipSuccess) {
+                // cached large buffers may hold the memory: return them and retry
+                (void)hipGetLastError();
+                large_trim(false);
+                e = st ? hipMallocAsync((void**)&p, bytes, st) : hipMalloc(&p, bytes);
+            }
+        }
         if (e != hipSuccess) {
             p = nullptr;
             n = 0;
+            held = 0;
             set_error(std::string("device allocation of ") + std::to_string(count * sizeof(T)) +
                       " bytes failed: " + hipGetErrorString(e));
             return HMG_ERR_OOM;
@@ -128,7 +158,9 @@ struct DevBuf {
     }
     void release() {
         if (p) {
-            if (s)
+            if (held)
+                large_free(p, held);
+            else if (s)
                 (void)hipFreeAsync(p, s);
             else
                 (void)hipFree(p);
@@ -136,6 +168,7 @@ struct DevBuf {
         p = nullptr;
         n = 0;
         s = nullptr;
+        held = 0;
     }
 };
 
@@ -447,6 +480,10 @@ struct Engine {
     std::vector<int> h_mx_wg;
     DevBuf<int> mx_wg;
     bool near_wg = false;
+    // mixed precision: low-rank wavefronts issue at a raised priority (s_setprio)
+    bool lr_prio = false;
+    // mixed-precision ACA: its FP64 scratch holds 1/aca_mixed_parts of the factors
+    int aca_mixed_parts = 16;
     int64_t mx_evals = 0;                       // kernel entries evaluated per pass (incl. masked)
     bool mx_sched = false;                      // near items built for the current tree
     Combine comb_mx;
@@ -513,6 +550,17 @@ hmg_status upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s)
         HMG_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
         HMG_HIP(hipStreamSynchronize(s));
     }
+    return HMG_OK;
+}
+
+// host object (a stack variable) -> device, on stream s; waits for the copy on
+// every path, the error path included, so the source may go out of scope at once
+inline hmg_status upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    const hipError_t ec = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    const hipError_t es = hipStreamSynchronize(s);
+    HMG_HIP(ec);
+    HMG_HIP(es);
     return HMG_OK;
 }
 

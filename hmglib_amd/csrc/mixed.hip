@@ -467,11 +467,12 @@ template <int NT>
 __global__ void __launch_bounds__(kThreads)
 lr_vb_kernel(const LvItem* __restrict__ items, int nitems, const __bf16* __restrict__ Vp,
              const __bf16* __restrict__ Xp, const int* __restrict__ tmap, __bf16* __restrict__ Tpan,
-             float* __restrict__ Tp)
+             float* __restrict__ Tp, int prio)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int q = blockIdx.x * kWaves + wave_id();
     if (q >= nitems) return;
+    if (prio) __builtin_amdgcn_s_setprio(2);
     const LvItem it = items[q];
     const int c32 = lane & 31, h = lane >> 5;
     const int64_t gstride = 16 * (int64_t)it.ld;
@@ -518,11 +519,12 @@ lr_vb_finalize_kernel(const LvSlice* __restrict__ slices, const float* __restric
 template <int NT>
 __global__ void __launch_bounds__(kThreads)
 lr_ub_kernel(const LrItem* __restrict__ items, int nitems, const __bf16* __restrict__ Up,
-             const __bf16* __restrict__ Tpan, float* __restrict__ P)
+             const __bf16* __restrict__ Tpan, float* __restrict__ P, int prio)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int q = blockIdx.x * kWaves + wave_id();
     if (q >= nitems) return;
+    if (prio) __builtin_amdgcn_s_setprio(2);
     const LrItem it = items[q];
     const int c32 = lane & 31, h = lane >> 5;
     const int64_t gstride = 16 * (int64_t)it.ld;
@@ -904,6 +906,7 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
             }
             eng.prof.end(PROF_LR_U, s);
         } else if (eng.naca > 0 && eng.n_lr_items > 0) {
+            const int prio = eng.lr_prio ? 1 : 0;
             eng.prof.begin(PROF_LR_VTX, s);
             // items grouped by interleaved tile count: [4 | 2 | 1]
             for (int g = 0; g < 3; g++) {
@@ -912,11 +915,14 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
                 const LvItem* li = eng.lv_items.p + b;
                 const unsigned gr = grid_for(e - b, kWaves);
                 if (g == 0)
-                    lr_vb_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xp.p, eng.tmap.p, eng.Tpan.p, eng.Tp32.p);
+                    lr_vb_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xp.p, eng.tmap.p, eng.Tpan.p, eng.Tp32.p,
+                                                    prio);
                 else if (g == 1)
-                    lr_vb_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xp.p, eng.tmap.p, eng.Tpan.p, eng.Tp32.p);
+                    lr_vb_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xp.p, eng.tmap.p, eng.Tpan.p, eng.Tp32.p,
+                                                    prio);
                 else
-                    lr_vb_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xp.p, eng.tmap.p, eng.Tpan.p, eng.Tp32.p);
+                    lr_vb_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xp.p, eng.tmap.p, eng.Tpan.p, eng.Tp32.p,
+                                                    prio);
                 HMG_HIP(hipGetLastError());
             }
             if (eng.n_lv_slices > 0) {
@@ -934,11 +940,11 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
                 float* Pg = eng.P32.p + (int64_t)b * kMxLrRows * kMxRhs;
                 const unsigned gr = grid_for(e - b, kWaves);
                 if (g == 0)
-                    lr_ub_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg);
+                    lr_ub_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg, prio);
                 else if (g == 1)
-                    lr_ub_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg);
+                    lr_ub_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg, prio);
                 else
-                    lr_ub_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg);
+                    lr_ub_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg, prio);
                 HMG_HIP(hipGetLastError());
             }
             eng.prof.end(PROF_LR_U, s);

@@ -352,7 +352,7 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
     root.set2_u = c2.n - 1;
     root.level_1 = (char)tp.root_level_1;
     root.level_2 = (char)tp.root_level_2;
-    HMG_HIP(hipMemcpyAsync(cur.p, &root, sizeof(work_item), hipMemcpyHostToDevice, s));
+    HMG_TRY(upload_bytes(cur.p, &root, sizeof(work_item), s));
     int ncur = 1;
     int nout = 0;
     size_t out_cap = 0;

@@ -68,7 +68,10 @@ def synchronize():
 
 def trim_pool():
     """Return the unused memory the device's default stream-ordered pool keeps
-    cached (the library frees into that pool) to the device."""
+    cached (the library frees into that pool) and the library's cached large
+    buffers to the device."""
+    import hmglib_amd
+    hmglib_amd.lib().hmg_release_cached_memory()
     pool = ctypes.c_void_p()
     _chk(_load().hipDeviceGetDefaultMemPool(ctypes.byref(pool), 0), "hipDeviceGetDefaultMemPool")
     _chk(_load().hipMemPoolTrimTo(pool, ctypes.c_size_t(0)), "hipMemPoolTrimTo")

@@ -243,11 +243,17 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
  *   each step's X panel groups through LDS; 0: one wavefront per row slice.  The
  *   products are bitwise the same (the autotuner picks the faster).
  * HMG_OPT_FP64_BLOCK_RHS [2]: FP64 products with at least this many right-hand
- *   sides run in passes of up to 32 (each factor element read once per pass, the
- *   near field evaluated once per entry and pass in row form) instead of one
- *   right-hand side at a time (symmetric near field).  A right-hand side's result
- *   is bitwise the same in every pass width; between the two paths it agrees to
- *   rounding.  1 sends every FP64 product through the block path. */
+ *   sides run in passes of 16 or 32 as FP64 MFMA products (each factor element
+ *   read once per pass, each near-field entry evaluated once per pass) instead of
+ *   one right-hand side at a time (symmetric near field).  A right-hand side's
+ *   result is bitwise the same in every pass width; between the two paths it
+ *   agrees to rounding.  1 sends every FP64 product through the block path.
+ * HMG_OPT_LR_PRIORITY [0]: mixed precision: 1 raises the issue priority of the
+ *   low-rank wavefronts over the near field's that run beside them on the same
+ *   compute units (results unchanged).
+ * HMG_OPT_MIXED_ACA_PARTS [16]: mixed precision: the FP64 ACA runs in batches whose
+ *   FP64 factors (before the split into BF16 panels) fill at most 1/parts of the
+ *   factors' size (at least 1 GiB); results unchanged. */
 #define HMG_OPT_NEAR_SYMMETRIC 1
 #define HMG_OPT_NEAR_LDS_BYTES 2
 #define HMG_OPT_SERIALIZE 3
@@ -260,6 +266,8 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
 #define HMG_OPT_EXACT_BOXES 10
 #define HMG_OPT_NEAR_WORKGROUP 11
 #define HMG_OPT_FP64_BLOCK_RHS 12
+#define HMG_OPT_LR_PRIORITY 13
+#define HMG_OPT_MIXED_ACA_PARTS 14
 #define HMG_PREC_FP64 0
 #define HMG_PREC_MIXED 1
 hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value);
@@ -290,6 +298,11 @@ hmg_status hmg_get_profile(const struct h_matrix_data* data, int kernel_class, d
 hmg_status hmg_copy_probe(const double* src, double* dst, int64_t n);
 /* last error message of this thread */
 const char* hmg_last_error(void);
+/* Device buffers of 8 GiB and more (factors, panels, scratch) stay mapped after
+ * their H-matrix releases them and are reused by later large allocations on the
+ * same device (re-mapping device memory costs about 28 ms per GB on MI355X); this
+ * returns them to the devices.  An allocation that fails returns them itself. */
+void hmg_release_cached_memory(void);
 /* sizeof / field offsets of the public structs, for FFI layout checks: which =
  * 0 h_matrix_data, 1 work_item, 2 point_set, 3 morton_code, 4 system_assembler,
  * 5 mat_vec_data_info, 6 hmg_server_config, 7 hmg_build_request, 8
