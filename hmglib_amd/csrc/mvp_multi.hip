@@ -61,9 +61,12 @@ __device__ __forceinline__ d4 mfma64(double a, double b, d4 c)
 
 // Tp[(q*64 + col)*RB + c] = sum_{j < nrows} V[vbase + col + j*ld] * Xs[(j0 + j)*RB + c]:
 // M = the item's 64 columns (four 16-column tiles), N = right-hand sides (RB / 16
-// tiles), K = points, four per step.  Points past nrows enter as zeros.
+// tiles), K = points, four per step.  Points past nrows enter as zeros.  Four
+// wavefronts per SIMD (the accumulators in ordinary registers): the step's loads
+// wait on memory, so occupancy pays (2 per SIMD with AGPR accumulators ran 8%
+// slower alone and twice as slow beside the near field).
 template <int RB>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
 lr_v_multi_kernel(const LvItem* __restrict__ items, int nitems, const double* __restrict__ V,
                   const double* __restrict__ Xs, double* __restrict__ Tp)
 {
@@ -81,11 +84,12 @@ lr_v_multi_kernel(const LvItem* __restrict__ items, int nitems, const double* __
     for (int t = 0; t < 4; t++)
 #pragma unroll
         for (int h = 0; h < NB; h++) acc[t][h] = d4{0.0, 0.0, 0.0, 0.0};
-    for (int j0 = 0; j0 < it.nrows; j0 += 4) {
+    // operands of step j0 (point j0 + g; points past nrows load the item's last
+    // point and enter as zeros)
+    auto load = [&](int j0, double (&a)[4], double (&b)[NB]) {
         const int pt = j0 + g;
         const bool ok = pt < it.nrows;
-        const int pc = ok ? pt : it.nrows - 1;  // loads stay on the item's rows
-        double a[4], b[NB];
+        const int pc = ok ? pt : it.nrows - 1;
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const double x = v[pc * ld + 16 * t];
@@ -96,10 +100,22 @@ lr_v_multi_kernel(const LvItem* __restrict__ items, int nitems, const double* __
             const double x = xb[(int64_t)pc * RB + 16 * h];
             b[h] = ok ? x : 0.0;
         }
+    };
+    double a[4], b[NB];
+    load(0, a, b);
+    for (int j0 = 0; j0 < it.nrows; j0 += 4) {
+        // the next step's operands load while this step's MFMAs run (past the last
+        // step the loads repeat it)
+        double an[4], bn[NB];
+        load(j0 + 4 < it.nrows ? j0 + 4 : j0, an, bn);
 #pragma unroll
         for (int t = 0; t < 4; t++)
 #pragma unroll
             for (int h = 0; h < NB; h++) acc[t][h] = mfma64(a[t], b[h], acc[t][h]);
+#pragma unroll
+        for (int t = 0; t < 4; t++) a[t] = an[t];
+#pragma unroll
+        for (int h = 0; h < NB; h++) b[h] = bn[h];
     }
 #pragma unroll
     for (int t = 0; t < 4; t++)
@@ -139,7 +155,7 @@ lr_v_finalize_multi_kernel(const LvSlice* __restrict__ slices, int nslices, cons
 // P[(q*64 + row)*RB + c] = sum_l U[uoff + row + l*ld] * T[(toff + l)*RB + c]:
 // M = the item's 64 rows, N = right-hand sides, K = rank columns, four per step
 template <int RB>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
 lr_u_multi_kernel(const LrItem* __restrict__ items, int nitems, const double* __restrict__ U,
                   const double* __restrict__ T, double* __restrict__ P)
 {
@@ -157,11 +173,12 @@ lr_u_multi_kernel(const LrItem* __restrict__ items, int nitems, const double* __
     for (int t = 0; t < 4; t++)
 #pragma unroll
         for (int h = 0; h < NB; h++) acc[t][h] = d4{0.0, 0.0, 0.0, 0.0};
-    for (int l0 = 0; l0 < it.K; l0 += 4) {
+    // operands of step l0 (rank column l0 + g; columns past K load the cluster's
+    // last column and enter as zeros)
+    auto load = [&](int l0, double (&a)[4], double (&b)[NB]) {
         const int l = l0 + g;
         const bool ok = l < it.K;
-        const int lc = ok ? l : it.K - 1;  // loads stay on the cluster's rank columns
-        double a[4], b[NB];
+        const int lc = ok ? l : it.K - 1;
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const double x = u[lc * ld + 16 * t];
@@ -172,10 +189,20 @@ lr_u_multi_kernel(const LrItem* __restrict__ items, int nitems, const double* __
             const double x = tb[(int64_t)lc * RB + 16 * h];
             b[h] = ok ? x : 0.0;
         }
+    };
+    double a[4], b[NB];
+    load(0, a, b);
+    for (int l0 = 0; l0 < it.K; l0 += 4) {
+        double an[4], bn[NB];
+        load(l0 + 4 < it.K ? l0 + 4 : l0, an, bn);
 #pragma unroll
         for (int t = 0; t < 4; t++)
 #pragma unroll
             for (int h = 0; h < NB; h++) acc[t][h] = mfma64(a[t], b[h], acc[t][h]);
+#pragma unroll
+        for (int t = 0; t < 4; t++) a[t] = an[t];
+#pragma unroll
+        for (int h = 0; h < NB; h++) b[h] = bn[h];
     }
 #pragma unroll
     for (int t = 0; t < 4; t++)
@@ -224,21 +251,47 @@ near_multi_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict_
         for (int t = 0; t < 4; t++)
 #pragma unroll
             for (int h = 0; h < NB; h++) acc[t][h] = d4{0.0, 0.0, 0.0, 0.0};
-        for (int bi = it.z; bi < it.w; bi++) {
-            const int4 b = blk[near_blk[bi]];
-            for (int j0 = b.z; j0 <= b.w; j0 += 4) {
-                const int col = j0 + g;
-                const bool ok = col <= b.w;
-                const int cc = ok ? col : b.w;
-                double xj[MAX_POINT_DIMENSION];
+        // steps (block bi, first column j0) four columns at a time; lane l takes
+        // column j0 + l / 16 (columns past the block's end load its last column and
+        // enter as zeros).  The next step's coordinates and right-hand sides load
+        // while this step's entries are evaluated (past the last step they repeat it).
+        auto load = [&](int j0, int last, int& cc, bool& ok, double (&xj)[MAX_POINT_DIMENSION],
+                        double (&bx)[NB]) {
+            const int col = j0 + g;
+            ok = col <= last;
+            cc = ok ? col : last;
 #pragma unroll
-                for (int d = 0; d < MAX_POINT_DIMENSION; d++) xj[d] = d < dim ? c2.x[d][cc] : 0.0;
-                double bx[NB];
+            for (int d = 0; d < MAX_POINT_DIMENSION; d++) xj[d] = d < dim ? c2.x[d][cc] : 0.0;
 #pragma unroll
-                for (int h = 0; h < NB; h++) {
-                    const double x = Xs[(int64_t)cc * RB + 16 * h + i16];
-                    bx[h] = ok ? x : 0.0;
+            for (int h = 0; h < NB; h++) {
+                const double x = Xs[(int64_t)cc * RB + 16 * h + i16];
+                bx[h] = ok ? x : 0.0;
+            }
+        };
+        if (it.z < it.w) {  // an item without blocks writes zero partials
+            int bi = it.z;
+            int4 b = blk[near_blk[bi]];
+            int j0 = b.z;
+            int cc;
+            bool ok;
+            double xj[MAX_POINT_DIMENSION], bx[NB];
+            load(j0, b.w, cc, ok, xj, bx);
+            for (;;) {
+                int bin = bi, j0n = j0 + 4;
+                int4 bn = b;
+                if (j0n > b.w) {
+                    bin = bi + 1;
+                    if (bin < it.w) {
+                        bn = blk[near_blk[bin]];
+                        j0n = bn.z;
+                    } else {
+                        j0n = j0;
+                    }
                 }
+                int ccn;
+                bool okn;
+                double xjn[MAX_POINT_DIMENSION], bxn[NB];
+                load(j0n, bn.w, ccn, okn, xjn, bxn);
                 double a[4];
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
@@ -258,6 +311,16 @@ near_multi_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict_
                 for (int t = 0; t < 4; t++)
 #pragma unroll
                     for (int h = 0; h < NB; h++) acc[t][h] = mfma64(a[t], bx[h], acc[t][h]);
+                if (bin >= it.w) break;
+                bi = bin;
+                b = bn;
+                j0 = j0n;
+                cc = ccn;
+                ok = okn;
+#pragma unroll
+                for (int d = 0; d < MAX_POINT_DIMENSION; d++) xj[d] = xjn[d];
+#pragma unroll
+                for (int h = 0; h < NB; h++) bx[h] = bxn[h];
             }
         }
         double* __restrict__ out = P + ((int64_t)q * 2 * kTileRows + half * kWave) * RB;
