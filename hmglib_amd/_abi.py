@@ -32,6 +32,7 @@ OPT_NEAR_WORKGROUP = 11
 OPT_FP64_BLOCK_RHS = 12
 OPT_LR_PRIORITY = 13
 OPT_MIXED_ACA_PARTS = 14
+OPT_ACA_LDS_BYTES = 15
 
 PREC_FP64 = 0
 PREC_MIXED = 1

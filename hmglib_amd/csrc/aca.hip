@@ -695,12 +695,12 @@ static hmg_status aca_launch(Engine& eng, const Coords& c1, const Coords& c2, co
         return HMG_OK;
     }
     if (tol.mode != 0) {
-        HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND, true><<<grid, kAcaTeam * kWave, 0, s>>>(
+        HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND, true><<<grid, kAcaTeam * kWave, eng.aca_lds_pad, s>>>(
                                        c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, nbig, nsmall,
                                        eng.kb.p, uoff, eng.ldu.p, voff, eng.ldv.p, U, V, ublk, vblk, keff, tie_last, scan,
                                        tol));
     } else {
-        HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND, false><<<grid, kAcaTeam * kWave, 0, s>>>(
+        HMG_DISPATCH_KIND(kp.kind, aca_kernel<T, KIND, false><<<grid, kAcaTeam * kWave, eng.aca_lds_pad, s>>>(
                                        c1, c2, kp, blk, eng.blocks.p + eng.ndense, list, nbig, nsmall,
                                        eng.kb.p, uoff, eng.ldu.p, voff, eng.ldv.p, U, V, ublk, vblk, keff, tie_last, scan,
                                        tol));

@@ -1167,6 +1167,13 @@ hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value)
     case HMG_OPT_LR_PRIORITY:
         e->lr_prio = value != 0;
         return HMG_OK;
+    case HMG_OPT_ACA_LDS_BYTES:
+        if (value < 0 || value > 96 * 1024) {
+            set_error("ACA LDS reservation must be in 0..98304 bytes");
+            return HMG_ERR_INVALID;
+        }
+        e->aca_lds_pad = (int)value;
+        return HMG_OK;
     case HMG_OPT_MIXED_ACA_PARTS:
         if (value < 1 || value > 4096) {
             set_error("mixed ACA parts must be in 1..4096");

@@ -484,6 +484,9 @@ struct Engine {
     bool lr_prio = false;
     // mixed-precision ACA: its FP64 scratch holds 1/aca_mixed_parts of the factors
     int aca_mixed_parts = 16;
+    // unused LDS reserved by every workgroup of the main ACA launch (caps the blocks
+    // in flight per compute unit)
+    int aca_lds_pad = 0;
     int64_t mx_evals = 0;                       // kernel entries evaluated per pass (incl. masked)
     bool mx_sched = false;                      // near items built for the current tree
     Combine comb_mx;

@@ -253,7 +253,10 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
  *   compute units (results unchanged).
  * HMG_OPT_MIXED_ACA_PARTS [16]: mixed precision: the FP64 ACA runs in batches whose
  *   FP64 factors (before the split into BF16 panels) fill at most 1/parts of the
- *   factors' size (at least 1 GiB); results unchanged. */
+ *   factors' size (at least 1 GiB); results unchanged.
+ * HMG_OPT_ACA_LDS_BYTES [0]: unused LDS each workgroup of the main ACA launch
+ *   reserves, capping the blocks in flight per compute unit (their factor columns,
+ *   re-read at every rank step, then stay in cache); results unchanged. */
 #define HMG_OPT_NEAR_SYMMETRIC 1
 #define HMG_OPT_NEAR_LDS_BYTES 2
 #define HMG_OPT_SERIALIZE 3
@@ -268,6 +271,7 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
 #define HMG_OPT_FP64_BLOCK_RHS 12
 #define HMG_OPT_LR_PRIORITY 13
 #define HMG_OPT_MIXED_ACA_PARTS 14
+#define HMG_OPT_ACA_LDS_BYTES 15
 #define HMG_PREC_FP64 0
 #define HMG_PREC_MIXED 1
 hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value);
