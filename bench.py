@@ -243,8 +243,11 @@ def cpu_baseline(X, x, cfg, stride):
 
 RANK24_NOTE = ("the error against exact FP64 rows is the rank-24 H-approximation error at leaf 256, eta 1.0 "
                "(the FP64 engine equals the oracle's H-MVP of the same tree and ranks to 1e-8 at these "
-               "parameters); the 1e-6 FP64 target is out of reach for rank <= 24 at this tree, the ACA "
-               "tolerance 1e-7 is met before rank 24 in almost no block")
+               "parameters); the 1e-6 FP64 target is out of reach for rank <= 24 at this tree. The ACA runs "
+               "at fixed rank 24: the reference's tolerance rule (its unbatched ACA) stops at "
+               "eps (1 - eta) / (1 + eps), which is 0 at eta = 1.0, and the relative rule at 1e-7 stops "
+               "early so rarely that it keeps 99.88% of the rank sum (25300223 of 25331760) at twice the "
+               "build time")
 
 WORKLOADS = {
     # mixed precision, 32 right-hand sides, against the plain FP64 product (headline)
@@ -748,6 +751,11 @@ def main():
     hm.destroy()
 
     def log_free(what):
+        # the library keeps released buffers of 8 GiB and more mapped for reuse, and
+        # the stream-ordered pool keeps freed memory: return both first, so that the
+        # figure shows what is still held
+        hip.synchronize()
+        hip.trim_pool()
         f, t = hip.mem_info()
         log(f"[bench] device memory before {what}: {f / 2**30:.1f} GiB free of {t / 2**30:.1f}")
 

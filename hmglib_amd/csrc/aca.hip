@@ -729,7 +729,7 @@ hmg_status aca_build(Engine& eng, const Coords& c1, const Coords& c2, const Kern
         cnt[cls] = (int)order.size() - before;
     }
     HMG_TRY(eng.aca_list.alloc(na));
-    HMG_HIP(hipMemcpyAsync(eng.aca_list.p, order.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
+    HMG_TRY(upload_bytes(eng.aca_list.p, order.data(), sizeof(int) * na, s));
     eng.prof.begin(PROF_ACA, s);
     HMG_TRY(aca_launch<double>(eng, c1, c2, kp, tie_last, eng.aca_list.p, cnt[0], cnt[1], cnt[2], eng.uoff.p,
                                eng.voff.p, U, V, keff, ublk, vblk));

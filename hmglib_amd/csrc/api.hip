@@ -33,6 +33,95 @@ std::mutex g_large_mu;
 std::vector<LargeBlock> g_large;  // released large buffers, still mapped
 }  // namespace
 
+namespace {
+constexpr size_t kPinChunk = (size_t)4 << 20;
+// this thread's pinned, device-mapped staging buffer (host and device addresses)
+thread_local void* t_pin = nullptr;
+thread_local void* t_pin_dev = nullptr;
+thread_local size_t t_pin_bytes = 0;
+
+hmg_status pinned(size_t bytes, void** host, void** dev)
+{
+    if (t_pin_bytes < bytes) {
+        if (t_pin) (void)hipHostFree(t_pin);
+        t_pin = t_pin_dev = nullptr;
+        t_pin_bytes = 0;
+        HMG_HIP(hipHostMalloc(&t_pin, bytes, hipHostMallocMapped | hipHostMallocCoherent));
+        HMG_HIP(hipHostGetDevicePointer(&t_pin_dev, t_pin, 0));
+        t_pin_bytes = bytes;
+    }
+    *host = t_pin;
+    *dev = t_pin_dev;
+    return HMG_OK;
+}
+
+// dst[0, n) = src[0, n) bytes; 16 bytes per thread and step when both ends and n
+// are 16-byte aligned
+__global__ void __launch_bounds__(256) copy_bytes_kernel(const unsigned char* __restrict__ src,
+                                                         unsigned char* __restrict__ dst, size_t n, int wide)
+{
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x, step = (size_t)gridDim.x * 256;
+    if (wide) {
+        const uint4* s4 = (const uint4*)src;
+        uint4* d4 = (uint4*)dst;
+        for (size_t i = t; i < n / 16; i += step) d4[i] = s4[i];
+    } else {
+        for (size_t i = t; i < n; i += step) dst[i] = src[i];
+    }
+}
+
+hmg_status copy_bytes(void* dst, const void* src, size_t n, hipStream_t s)
+{
+    const int wide = (((uintptr_t)dst | (uintptr_t)src | n) & 15) == 0;
+    const size_t units = wide ? n / 16 : n;
+    const unsigned grid = (unsigned)std::min<size_t>((units + 255) / 256, 1024);
+    copy_bytes_kernel<<<grid, 256, 0, s>>>((const unsigned char*)src, (unsigned char*)dst, n, wide);
+    return hipGetLastError() == hipSuccess ? HMG_OK : HMG_ERR_HIP;
+}
+}  // namespace
+
+// The copies between the caller's host memory and the device run as a copy kernel
+// reading or writing the thread's device-mapped pinned buffer: the runtime's own
+// copy paths for pageable memory (staging, and CPU reads of small device buffers
+// through the host mapping) are never used.
+hmg_status upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return HMG_OK;
+    void *pin, *pin_dev;
+    HMG_TRY(pinned(std::min(bytes, kPinChunk), &pin, &pin_dev));
+    for (size_t off = 0; off < bytes; off += kPinChunk) {
+        const size_t n = std::min(kPinChunk, bytes - off);
+        std::memcpy(pin, (const char*)src + off, n);
+        const hmg_status ec = copy_bytes((char*)dst + off, pin_dev, n, s);
+        const hipError_t es = hipStreamSynchronize(s);  // also on error: the buffer is reused
+        if (ec != HMG_OK) {
+            set_error("host-to-device copy launch failed");
+            return ec;
+        }
+        HMG_HIP(es);
+    }
+    return HMG_OK;
+}
+
+hmg_status download_bytes(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return HMG_OK;
+    void *pin, *pin_dev;
+    HMG_TRY(pinned(std::min(bytes, kPinChunk), &pin, &pin_dev));
+    for (size_t off = 0; off < bytes; off += kPinChunk) {
+        const size_t n = std::min(kPinChunk, bytes - off);
+        const hmg_status ec = copy_bytes(pin_dev, (const char*)src + off, n, s);
+        const hipError_t es = hipStreamSynchronize(s);
+        if (ec != HMG_OK) {
+            set_error("device-to-host copy launch failed");
+            return ec;
+        }
+        HMG_HIP(es);
+        std::memcpy((char*)dst + off, pin, n);
+    }
+    return HMG_OK;
+}
+
 hipError_t large_alloc(void** p, size_t bytes, size_t* held)
 {
     *p = nullptr;
@@ -168,8 +257,7 @@ hmg_status read_kernel(const h_matrix_data* d, KernelParams* kp)
     // for every other H-matrix's work on the device
     Engine* e = engine_of(d);
     hipStream_t s = e ? e->stream : nullptr;
-    HMG_HIP(hipMemcpyAsync(&a, d->assem, sizeof(a), hipMemcpyDeviceToHost, s));
-    HMG_HIP(hipStreamSynchronize(s));
+    HMG_TRY(download_bytes(&a, d->assem, sizeof(a), s));
     if (a.kind < HMG_KERNEL_GAUSSIAN_REF || a.kind > HMG_KERNEL_EXPONENTIAL) {
         set_error("unknown kernel kind " + std::to_string(a.kind));
         return HMG_ERR_INVALID;
@@ -527,8 +615,7 @@ hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
             HMG_TRY(flag_differ_f64(data->coords_d[0][d], data->coords_d[1][d], data->point_count[0],
                                     flag.p, s));
         int differ = 1;
-        HMG_HIP(hipMemcpyAsync(&differ, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        HMG_HIP(hipStreamSynchronize(s));
+        HMG_TRY(download_bytes(&differ, flag.p, sizeof(int), s));
         same_input = differ == 0;
     }
     for (int i = 0; i < 2; i++) {
@@ -554,9 +641,8 @@ hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
             HMG_TRY(minmax_f64(data->coords_d[i][d], n, data->min_per_dim_d[i] + d,
                                data->max_per_dim_d[i] + d, s));
         double mn[MAX_POINT_DIMENSION], mx[MAX_POINT_DIMENSION];
-        HMG_HIP(hipMemcpyAsync(mn, data->min_per_dim_d[i], sizeof(double) * dim, hipMemcpyDeviceToHost, s));
-        HMG_HIP(hipMemcpyAsync(mx, data->max_per_dim_d[i], sizeof(double) * dim, hipMemcpyDeviceToHost, s));
-        HMG_HIP(hipStreamSynchronize(s));
+        HMG_TRY(download_bytes(mn, data->min_per_dim_d[i], sizeof(double) * dim, s));
+        HMG_TRY(download_bytes(mx, data->max_per_dim_d[i], sizeof(double) * dim, s));
         for (int d = 0; d < dim; d++) {
             if (!(mx[d] > mn[d])) {
                 set_error("point set " + std::to_string(i) + " has no extent in dimension " +
@@ -595,8 +681,7 @@ hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
             HMG_TRY(flag_differ_f64(data->coords_d[0][d], data->coords_d[1][d], data->point_count[0],
                                     flag.p, s));
         int differ = 1;
-        HMG_HIP(hipMemcpyAsync(&differ, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        HMG_HIP(hipStreamSynchronize(s));
+        HMG_TRY(download_bytes(&differ, flag.p, sizeof(int), s));
         flag.release();
         e->same_sets = differ == 0;
     }

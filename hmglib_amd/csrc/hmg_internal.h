@@ -542,28 +542,20 @@ struct TreeParams {
 hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uint64_t* codes1,
                       const uint64_t* codes2, const TreeParams& tp);
 
-// host vector -> device buffer (grown as needed), on stream s.  The copy has
-// completed when this returns, so the caller may free or reuse the host vector
-// at once (the vectors are pageable host memory).
+// Host <-> device copies of the set-up path (api.hip).  The bytes pass through a
+// pinned buffer of the calling thread (at most 4 MiB, allocated once per thread),
+// so the runtime never stages pageable memory; each call waits for its copies on
+// every path, the error path included, so the host side may go out of scope or
+// be freed at once.
+hmg_status upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t s);
+hmg_status download_bytes(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// host vector -> device buffer (grown as needed), on stream s
 template <class T>
 hmg_status upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s)
 {
     HMG_TRY(d.alloc(std::max<size_t>(h.size(), 1)));
-    if (!h.empty()) {
-        HMG_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
-        HMG_HIP(hipStreamSynchronize(s));
-    }
-    return HMG_OK;
-}
-
-// host object (a stack variable) -> device, on stream s; waits for the copy on
-// every path, the error path included, so the source may go out of scope at once
-inline hmg_status upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t s)
-{
-    const hipError_t ec = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
-    const hipError_t es = hipStreamSynchronize(s);
-    HMG_HIP(ec);
-    HMG_HIP(es);
+    if (!h.empty()) HMG_TRY(upload_bytes(d.p, h.data(), sizeof(T) * h.size(), s));
     return HMG_OK;
 }
 

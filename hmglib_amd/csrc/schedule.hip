@@ -637,7 +637,7 @@ hmg_status aca_layout(Engine& eng, int k, int prec)
     HMG_TRY(eng.kb.alloc(std::max(na, 1)));
     HMG_TRY(eng.keff.alloc(std::max(na, 1)));
     if (na > 0) {
-        HMG_HIP(hipMemcpyAsync(eng.kb.p, eng.h_kb.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
+        HMG_TRY(upload_bytes(eng.kb.p, eng.h_kb.data(), sizeof(int) * na, s));
         HMG_HIP(hipMemsetAsync(eng.keff.p, 0, sizeof(int) * na, s));
     }
     HMG_TRY(upload(eng.uoff, eng.h_uoff, s));

@@ -150,8 +150,7 @@ system_assembler* kernel_for(hmg_server* s, const hmg_build_request& b, hipStrea
     h.kind = b.kernel_kind;
     h.regularization = b.regularization;
     h.length_scale = b.length_scale;
-    if (hipMemcpyAsync(k, &h, sizeof(h), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
+    if (upload_bytes(k, &h, sizeof(h), st) != HMG_OK) {
         set_error("kernel descriptor upload failed");
         return nullptr;
     }
@@ -222,18 +221,10 @@ hmg_status run_build(hmg_server* s, Matrix* m, Request* r, hipStream_t slot, std
         Engine* e = engine_of(&d);
         for (int i = 0; i < 2 && st == HMG_OK; i++) {
             const std::vector<double>& c = r->coords[r->coords[1].empty() ? 0 : i];
-            for (int k = 0; k < b.dim; k++) {
-                if (hipMemcpyAsync(d.coords_d[i][k], c.data() + (size_t)k * b.n[i], sizeof(double) * b.n[i],
-                                   hipMemcpyHostToDevice, e->stream) != hipSuccess) {
-                    set_error("coordinate upload failed");
-                    st = HMG_ERR_HIP;
-                    break;
-                }
-            }
+            for (int k = 0; k < b.dim && st == HMG_OK; k++)
+                st = upload_bytes(d.coords_d[i][k], c.data() + (size_t)k * b.n[i], sizeof(double) * b.n[i],
+                                  e->stream);  // complete when it returns, error or not
         }
-        // every path waits for the uploads issued so far before their host
-        // source goes away (an error above may leave earlier copies in flight)
-        if (hipStreamSynchronize(e->stream) != hipSuccess) st = HMG_ERR_HIP;
         r->coords[0].clear();
         r->coords[0].shrink_to_fit();
         r->coords[1].clear();

@@ -409,8 +409,7 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
         HMG_TRY(scan_i32(nch.p, choff.p, ncur, false, dTotCh, s));
         HMG_TRY(scan_i32(nmv.p, mvoff.p, ncur, false, dTotMv, s));
         int h[10];
-        HMG_HIP(hipMemcpyAsync(h, scal.p, sizeof(int) * 10, hipMemcpyDeviceToHost, s));
-        HMG_HIP(hipStreamSynchronize(s));
+        HMG_TRY(download_bytes(h, scal.p, sizeof(int) * 10, s));
         const int tot_ch = h[2], tot_mv = h[3];
         if (h[8]) eng.quirk_levels++;
         if ((size_t)(nout + tot_mv) > out_cap) {
@@ -451,8 +450,7 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
         partition_kernel<<<grid_for(nout, kT), kT, 0, s>>>(out.p, nout, f.p, dpos.p, dNd,
                                                            eng.blocks.p, eng.blk.p);
         HMG_HIP(hipGetLastError());
-        HMG_HIP(hipMemcpyAsync(&ndense, dNd, sizeof(int), hipMemcpyDeviceToHost, s));
-        HMG_HIP(hipStreamSynchronize(s));
+        HMG_TRY(download_bytes(&ndense, dNd, sizeof(int), s));
         f.release();
         dpos.release();
     }
@@ -460,8 +458,7 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
     eng.naca = nout - ndense;
     eng.h_blk.resize(nout);
     if (nout > 0) {
-        HMG_HIP(hipMemcpyAsync(eng.h_blk.data(), eng.blk.p, sizeof(int4) * nout, hipMemcpyDeviceToHost, s));
-        HMG_HIP(hipStreamSynchronize(s));
+        HMG_TRY(download_bytes(eng.h_blk.data(), eng.blk.p, sizeof(int4) * nout, s));
     }
     for (DevBuf<int>* b : {&flags, &rank, &ub, &keys, &S, &runid, &Lc, &Uc, &rstart, &rend, &scal,
                            &nch, &nmv, &choff, &mvoff, &adm})
