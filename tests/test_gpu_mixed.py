@@ -293,3 +293,76 @@ def test_lr_workgroup_form_bitwise_equal(gpu, case):
     kern.free()
     hm64.destroy()
     k64.free()
+
+
+def test_tuning_options_leave_results_unchanged(gpu):
+    """The low-rank issue priority, the batch count of the mixed ACA and the ACA's
+    LDS reservation change launch shapes only: ranks, factors and products stay
+    bitwise the same (the mixed ACA's batches deal blocks round-robin into a
+    bounded FP64 scratch; each block's ACA is the same arithmetic in any batch)."""
+    from hmglib_amd import hip
+    n = 30000
+    X = _points(1301, 3, n)
+    R = 32
+    Xr = O.normal(1303, n * R).reshape(n, R, order="F")
+    ref = None
+    for parts, lds, prio in ((16, 0, 0), (1, 0, 1), (4, 49152, 0), (64, 24576, 1)):
+        hm = gpu.HMatrix(X, X, 20, 1.0, 50, 256, 24)
+        hm.setup()
+        kern = gpu.Kernel(O.K_EXPONENTIAL, 0.0, 0.1)
+        hm.set_kernel(kern)
+        hm.set_precision(gpu.PREC_MIXED)
+        hm.set_option(gpu.OPT_MIXED_ACA_PARTS, parts)
+        hm.set_option(gpu.OPT_ACA_LDS_BYTES, lds)
+        hm.set_option(gpu.OPT_LR_PRIORITY, prio)
+        hm.precompute_aca()
+        out = (hm.work()["rank_sum"], hm.apply(Xr))
+        if ref is None:
+            ref = out
+        else:
+            assert out[0] == ref[0]
+            np.testing.assert_array_equal(out[1], ref[1])
+        hm.destroy()
+        kern.free()
+    with pytest.raises(gpu.HMGError):
+        hm = gpu.HMatrix(X, X, 20, 1.0, 50, 256, 24)
+        try:
+            hm.set_option(gpu.OPT_MIXED_ACA_PARTS, 0)
+        finally:
+            hm.destroy()
+    # the library's cached large buffers go back to the device on request
+    hip.synchronize()
+    hip.trim_pool()
+    f = hip.mem_info()[0]
+    assert f > 0
+
+
+def test_released_large_buffers_are_reused_and_returned(gpu):
+    """Factor buffers of 8 GiB and more stay mapped after their H-matrix is
+    destroyed, are reused by the next build, and hmg_release_cached_memory returns
+    them: the free device memory after the release is back where it started."""
+    from hmglib_amd import hip
+    n = 1 << 18
+    X = _points(1401, 3, n)
+    hip.synchronize()
+    hip.trim_pool()
+    f0 = hip.mem_info()[0]
+    ys = []
+    for _ in range(2):
+        # FP64 factors of this tree at rank 48: U and V above 8 GiB each
+        hm = gpu.HMatrix(X, X, 20, 1.0, 50, 256, 48)
+        hm.setup()
+        kern = gpu.Kernel(O.K_EXPONENTIAL, 0.0, 0.1)
+        hm.set_kernel(kern)
+        hm.precompute_aca()
+        assert sum(hm.memory()) > 0
+        ys.append(hm.apply(O.normal(1403, n)))
+        hm.destroy()
+        kern.free()
+    np.testing.assert_array_equal(ys[0], ys[1])
+    hip.synchronize()
+    f_held = hip.mem_info()[0]
+    hip.trim_pool()
+    f1 = hip.mem_info()[0]
+    assert f1 >= f0 - (512 << 20), f"{(f0 - f1) / 2**30:.2f} GiB not returned"
+    assert f1 >= f_held
