@@ -151,7 +151,7 @@ hipError_t large_alloc(void** p, size_t bytes, size_t* held)
     e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        large_trim(false);
+        reclaim_device_memory();
         e = hipMalloc(p, bytes);
     }
     if (e != hipSuccess) {
@@ -174,6 +174,17 @@ void large_free(void* p, size_t held)
     }
     std::lock_guard<std::mutex> lk(g_large_mu);
     g_large.push_back(LargeBlock{dev, p, held});
+}
+
+void reclaim_device_memory()
+{
+    large_trim(false);
+    int dev = 0;
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceSynchronize() == hipSuccess && hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
+        (void)hipMemPoolTrimTo(pool, 0);
+    (void)hipGetLastError();
 }
 
 void large_trim(bool every)

@@ -240,6 +240,8 @@ struct StreamPair {
     }
 };
 
+// ACA launch sets of one hmg_precompute_aca_batch call that run at once
+constexpr int kBatchSets = 2;
 // the FP64 scratch elements one launch of the mixed path may use (16 GiB)
 constexpr int64_t kBatchScratch = (int64_t)1 << 31;
 
@@ -330,8 +332,10 @@ extern "C" hmg_status hmg_precompute_aca_batch(struct h_matrix_data* const* data
         if (!groups.count(k)) order.push_back(k);
         groups[k].push_back(i);
     }
-    StreamPair sp;
-    HMG_TRY(sp.create());
+    // the launch sets (one per group, or per part of a mixed group whose scratch
+    // would pass kBatchScratch) run side by side, each on a stream pair of its own:
+    // every set ends with its largest blocks, and the sets' tails overlap
+    std::vector<std::vector<int>> jobs;
     for (const GroupKey& k : order) {
         const std::vector<int>& g = groups[k];
         const bool mixed = engine_of(data[g[0]])->precision == HMG_PREC_MIXED;
@@ -344,7 +348,7 @@ extern "C" hmg_status hmg_precompute_aca_batch(struct h_matrix_data* const* data
                 for (int a = 0; a < e->naca; a++) need += scratch_of(*e, a);
             }
             if (q == g.size() || (!part.empty() && used + need > kBatchScratch)) {
-                if (!part.empty()) HMG_TRY(run_group(data, part, kps[part[0]], sp.s, sp.s2, sp.fork, sp.join));
+                if (!part.empty()) jobs.push_back(part);
                 part.clear();
                 used = 0;
             }
@@ -354,6 +358,15 @@ extern "C" hmg_status hmg_precompute_aca_batch(struct h_matrix_data* const* data
             }
         }
     }
+    // at most kBatchSets sets at once (each mixed set holds up to kBatchScratch)
+    HMG_TRY(parallel_for(
+        (int)jobs.size(), std::min(threads > 0 ? threads : 8, kBatchSets),
+        [&](int j) {
+            StreamPair sp;
+            HMG_TRY(sp.create());
+            return run_group(data, jobs[j], kps[jobs[j][0]], sp.s, sp.s2, sp.fork, sp.join);
+        },
+        "batched ACA launch set"));
     for (int i = 0; i < count; i++) {
         Engine* e = engine_of(data[i]);
         if (e->precision == HMG_PREC_MIXED) {

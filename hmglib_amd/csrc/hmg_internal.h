@@ -90,6 +90,9 @@ hipError_t large_alloc(void** p, size_t bytes, size_t* held);
 void large_free(void* p, size_t held);
 // returns the cached large buffers of the current device (all devices: every = true)
 void large_trim(bool every);
+// after a failed allocation: returns the cached large buffers and, once the device
+// is idle, the stream-ordered pool's free memory of the current device
+void reclaim_device_memory();
 
 template <class T>
 struct DevBuf {
@@ -138,9 +141,9 @@ struct DevBuf {
             st = g_alloc_stream;
             e = st ? hipMallocAsync((void**)&p, bytes, st) : hipMalloc(&p, bytes);
             if (e != hipSuccess) {
-                // cached large buffers may hold the memory: return them and retry
+                // cached large buffers or the pool may hold the memory: return it and retry
                 (void)hipGetLastError();
-                large_trim(false);
+                reclaim_device_memory();
                 e = st ? hipMallocAsync((void**)&p, bytes, st) : hipMalloc(&p, bytes);
             }
         }
