@@ -700,6 +700,18 @@ hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
     TreeParams tp{data->eta, data->max_level, data->c_leaf, (int)data->root_level_set_1,
                   (int)data->root_level_set_2, e->exact_boxes};
     HMG_TRY(build_tree(*e, coords_of(data, 0), coords_of(data, 1), data->code_d[0], data->code_d[1], tp));
+    return finish_setup(data, e);
+}
+
+}  // extern "C"
+
+namespace hmg {
+
+void release_build_state(Engine& e, h_matrix_data* d) { release_build(e, d); }
+
+hmg_status finish_setup(h_matrix_data* data, Engine* e)
+{
+    AllocScope scope(e->stream);
     data->mat_vec_data = (work_item**)malloc(sizeof(work_item*));
     *data->mat_vec_data = e->blocks.p;
     data->mat_vec_data_count = e->nblocks;
@@ -752,6 +764,10 @@ hmg_status hmg_setup_h_matrix(struct h_matrix_data* data)
     e->built = true;
     return HMG_OK;
 }
+
+}  // namespace hmg
+
+extern "C" {
 
 // ------------------------------------------------------------------ precompute
 hmg_status hmg_precompute_aca(struct h_matrix_data* data)

@@ -280,6 +280,282 @@ hmg_status parallel_for(int count, int threads, F f, const char* what)
     return HMG_OK;
 }
 
+// ---- the setup of many clouds in one set of launches ---------------------------
+// Cloud q's points sit at [off[q], off[q] + n[q]) of one array per dimension.
+constexpr int kBT = 256;
+
+// Both sets' coordinates side by side (the first set's copy), the cloud of every
+// point, and whether a cloud's second set differs bitwise from its first
+__global__ void __launch_bounds__(kBT)
+gather_clouds_kernel(const double* const* __restrict__ src0, const double* const* __restrict__ src1,
+                     const int* __restrict__ off, const int* __restrict__ n, int dim, Coords dst,
+                     int* __restrict__ cid, int* __restrict__ differ)
+{
+    const int q = blockIdx.y;
+    const int i = blockIdx.x * kBT + threadIdx.x;
+    if (i >= n[q]) return;
+    const int p = off[q] + i;
+    cid[p] = q;
+    bool diff = false;
+    for (int d = 0; d < dim; d++) {
+        const double* a = src0[q * dim + d];
+        const double* b = src1[q * dim + d];
+        const double v = a[i];
+        ((double*)dst.x[d])[p] = v;
+        if (a != b && __double_as_longlong(v) != __double_as_longlong(b[i])) diff = true;
+    }
+    if (diff) differ[q] = 1;
+}
+
+// extremal values per cloud and dimension (one workgroup per cloud)
+__global__ void __launch_bounds__(kBT)
+cloud_minmax_kernel(Coords x, const int* __restrict__ off, const int* __restrict__ n, int dim,
+                    double* __restrict__ cmin, double* __restrict__ cmax)
+{
+    __shared__ double smin[kBT], smax[kBT];
+    const int q = blockIdx.x;
+    const int o = off[q], m = n[q];
+    for (int d = 0; d < dim; d++) {
+        double lo = __builtin_inf(), hi = -__builtin_inf();
+        for (int i = threadIdx.x; i < m; i += kBT) {
+            const double v = x.x[d][o + i];
+            lo = v < lo ? v : lo;
+            hi = hi < v ? v : hi;
+        }
+        smin[threadIdx.x] = lo;
+        smax[threadIdx.x] = hi;
+        __syncthreads();
+        for (int st = kBT / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < st) {
+                const double a = smin[threadIdx.x + st], b = smax[threadIdx.x + st];
+                smin[threadIdx.x] = a < smin[threadIdx.x] ? a : smin[threadIdx.x];
+                smax[threadIdx.x] = smax[threadIdx.x] < b ? b : smax[threadIdx.x];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            cmin[(int64_t)q * dim + d] = smin[0];
+            cmax[(int64_t)q * dim + d] = smax[0];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void cloud_key_kernel(const uint32_t* __restrict__ perm, const int* __restrict__ cid, int64_t n,
+                                 uint64_t* __restrict__ key)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBT + threadIdx.x;
+    if (i < n) key[i] = (uint64_t)cid[perm[i]];
+}
+
+// Sorted position i of cloud q: its point perm[off[q] + i] - off[q] of the cloud.
+// Writes the sorted coordinates and codes side by side (xs, codes_s) and into
+// both sets of the cloud's own arrays (code, order, point ids, coordinates,
+// extrema), as setup_h_matrix leaves them.
+struct CloudArrays {
+    uint64_t* const* code[2];
+    uint64_t* const* order[2];
+    unsigned* const* ids[2];
+    double* const* x[2];      // [q * dim + d]
+    double* const* mn[2];
+    double* const* mx[2];
+};
+
+__global__ void __launch_bounds__(kBT)
+scatter_clouds_kernel(const uint32_t* __restrict__ perm, const uint64_t* __restrict__ codes, Coords xc,
+                      const int* __restrict__ off, const int* __restrict__ n, int dim,
+                      const double* __restrict__ cmin, const double* __restrict__ cmax, CloudArrays a,
+                      Coords xs, uint64_t* __restrict__ codes_s)
+{
+    const int q = blockIdx.y;
+    const int i = blockIdx.x * kBT + threadIdx.x;
+    if (i >= n[q]) return;
+    const int o = off[q];
+    const uint32_t src = perm[o + i];
+    const uint64_t code = codes[src];
+    const uint64_t local = (uint64_t)(src - (uint32_t)o);
+    codes_s[o + i] = code;
+    for (int set = 0; set < 2; set++) {
+        a.code[set][q][i] = code;
+        a.order[set][q][i] = local;
+        a.ids[set][q][i] = (unsigned)local;
+    }
+    for (int d = 0; d < dim; d++) {
+        const double v = xc.x[d][src];
+        ((double*)xs.x[d])[o + i] = v;
+        a.x[0][q * dim + d][i] = v;
+        a.x[1][q * dim + d][i] = v;
+    }
+    if (i < dim) {
+        for (int set = 0; set < 2; set++) {
+            a.mn[set][q][i] = cmin[(int64_t)q * dim + i];
+            a.mx[set][q][i] = cmax[(int64_t)q * dim + i];
+        }
+    }
+}
+
+// The setup of every listed H-matrix in one set of launches per step (coordinates,
+// extrema, Morton codes, one sort, the level-by-level traversal of all trees at
+// once, the DENSE-first partition), then each H-matrix's schedule on host threads.
+// Applies when every H-matrix has one point set against itself (bitwise equal
+// sets), exact cluster boxes, root levels 0 and the same dimension, code width and
+// tree parameters; *applied = false otherwise (nothing changed).  The result is
+// setup_h_matrix's on each H-matrix alone: the same points, codes, stable order,
+// boxes and BFS leaf order within each cloud.
+hmg_status setup_fused(h_matrix_data* const* data, int count, int threads, bool* applied)
+{
+    *applied = false;
+    if (count < 2 || count > 65535) return HMG_OK;
+    const h_matrix_data* d0 = data[0];
+    std::vector<Engine*> eng((size_t)count);
+    std::vector<int> off((size_t)count), n((size_t)count);
+    int64_t total = 0;
+    for (int q = 0; q < count; q++) {
+        const h_matrix_data* d = data[q];
+        Engine* e = engine_of(d);
+        if (!e || !e->exact_boxes || d->dim != d0->dim || d->bits != d0->bits || d->eta != d0->eta ||
+            d->max_level != d0->max_level || d->c_leaf != d0->c_leaf || d->root_level_set_1 != 0 ||
+            d->root_level_set_2 != 0 || d->point_count[0] != d->point_count[1] || d->point_count[0] < 1 ||
+            d->max_level < 0 || d->c_leaf < 0)
+            return HMG_OK;
+        for (int j = 0; j < q; j++)
+            if (data[j] == d) return HMG_OK;
+        eng[q] = e;
+        off[q] = (int)total;
+        n[q] = d->point_count[0];
+        total += n[q];
+    }
+    if (total > (int64_t)INT32_MAX - 1) return HMG_OK;
+    const int dim = d0->dim, N = (int)total;
+    const int nmax = *std::max_element(n.begin(), n.end());
+    hipStream_t s = nullptr;
+    HMG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard()
+        {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    } guard{s};
+    AllocScope scope(s);
+    std::vector<const double*> src0((size_t)count * dim), src1((size_t)count * dim);
+    for (int q = 0; q < count; q++)
+        for (int k = 0; k < dim; k++) {
+            src0[(size_t)q * dim + k] = data[q]->coords_d[0][k];
+            src1[(size_t)q * dim + k] = data[q]->coords_d[1][k];
+        }
+    DevBuf<const double*> dsrc0, dsrc1;
+    DevBuf<int> doff, dn, cid, differ;
+    HMG_TRY(upload(dsrc0, src0, s));
+    HMG_TRY(upload(dsrc1, src1, s));
+    HMG_TRY(upload(doff, off, s));
+    HMG_TRY(upload(dn, n, s));
+    HMG_TRY(cid.alloc(N));
+    HMG_TRY(differ.alloc(count));
+    HMG_HIP(hipMemsetAsync(differ.p, 0, sizeof(int) * count, s));
+    DevBuf<double> xc[MAX_POINT_DIMENSION], xs[MAX_POINT_DIMENSION];
+    Coords cc{}, cs{};
+    cc.n = cs.n = N;
+    for (int k = 0; k < dim; k++) {
+        HMG_TRY(xc[k].alloc(N));
+        HMG_TRY(xs[k].alloc(N));
+        cc.x[k] = xc[k].p;
+        cs.x[k] = xs[k].p;
+    }
+    const dim3 g2((unsigned)((nmax + kBT - 1) / kBT), (unsigned)count);
+    gather_clouds_kernel<<<g2, kBT, 0, s>>>(dsrc0.p, dsrc1.p, doff.p, dn.p, dim, cc, cid.p, differ.p);
+    HMG_HIP(hipGetLastError());
+    std::vector<int> hdiff((size_t)count);
+    HMG_TRY(download_bytes(hdiff.data(), differ.p, sizeof(int) * count, s));
+    for (int q = 0; q < count; q++)
+        if (hdiff[q]) return HMG_OK;  // a cloud with two different sets: the per-cloud path
+    // from here on the listed H-matrices are rebuilt
+    *applied = true;
+    for (int q = 0; q < count; q++) release_build_state(*eng[q], data[q]);
+    DevBuf<double> cmin, cmax;
+    HMG_TRY(cmin.alloc((size_t)count * dim));
+    HMG_TRY(cmax.alloc((size_t)count * dim));
+    cloud_minmax_kernel<<<(unsigned)count, kBT, 0, s>>>(cc, doff.p, dn.p, dim, cmin.p, cmax.p);
+    HMG_HIP(hipGetLastError());
+    {
+        std::vector<double> hmin((size_t)count * dim), hmax((size_t)count * dim);
+        HMG_TRY(download_bytes(hmin.data(), cmin.p, sizeof(double) * hmin.size(), s));
+        HMG_TRY(download_bytes(hmax.data(), cmax.p, sizeof(double) * hmax.size(), s));
+        for (int q = 0; q < count; q++)
+            for (int k = 0; k < dim; k++)
+                if (!(hmax[(size_t)q * dim + k] > hmin[(size_t)q * dim + k])) {
+                    set_error("H-matrix " + std::to_string(q) + ": point set 0 has no extent in dimension " +
+                              std::to_string(k) + " (Morton quantisation would divide by zero)");
+                    return HMG_ERR_INVALID;
+                }
+    }
+    // Morton codes on each cloud's box; one stable sort by code, then one by cloud:
+    // within a cloud the order is the cloud's own stable sort by code
+    DevBuf<uint64_t> codes, keys;
+    DevBuf<uint32_t> perm;
+    HMG_TRY(codes.alloc(N));
+    HMG_TRY(keys.alloc(N));
+    HMG_TRY(perm.alloc(N));
+    HMG_TRY(morton_codes_batched(cc, dim, d0->bits, cid.p, cmin.p, cmax.p, codes.p, s));
+    HMG_HIP(hipMemcpyAsync(keys.p, codes.p, sizeof(uint64_t) * N, hipMemcpyDeviceToDevice, s));
+    HMG_TRY(iota_u32(perm.p, N, s));
+    const bool spread_defined = (dim == 3 && d0->bits == 20) || (dim == 2 && d0->bits == 32) ||
+                                (dim == 10 && d0->bits == 4);
+    const int key_bits = spread_defined ? std::min(64, dim * d0->bits) : 8;
+    HMG_TRY(radix_sort_u64(keys.p, perm.p, N, key_bits, s));
+    cloud_key_kernel<<<(unsigned)((N + kBT - 1) / kBT), kBT, 0, s>>>(perm.p, cid.p, N, keys.p);
+    HMG_HIP(hipGetLastError());
+    int cbits = 1;
+    while ((1ll << cbits) < count) cbits++;
+    HMG_TRY(radix_sort_u64(keys.p, perm.p, N, cbits, s));
+    // the sorted clouds side by side and in each H-matrix's own arrays
+    std::vector<uint64_t*> pcode[2], pord[2];
+    std::vector<unsigned*> pids[2];
+    std::vector<double*> px[2], pmn[2], pmx[2];
+    for (int set = 0; set < 2; set++) {
+        for (int q = 0; q < count; q++) {
+            h_matrix_data* d = data[q];
+            pcode[set].push_back(d->code_d[set]);
+            pord[set].push_back(d->order[set]);
+            pids[set].push_back(d->point_ids_d[set]);
+            pmn[set].push_back(d->min_per_dim_d[set]);
+            pmx[set].push_back(d->max_per_dim_d[set]);
+            for (int k = 0; k < dim; k++) px[set].push_back(d->coords_d[set][k]);
+        }
+    }
+    DevBuf<uint64_t*> dcode[2], dord[2];
+    DevBuf<unsigned*> dids[2];
+    DevBuf<double*> dx[2], dmn[2], dmx[2];
+    CloudArrays ca{};
+    for (int set = 0; set < 2; set++) {
+        HMG_TRY(upload(dcode[set], pcode[set], s));
+        HMG_TRY(upload(dord[set], pord[set], s));
+        HMG_TRY(upload(dids[set], pids[set], s));
+        HMG_TRY(upload(dx[set], px[set], s));
+        HMG_TRY(upload(dmn[set], pmn[set], s));
+        HMG_TRY(upload(dmx[set], pmx[set], s));
+        ca.code[set] = dcode[set].p;
+        ca.order[set] = dord[set].p;
+        ca.ids[set] = dids[set].p;
+        ca.x[set] = dx[set].p;
+        ca.mn[set] = dmn[set].p;
+        ca.mx[set] = dmx[set].p;
+    }
+    DevBuf<uint64_t> codes_s;
+    HMG_TRY(codes_s.alloc(N));
+    scatter_clouds_kernel<<<g2, kBT, 0, s>>>(perm.p, codes.p, cc, doff.p, dn.p, dim, cmin.p, cmax.p, ca, cs,
+                                             codes_s.p);
+    HMG_HIP(hipGetLastError());
+    // all trees in one traversal, each engine's DENSE-first block list
+    const TreeParams tp{d0->eta, d0->max_level, d0->c_leaf, 0, 0, true};
+    HMG_TRY(build_trees_batched(s, dim, cs, codes_s.p, tp, off, n, cid.p, eng.data()));
+    HMG_HIP(hipStreamSynchronize(s));
+    for (int q = 0; q < count; q++) eng[q]->same_sets = true;
+    // each H-matrix's schedule (host work and its uploads, on its own stream)
+    return parallel_for(count, threads, [&](int q) { return finish_setup(data[q], eng[q]); }, "batched setup");
+}
+
 }  // namespace
 }  // namespace hmg
 
@@ -289,7 +565,39 @@ extern "C" hmg_status hmg_setup_batch(struct h_matrix_data* const* data, int cou
         set_error("invalid batch");
         return HMG_ERR_INVALID;
     }
-    return parallel_for(count, threads, [&](int i) { return hmg_setup_h_matrix(data[i]); }, "batched setup");
+    for (int q = 0; q < count; q++)
+        if (!data[q] || !engine_of(data[q])) {
+            set_error("batched setup: H-matrix " + std::to_string(q) + " not initialised");
+            return HMG_ERR_NOT_READY;
+        }
+    // H-matrices that can share the fused setup, grouped by point dimension, code
+    // width and tree parameters; the others are set up one by one on host threads
+    using SetupKey = std::tuple<int, int, double, int, int>;
+    std::map<SetupKey, std::vector<h_matrix_data*>> groups;
+    std::vector<SetupKey> order;
+    std::vector<h_matrix_data*> single;
+    for (int q = 0; q < count; q++) {
+        h_matrix_data* d = data[q];
+        const Engine* e = engine_of(d);
+        bool repeated = false;
+        for (int j = 0; j < q; j++) repeated = repeated || data[j] == d;
+        if (repeated || !e->exact_boxes || d->root_level_set_1 != 0 || d->root_level_set_2 != 0 ||
+            d->point_count[0] != d->point_count[1]) {
+            single.push_back(d);
+            continue;
+        }
+        const SetupKey k{d->dim, d->bits, d->eta, d->max_level, d->c_leaf};
+        if (!groups.count(k)) order.push_back(k);
+        groups[k].push_back(d);
+    }
+    for (const SetupKey& k : order) {
+        std::vector<h_matrix_data*>& g = groups[k];
+        bool applied = false;
+        HMG_TRY(setup_fused(g.data(), (int)g.size(), threads, &applied));
+        if (!applied) single.insert(single.end(), g.begin(), g.end());
+    }
+    return parallel_for((int)single.size(), threads, [&](int i) { return hmg_setup_h_matrix(single[i]); },
+                        "batched setup");
 }
 
 extern "C" hmg_status hmg_apply_batch(struct h_matrix_data* const* data, const double* const* X, double* const* Y,

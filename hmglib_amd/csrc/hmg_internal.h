@@ -527,6 +527,10 @@ hmg_status flag_differ_f64(const double* a, const double* b, int64_t n, int* dif
 // ---- Morton (morton.hip) ------------------------------------------------------
 hmg_status morton_codes(const Coords& c, int dim, int bits, const double* dmin,
                         const double* dmax, uint64_t* codes, hipStream_t s);
+// several clouds side by side in c: point p quantised on the box of cloud cid[p]
+// (cmin / cmax: dim values per cloud)
+hmg_status morton_codes_batched(const Coords& c, int dim, int bits, const int* cid, const double* cmin,
+                                const double* cmax, uint64_t* codes, hipStream_t s);
 hmg_status iota_u32(uint32_t* v, int64_t n, hipStream_t s);
 hmg_status gather_f64(const double* in, const uint32_t* idx, double* out, int64_t n,
                       hipStream_t s);
@@ -544,6 +548,18 @@ struct TreeParams {
 // Builds the DENSE-first block list into eng.blocks / eng.blk / eng.h_blk.
 hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uint64_t* codes1,
                       const uint64_t* codes2, const TreeParams& tp);
+// The trees of several point clouds, each against itself, in one traversal: cloud q's
+// sorted points at [off[q], off[q] + n[q]) of c and codes (cid: the cloud of each
+// point), exact boxes, the same parameters for all; each engine receives its own
+// DENSE-first block list (its own indices) in blocks / blk / h_blk and its counts.
+hmg_status build_trees_batched(hipStream_t s, int dim, const Coords& c, const uint64_t* codes,
+                               const TreeParams& tp, const std::vector<int>& off, const std::vector<int>& n,
+                               const int* cid, Engine* const* eng);
+// What setup_h_matrix does after the tree (api.hip): block list pointers and counts,
+// batch sizing, row/column groups, the rank layout and the MVP schedule.
+hmg_status finish_setup(h_matrix_data* data, Engine* e);
+// returns what a previous setup built (the state a new setup starts from; api.hip)
+void release_build_state(Engine& e, h_matrix_data* d);
 
 // Host <-> device copies of the set-up path (api.hip).  The bytes pass through a
 // pinned buffer of the calling thread (at most 4 MiB, allocated once per thread),

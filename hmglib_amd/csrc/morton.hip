@@ -38,11 +38,10 @@ __device__ __forceinline__ uint64_t spread(uint64_t x, int dim, int bits)
     return 0;
 }
 
-__global__ void morton_kernel(Coords c, int dim, int bits, const double* __restrict__ dmin,
-                              const double* __restrict__ dmax, uint64_t* __restrict__ codes)
+// code of point p of c quantised on the box [dmin, dmax] (morton.cu:127-155)
+__device__ __forceinline__ uint64_t morton_code(const Coords& c, int64_t p, int dim, int bits,
+                                                const double* __restrict__ dmin, const double* __restrict__ dmax)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= c.n) return;
     const double scale = (double)(1ull << bits);
     const uint64_t cap = (1ull << bits) - 1;
     uint64_t code = 0;
@@ -54,7 +53,26 @@ __global__ void morton_kernel(Coords c, int dim, int bits, const double* __restr
         q = q < cap ? q : cap;
         code |= spread(q, dim, bits) << d;
     }
-    codes[p] = code;
+    return code;
+}
+
+// several clouds side by side: point p of cloud cid[p] on that cloud's box
+__global__ void morton_batched_kernel(Coords c, int dim, int bits, const int* __restrict__ cid,
+                                      const double* __restrict__ cmin, const double* __restrict__ cmax,
+                                      uint64_t* __restrict__ codes)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= c.n) return;
+    const int q = cid[p];
+    codes[p] = morton_code(c, p, dim, bits, cmin + (int64_t)q * dim, cmax + (int64_t)q * dim);
+}
+
+__global__ void morton_kernel(Coords c, int dim, int bits, const double* __restrict__ dmin,
+                              const double* __restrict__ dmax, uint64_t* __restrict__ codes)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= c.n) return;
+    codes[p] = morton_code(c, p, dim, bits, dmin, dmax);
 }
 
 __global__ void iota_kernel(uint32_t* v, int64_t n)
@@ -89,6 +107,15 @@ hmg_status morton_codes(const Coords& c, int dim, int bits, const double* dmin, 
 {
     if (c.n <= 0) return HMG_OK;
     morton_kernel<<<grid_for(c.n, 256), 256, 0, s>>>(c, dim, bits, dmin, dmax, codes);
+    HMG_HIP(hipGetLastError());
+    return HMG_OK;
+}
+
+hmg_status morton_codes_batched(const Coords& c, int dim, int bits, const int* cid, const double* cmin,
+                                const double* cmax, uint64_t* codes, hipStream_t s)
+{
+    if (c.n <= 0) return HMG_OK;
+    morton_batched_kernel<<<grid_for(c.n, 256), 256, 0, s>>>(c, dim, bits, cid, cmin, cmax, codes);
     HMG_HIP(hipGetLastError());
     return HMG_OK;
 }

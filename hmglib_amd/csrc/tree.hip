@@ -308,13 +308,33 @@ __global__ void partition_kernel(const work_item* __restrict__ out, int n, const
     blk[pos] = make_int4(w.set1_l, w.set1_u, w.set2_l, w.set2_u);
 }
 
-}  // namespace
-
-hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uint64_t* codes1,
-                      const uint64_t* codes2, const TreeParams& tp)
+// per level: the clusters' cloud (the point's cloud of the item's first row
+// cluster) had work at this level
+__global__ void level_mark_kernel(const work_item* __restrict__ cur, int ncur, const int* __restrict__ cid,
+                                  int level, int* __restrict__ last)
 {
-    hipStream_t s = eng.stream;
-    const int dim = eng.dim;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncur) return;
+    const work_item w = cur[i];
+    atomicMax(&last[cid[min(w.set1_l, w.set1_u)]], level);
+}
+
+// Level-by-level traversal (tree.cu:1429-1557) from the given root items; the
+// leaves (node order per level) go to tr.out.  With cid != nullptr the items may
+// belong to several point clouds laid side by side (index ranges into the same
+// coordinate and code arrays, exact boxes only): last[c] gets the last level at
+// which cloud c had items.
+struct Traversal {
+    DevBuf<work_item> out;
+    int nout = 0;
+    int levels = 0;
+    int quirk_levels = 0;
+};
+
+hmg_status traverse(hipStream_t s, int dim, const Coords& c1, const Coords& c2, const uint64_t* codes1,
+                    const uint64_t* codes2, const TreeParams& tp, const std::vector<work_item>& roots,
+                    const int* cid, int* last, Traversal& tr)
+{
     const int nmax = std::max(c1.n, c2.n);
     const Coords* cs[2] = {&c1, &c2};
 
@@ -323,13 +343,15 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
     HMG_TRY(flags.alloc(nmax));
     HMG_TRY(rank.alloc(nmax));
     HMG_TRY(ub.alloc(nmax));
-    HMG_TRY(keys.alloc(nmax));
-    HMG_TRY(S.alloc(nmax));
-    HMG_TRY(runid.alloc(nmax));
     HMG_TRY(Lc.alloc(nmax));
     HMG_TRY(Uc.alloc(nmax));
-    HMG_TRY(rstart.alloc(nmax));
-    HMG_TRY(rend.alloc(nmax));
+    if (!tp.exact_boxes) {
+        HMG_TRY(keys.alloc(nmax));
+        HMG_TRY(S.alloc(nmax));
+        HMG_TRY(runid.alloc(nmax));
+        HMG_TRY(rstart.alloc(nmax));
+        HMG_TRY(rend.alloc(nmax));
+    }
     HMG_TRY(tmin.alloc((size_t)nmax * MAX_POINT_DIMENSION));
     HMG_TRY(tmax.alloc((size_t)nmax * MAX_POINT_DIMENSION));
     // scal: [0]=T [1]=runs [2]=total children [3]=total leaves [4]=ndense
@@ -339,28 +361,24 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
     int* dR = scal.p + 1;
     int* dTotCh = scal.p + 2;
     int* dTotMv = scal.p + 3;
-    int* dNd = scal.p + 4;
     int* dDiag = scal.p + 8;
 
-    DevBuf<work_item> cur, next, out;
+    DevBuf<work_item> cur, next;
     DevBuf<int> nch, nmv, choff, mvoff, adm;
-    HMG_TRY(cur.alloc(1));
-    work_item root{};
-    root.set1_l = 0;
-    root.set1_u = c1.n - 1;
-    root.set2_l = 0;
-    root.set2_u = c2.n - 1;
-    root.level_1 = (char)tp.root_level_1;
-    root.level_2 = (char)tp.root_level_2;
-    HMG_TRY(upload_bytes(cur.p, &root, sizeof(work_item), s));
-    int ncur = 1;
-    int nout = 0;
+    HMG_TRY(cur.alloc(std::max<size_t>(roots.size(), 1)));
+    HMG_TRY(upload_bytes(cur.p, roots.data(), sizeof(work_item) * roots.size(), s));
+    int ncur = (int)roots.size();
     size_t out_cap = 0;
-    eng.levels = 0;
-    eng.quirk_levels = 0;
+    tr.nout = 0;
+    tr.levels = 0;
+    tr.quirk_levels = 0;
 
-    for (int level = 0; level < tp.max_level; level++) {
-        eng.levels++;
+    for (int level = 0; level < tp.max_level && ncur > 0; level++) {
+        tr.levels++;
+        if (cid) {
+            level_mark_kernel<<<grid_for(ncur, kT), kT, 0, s>>>(cur.p, ncur, cid, level, last);
+            HMG_HIP(hipGetLastError());
+        }
         HMG_HIP(hipMemsetAsync(dDiag, 0, 2 * sizeof(int), s));
         for (int set = 0; set < 2; set++) {
             const int n = cs[set]->n;
@@ -411,29 +429,87 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
         int h[10];
         HMG_TRY(download_bytes(h, scal.p, sizeof(int) * 10, s));
         const int tot_ch = h[2], tot_mv = h[3];
-        if (h[8]) eng.quirk_levels++;
-        if ((size_t)(nout + tot_mv) > out_cap) {
+        if (h[8]) tr.quirk_levels++;
+        if ((size_t)(tr.nout + tot_mv) > out_cap) {
             size_t cap = std::max<size_t>(1024, out_cap);
-            while (cap < (size_t)(nout + tot_mv)) cap *= 2;
+            while (cap < (size_t)(tr.nout + tot_mv)) cap *= 2;
             DevBuf<work_item> bigger;
             HMG_TRY(bigger.alloc(cap));
-            if (nout > 0)
-                HMG_HIP(hipMemcpyAsync(bigger.p, out.p, sizeof(work_item) * nout,
+            if (tr.nout > 0)
+                HMG_HIP(hipMemcpyAsync(bigger.p, tr.out.p, sizeof(work_item) * tr.nout,
                                        hipMemcpyDeviceToDevice, s));
             HMG_HIP(hipStreamSynchronize(s));
-            out = std::move(bigger);
+            tr.out = std::move(bigger);
             out_cap = cap;
         }
         HMG_TRY(next.alloc(std::max(tot_ch, 1)));
         generate_kernel<<<grid_for(ncur, kT), kT, 0, s>>>(cur.p, ncur, nch.p, choff.p, mvoff.p, adm.p,
-                                                          codes1, codes2, next.p, out.p + nout);
+                                                          codes1, codes2, next.p, tr.out.p + tr.nout);
         HMG_HIP(hipGetLastError());
-        nout += tot_mv;
+        tr.nout += tot_mv;
         std::swap(cur, next);
         ncur = tot_ch;
-        if (ncur == 0) break;
     }
     HMG_HIP(hipStreamSynchronize(s));
+    return HMG_OK;
+}
+
+// key of a leaf for the partition of several clouds' block lists: cloud, then
+// DENSE before ACA
+__global__ void leaf_key_kernel(const work_item* __restrict__ out, int n, const int* __restrict__ cid,
+                                uint64_t* __restrict__ key, uint32_t* __restrict__ idx,
+                                int* __restrict__ counts)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const work_item w = out[i];
+    const int k = 2 * cid[w.set1_l] + (w.work_type == WT_DENSE ? 0 : 1);
+    key[i] = (uint64_t)k;
+    idx[i] = (uint32_t)i;
+    atomicAdd(&counts[k], 1);
+}
+
+// sorted leaf q of cloud c -> its engine's block arrays at position q - first[c],
+// row / column ranges back in the cloud's own indices
+__global__ void leaf_scatter_kernel(const work_item* __restrict__ out, const uint32_t* __restrict__ idx,
+                                    const uint64_t* __restrict__ key, int n, const int* __restrict__ first,
+                                    const int* __restrict__ off, work_item* const* __restrict__ blocks,
+                                    int4* const* __restrict__ blk, int4* __restrict__ all_blk)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int c = (int)(key[q] >> 1);
+    work_item w = out[idx[q]];
+    const int o = off[c];
+    w.set1_l -= o;
+    w.set1_u -= o;
+    w.set2_l -= o;
+    w.set2_u -= o;
+    const int pos = q - first[c];
+    const int4 b = make_int4(w.set1_l, w.set1_u, w.set2_l, w.set2_u);
+    blocks[c][pos] = w;
+    blk[c][pos] = b;
+    all_blk[q] = b;
+}
+
+}  // namespace
+
+hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uint64_t* codes1,
+                      const uint64_t* codes2, const TreeParams& tp)
+{
+    hipStream_t s = eng.stream;
+    work_item root{};
+    root.set1_l = 0;
+    root.set1_u = c1.n - 1;
+    root.set2_l = 0;
+    root.set2_u = c2.n - 1;
+    root.level_1 = (char)tp.root_level_1;
+    root.level_2 = (char)tp.root_level_2;
+    Traversal tr;
+    HMG_TRY(traverse(s, eng.dim, c1, c2, codes1, codes2, tp, {root}, nullptr, nullptr, tr));
+    eng.levels = tr.levels;
+    eng.quirk_levels = tr.quirk_levels;
+    const int nout = tr.nout;
 
     // stable partition, DENSE first (linear_algebra.cu:4037-4051)
     eng.nblocks = nout;
@@ -441,33 +517,108 @@ hmg_status build_tree(Engine& eng, const Coords& c1, const Coords& c2, const uin
     HMG_TRY(eng.blk.alloc(std::max(nout, 1)));
     int ndense = 0;
     if (nout > 0) {
-        DevBuf<int> f, dpos;
+        DevBuf<int> f, dpos, nd;
         HMG_TRY(f.alloc(nout));
         HMG_TRY(dpos.alloc(nout));
-        dense_flag_kernel<<<grid_for(nout, kT), kT, 0, s>>>(out.p, nout, f.p);
+        HMG_TRY(nd.alloc(1));
+        dense_flag_kernel<<<grid_for(nout, kT), kT, 0, s>>>(tr.out.p, nout, f.p);
         HMG_HIP(hipGetLastError());
-        HMG_TRY(scan_i32(f.p, dpos.p, nout, false, dNd, s));
-        partition_kernel<<<grid_for(nout, kT), kT, 0, s>>>(out.p, nout, f.p, dpos.p, dNd,
+        HMG_TRY(scan_i32(f.p, dpos.p, nout, false, nd.p, s));
+        partition_kernel<<<grid_for(nout, kT), kT, 0, s>>>(tr.out.p, nout, f.p, dpos.p, nd.p,
                                                            eng.blocks.p, eng.blk.p);
         HMG_HIP(hipGetLastError());
-        HMG_TRY(download_bytes(&ndense, dNd, sizeof(int), s));
-        f.release();
-        dpos.release();
+        HMG_TRY(download_bytes(&ndense, nd.p, sizeof(int), s));
     }
     eng.ndense = ndense;
     eng.naca = nout - ndense;
     eng.h_blk.resize(nout);
-    if (nout > 0) {
-        HMG_TRY(download_bytes(eng.h_blk.data(), eng.blk.p, sizeof(int4) * nout, s));
+    if (nout > 0) HMG_TRY(download_bytes(eng.h_blk.data(), eng.blk.p, sizeof(int4) * nout, s));
+    return HMG_OK;
+}
+
+hmg_status build_trees_batched(hipStream_t s, int dim, const Coords& c, const uint64_t* codes,
+                               const TreeParams& tp, const std::vector<int>& off, const std::vector<int>& n,
+                               const int* cid, Engine* const* eng)
+{
+    const int count = (int)n.size();
+    std::vector<work_item> roots((size_t)count);
+    for (int q = 0; q < count; q++) {
+        work_item& r = roots[q];
+        r = work_item{};
+        r.set1_l = r.set2_l = off[q];
+        r.set1_u = r.set2_u = off[q] + n[q] - 1;
+        r.level_1 = (char)tp.root_level_1;
+        r.level_2 = (char)tp.root_level_2;
     }
-    for (DevBuf<int>* b : {&flags, &rank, &ub, &keys, &S, &runid, &Lc, &Uc, &rstart, &rend, &scal,
-                           &nch, &nmv, &choff, &mvoff, &adm})
-        b->release();
-    tmin.release();
-    tmax.release();
-    cur.release();
-    next.release();
-    out.release();
+    DevBuf<int> last;
+    HMG_TRY(last.alloc(count));
+    HMG_HIP(hipMemsetAsync(last.p, 0xff, sizeof(int) * count, s));  // -1: no level yet
+    Traversal tr;
+    HMG_TRY(traverse(s, dim, c, c, codes, codes, tp, roots, cid, last.p, tr));
+    const int nout = tr.nout;
+    std::vector<int> lastlv((size_t)count);
+    HMG_TRY(download_bytes(lastlv.data(), last.p, sizeof(int) * count, s));
+    // partition: a stable sort of the leaves by (cloud, DENSE before ACA)
+    DevBuf<uint64_t> key;
+    DevBuf<uint32_t> idx;
+    DevBuf<int> counts;
+    HMG_TRY(key.alloc(std::max(nout, 1)));
+    HMG_TRY(idx.alloc(std::max(nout, 1)));
+    HMG_TRY(counts.alloc(2 * count));
+    HMG_HIP(hipMemsetAsync(counts.p, 0, sizeof(int) * 2 * count, s));
+    if (nout > 0) {
+        leaf_key_kernel<<<grid_for(nout, kT), kT, 0, s>>>(tr.out.p, nout, cid, key.p, idx.p, counts.p);
+        HMG_HIP(hipGetLastError());
+        int kbits = 1;
+        while ((1ll << kbits) < 2ll * count) kbits++;
+        HMG_TRY(radix_sort_u64(key.p, idx.p, nout, kbits, s));
+    }
+    std::vector<int> cnt((size_t)2 * count);
+    HMG_TRY(download_bytes(cnt.data(), counts.p, sizeof(int) * 2 * count, s));
+    std::vector<int> first((size_t)count);
+    int run = 0;
+    for (int q = 0; q < count; q++) {
+        first[q] = run;
+        run += cnt[2 * q] + cnt[2 * q + 1];
+    }
+    // each engine's block arrays (on its own stream, complete before the scatter)
+    std::vector<work_item*> pb((size_t)count);
+    std::vector<int4*> pk((size_t)count);
+    for (int q = 0; q < count; q++) {
+        Engine& e = *eng[q];
+        AllocScope scope(e.stream);
+        const int nb = cnt[2 * q] + cnt[2 * q + 1];
+        e.nblocks = nb;
+        e.ndense = cnt[2 * q];
+        e.naca = cnt[2 * q + 1];
+        e.levels = lastlv[q] + 1;
+        e.quirk_levels = 0;
+        HMG_TRY(e.blocks.alloc(std::max(nb, 1)));
+        HMG_TRY(e.blk.alloc(std::max(nb, 1)));
+        pb[q] = e.blocks.p;
+        pk[q] = e.blk.p;
+    }
+    for (int q = 0; q < count; q++) HMG_HIP(hipStreamSynchronize(eng[q]->stream));
+    if (nout > 0) {
+        DevBuf<work_item*> dpb;
+        DevBuf<int4*> dpk;
+        DevBuf<int> dfirst, doff;
+        DevBuf<int4> all;
+        HMG_TRY(upload(dpb, pb, s));
+        HMG_TRY(upload(dpk, pk, s));
+        HMG_TRY(upload(dfirst, first, s));
+        HMG_TRY(upload(doff, off, s));
+        HMG_TRY(all.alloc(nout));
+        leaf_scatter_kernel<<<grid_for(nout, kT), kT, 0, s>>>(tr.out.p, idx.p, key.p, nout, dfirst.p, doff.p,
+                                                              dpb.p, dpk.p, all.p);
+        HMG_HIP(hipGetLastError());
+        std::vector<int4> h((size_t)nout);
+        HMG_TRY(download_bytes(h.data(), all.p, sizeof(int4) * nout, s));
+        for (int q = 0; q < count; q++)
+            eng[q]->h_blk.assign(h.begin() + first[q], h.begin() + first[q] + eng[q]->nblocks);
+    } else {
+        for (int q = 0; q < count; q++) eng[q]->h_blk.clear();
+    }
     return HMG_OK;
 }
 

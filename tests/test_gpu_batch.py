@@ -107,3 +107,65 @@ def test_batch_rejects_repeated_and_unready(gpu):
     precompute_aca_batch([hm])
     assert hm.work()["rank_sum"] > 0
     hm.destroy()
+
+
+def test_fused_setup_same_as_alone(gpu):
+    """hmg_setup_batch runs the setup of clouds that share a dimension and tree
+    parameters (one point set against itself, exact cluster boxes) as one set of
+    launches per step: all clouds' coordinates side by side, extrema, Morton codes,
+    one sort, every tree in one level-by-level traversal, one partition.  Each
+    H-matrix must come out bitwise as setup_h_matrix leaves it alone: codes, order,
+    point ids, sorted coordinates, extrema of both sets, the complete block list,
+    its counts and levels, and the products built on it.  Clouds outside a group
+    (independent point sets, the reference's lookup-table boxes) take the
+    one-at-a-time path in the same call."""
+    from hmglib_amd.hmatrix import setup_batch
+    clouds = _clouds(24, seed0=4501, lo=9, span=5)
+    made = []
+    for i, (X, dim, n, nrhs, Xh) in enumerate(clouds):
+        pair = []
+        for _ in range(2):
+            hm, kern = _hm(gpu, X, dim, gpu.PREC_FP64)
+            hm.set_option(gpu.OPT_EXACT_BOXES, 1)
+            pair.append((hm, kern))
+        made.append(pair)
+    # outside the groups: two independent sets, and lookup-table boxes
+    rng = np.random.default_rng(4599)
+    Xa, Xb = rng.random((2, 3000)), rng.random((2, 2500))
+    ind = [gpu.HMatrix(Xa, Xb, 32, 1.2, 50, 64, 32) for _ in range(2)]
+    for h in ind:
+        h.set_option(gpu.OPT_EXACT_BOXES, 1)
+    lut = [_hm(gpu, clouds[0][0], clouds[0][1], gpu.PREC_FP64)[0] for _ in range(2)]
+    batch = [p[0][0] for p in made] + [ind[0], lut[0]]
+    setup_batch(batch, 8)
+    for p in made:
+        p[1][0].setup()
+    ind[1].setup()
+    lut[1].setup()
+    pairs = [(p[0][0], p[1][0]) for p in made] + [(ind[0], ind[1]), (lut[0], lut[1])]
+    for hb, hs in pairs:
+        for s in (0, 1):
+            np.testing.assert_array_equal(hb.codes(s), hs.codes(s))
+            np.testing.assert_array_equal(hb.order(s), hs.order(s))
+            np.testing.assert_array_equal(hb.point_ids(s), hs.point_ids(s))
+            np.testing.assert_array_equal(hb.sorted_coords(s), hs.sorted_coords(s))
+            for a, b in zip(hb.minmax(s), hs.minmax(s)):
+                np.testing.assert_array_equal(a, b)
+        bb, bs = hb.blocks(O.WORK_ITEM_DTYPE), hs.blocks(O.WORK_ITEM_DTYPE)
+        assert bb.tobytes() == bs.tobytes()
+        assert hb.work() == hs.work()
+        assert hb.schedule() == hs.schedule()
+    # products on the fused setup: bitwise those of the H-matrix set up alone
+    for i in (0, 1, 7, 23):
+        X, dim, n, nrhs, Xh = clouds[i]
+        (hb, kb), (hs, ks) = made[i]
+        for hm, kern in ((hb, kb), (hs, ks)):
+            hm.set_kernel(kern)
+            hm.precompute_aca()
+        np.testing.assert_array_equal(hb.apply(Xh), hs.apply(Xh))
+    for p in made:
+        for hm, kern in p:
+            hm.destroy()
+            kern.free()
+    for h in ind + lut:
+        h.destroy()
