@@ -1277,7 +1277,11 @@ hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value)
         e->fp64_block_min = (int)value;
         return HMG_OK;
     case HMG_OPT_LR_PRIORITY:
-        e->lr_prio = value != 0;
+        if (value < 0 || value > 3) {
+            set_error("low-rank priority must be 0..3 (bit 0: V^T X, bit 1: U T)");
+            return HMG_ERR_INVALID;
+        }
+        e->lr_prio = (int)value;
         return HMG_OK;
     case HMG_OPT_ACA_LDS_BYTES:
         if (value < 0 || value > 96 * 1024) {

@@ -483,8 +483,9 @@ struct Engine {
     std::vector<int> h_mx_wg;
     DevBuf<int> mx_wg;
     bool near_wg = false;
-    // mixed precision: low-rank wavefronts issue at a raised priority (s_setprio)
-    bool lr_prio = false;
+    // mixed precision: low-rank wavefronts issue at a raised priority (s_setprio):
+    // bit 0 V^T X, bit 1 U T
+    int lr_prio = 0;
     // mixed-precision ACA: its FP64 scratch holds 1/aca_mixed_parts of the factors
     int aca_mixed_parts = 16;
     // unused LDS reserved by every workgroup of the main ACA launch (caps the blocks

@@ -906,7 +906,7 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
             }
             eng.prof.end(PROF_LR_U, s);
         } else if (eng.naca > 0 && eng.n_lr_items > 0) {
-            const int prio = eng.lr_prio ? 1 : 0;
+            const int prio_v = eng.lr_prio & 1, prio_u = (eng.lr_prio >> 1) & 1;
             eng.prof.begin(PROF_LR_VTX, s);
             // items grouped by interleaved tile count: [4 | 2 | 1]
             for (int g = 0; g < 3; g++) {
@@ -916,13 +916,13 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
                 const unsigned gr = grid_for(e - b, kWaves);
                 if (g == 0)
                     lr_vb_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xp.p, eng.tmap.p, eng.Tpan.p, eng.Tp32.p,
-                                                    prio);
+                                                    prio_v);
                 else if (g == 1)
                     lr_vb_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xp.p, eng.tmap.p, eng.Tpan.p, eng.Tp32.p,
-                                                    prio);
+                                                    prio_v);
                 else
                     lr_vb_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, V, eng.Xp.p, eng.tmap.p, eng.Tpan.p, eng.Tp32.p,
-                                                    prio);
+                                                    prio_v);
                 HMG_HIP(hipGetLastError());
             }
             if (eng.n_lv_slices > 0) {
@@ -940,11 +940,11 @@ hmg_status hmvp_mixed(Engine& eng, const Coords& c1, const Coords& c2, const Ker
                 float* Pg = eng.P32.p + (int64_t)b * kMxLrRows * kMxRhs;
                 const unsigned gr = grid_for(e - b, kWaves);
                 if (g == 0)
-                    lr_ub_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg, prio);
+                    lr_ub_kernel<4><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg, prio_u);
                 else if (g == 1)
-                    lr_ub_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg, prio);
+                    lr_ub_kernel<2><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg, prio_u);
                 else
-                    lr_ub_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg, prio);
+                    lr_ub_kernel<1><<<gr, kThreads, 0, s>>>(li, e - b, U, eng.Tpan.p, Pg, prio_u);
                 HMG_HIP(hipGetLastError());
             }
             eng.prof.end(PROF_LR_U, s);
