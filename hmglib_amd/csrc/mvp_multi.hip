@@ -217,9 +217,12 @@ lr_u_multi_kernel(const LrItem* __restrict__ items, int nitems, const double* __
 }
 
 // Near field on the fly over the stored-near-field schedule (items {row0, nrows <=
-// 128, b0, b1} over near_blk), as FP64 MFMA GEMMs: M = 64 rows per pass (four
-// 16-row tiles; two passes for 128 rows), N = right-hand sides, K = the item's
-// dense blocks' columns in block order, four per step.  Lane l evaluates the
+// 128, b0, b1} over near_blk), as FP64 MFMA GEMMs: M = 32 rows per pass over the
+// columns (two 16-row tiles; four passes for 128 rows), N = right-hand sides, K =
+// the item's dense blocks' columns in block order, four per step.  Two tiles per
+// pass keep the kernel at 140 registers (three wavefronts per SIMD, launched
+// without an LDS reservation); four tiles per pass took 248 (two per SIMD) and
+// 86 ms alone at N = 2^20 against 76 ms.  Lane l evaluates the
 // entries it feeds as A operand: rows 16t + l % 16 against column l / 16 of the
 // step (each entry evaluated once for all right-hand sides).  Columns past a
 // block's end enter as zeros.  P[(q*128 + r)*RB + c].
@@ -236,19 +239,21 @@ near_multi_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict_
     const int4 it = items[q];
     const int dim = DIM > 0 ? DIM : kp.dim;
     const int i16 = lane & 15, g = lane >> 4;
-    for (int half = 0; half * kWave < it.y; half++) {
-        int row[4];
-        double xi[4][MAX_POINT_DIMENSION];
+    constexpr int NT = 2;          // 16-row tiles per pass over the columns
+    constexpr int PR = 16 * NT;    // rows per pass
+    for (int half = 0; half * PR < it.y; half++) {
+        int row[NT];
+        double xi[NT][MAX_POINT_DIMENSION];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int rr = half * kWave + 16 * t + i16;
+        for (int t = 0; t < NT; t++) {
+            const int rr = half * PR + 16 * t + i16;
             row[t] = it.x + (rr < it.y ? rr : 0);  // rows past the item: its first row
 #pragma unroll
             for (int d = 0; d < MAX_POINT_DIMENSION; d++) xi[t][d] = d < dim ? c1.x[d][row[t]] : 0.0;
         }
-        d4 acc[4][NB];
+        d4 acc[NT][NB];
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int t = 0; t < NT; t++)
 #pragma unroll
             for (int h = 0; h < NB; h++) acc[t][h] = d4{0.0, 0.0, 0.0, 0.0};
         // steps (block bi, first column j0) four columns at a time; lane l takes
@@ -292,9 +297,9 @@ near_multi_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict_
                 bool okn;
                 double xjn[MAX_POINT_DIMENSION], bxn[NB];
                 load(j0n, bn.w, ccn, okn, xjn, bxn);
-                double a[4];
+                double a[NT];
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
+                for (int t = 0; t < NT; t++) {
                     double r2 = 0.0;
 #pragma unroll
                     for (int d = 0; d < MAX_POINT_DIMENSION; d++) {
@@ -308,7 +313,7 @@ near_multi_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict_
                     a[t] = ok ? kv : 0.0;
                 }
 #pragma unroll
-                for (int t = 0; t < 4; t++)
+                for (int t = 0; t < NT; t++)
 #pragma unroll
                     for (int h = 0; h < NB; h++) acc[t][h] = mfma64(a[t], bx[h], acc[t][h]);
                 if (bin >= it.w) break;
@@ -323,13 +328,13 @@ near_multi_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict_
                 for (int h = 0; h < NB; h++) bx[h] = bxn[h];
             }
         }
-        double* __restrict__ out = P + ((int64_t)q * 2 * kTileRows + half * kWave) * RB;
+        double* __restrict__ out = P + ((int64_t)q * 2 * kTileRows + half * PR) * RB;
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int t = 0; t < NT; t++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int rr = 16 * t + 4 * r + g;
-                if (half * kWave + rr < it.y) {
+                if (half * PR + rr < it.y) {
 #pragma unroll
                     for (int h = 0; h < NB; h++) out[(int64_t)rr * RB + 16 * h + i16] = acc[t][h][r];
                 }
@@ -382,7 +387,7 @@ template <int KIND, int DIM, int RB>
 void launch_near_multi(hipStream_t s, const Coords& c1, const Coords& c2, const KernelParams& kp,
                        const Engine& eng, double* Pn)
 {
-    near_multi_kernel<KIND, DIM, RB><<<grid_for(eng.n_near_items, kWaves), kThreads, eng.near_lds_pad, s>>>(
+    near_multi_kernel<KIND, DIM, RB><<<grid_for(eng.n_near_items, kWaves), kThreads, 0, s>>>(
         c1, c2, kp, eng.near_items.p, eng.n_near_items, eng.near_blk.p, eng.blk.p, eng.X64.p, Pn);
 }
 
