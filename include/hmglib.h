@@ -131,8 +131,13 @@ hmg_status hmg_apply_h_matrix_mvp_multi(const double* X, double* Y, int nrhs, st
  * independent point clouds).  Each entry of data[] is an H-matrix of its own
  * (init_h_matrix_data, coordinates, parameters as usual); threads = host threads
  * that drive them (0: 8).
- * hmg_setup_batch: setup_h_matrix of every entry, the entries' setups running
- *   concurrently on their own streams.
+ * hmg_setup_batch: setup_h_matrix of every entry.  Entries with one point set
+ *   against itself (the two sets bitwise equal), exact cluster boxes
+ *   (HMG_OPT_EXACT_BOXES), root levels 0 and the same dimension, code width and
+ *   tree parameters are set up together, one set of launches per step for all of
+ *   them (one sort, every tree in one level-by-level traversal); every other entry
+ *   is set up alone, `threads` at a time.  Either way each entry ends exactly as
+ *   setup_h_matrix leaves it.
  * hmg_precompute_aca_batch: precompute_aca of every entry (after setup and with
  *   data[i]->assem set), in ONE launch set for all entries that share a kernel,
  *   point dimension, stopping rule and precision: their admissible blocks run side
