@@ -175,6 +175,10 @@ SIGNATURES = {
     "hmg_release_cached_memory": (None, []),
     "hmg_memory": (ctypes.c_int, [_D, _P(ctypes.c_int64)]),
     "hmg_setup_batch": (ctypes.c_int, [_P(_D), ctypes.c_int, ctypes.c_int]),
+    "hmg_apply_batch_graph_create": (ctypes.c_int, [_P(_D), _P(_vp), _P(_vp), _P(ctypes.c_int), ctypes.c_int,
+                                                    _P(_vp)]),
+    "hmg_apply_batch_graph_launch": (ctypes.c_int, [_vp]),
+    "hmg_apply_batch_graph_destroy": (None, [_vp]),
     "hmg_precompute_aca_batch": (ctypes.c_int, [_P(_D), ctypes.c_int, ctypes.c_int]),
     "hmg_apply_batch": (ctypes.c_int, [_P(_D), _P(_vp), _P(_vp), _P(ctypes.c_int), ctypes.c_int,
                                        ctypes.c_int]),

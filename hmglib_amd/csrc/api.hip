@@ -330,6 +330,7 @@ int aca_work_size_at(const std::vector<int4>& bl, int nblocks, int64_t max_size,
 
 void release_factors(Engine& e, h_matrix_data* d)
 {
+    e.epoch++;
     e.U.release();
     e.V.release();
     e.Up.release();
@@ -712,6 +713,7 @@ void release_build_state(Engine& e, h_matrix_data* d) { release_build(e, d); }
 hmg_status finish_setup(h_matrix_data* data, Engine* e)
 {
     AllocScope scope(e->stream);
+    e->epoch++;
     data->mat_vec_data = (work_item**)malloc(sizeof(work_item*));
     *data->mat_vec_data = e->blocks.p;
     data->mat_vec_data_count = e->nblocks;
@@ -791,6 +793,7 @@ hmg_status hmg_precompute_aca(struct h_matrix_data* data)
         data->V = e->V.p;
     }
     e->fact_prec = e->precision;
+    e->epoch++;
     return HMG_OK;
 }
 
@@ -822,6 +825,42 @@ hmg_status hmg_precompute_dense(struct h_matrix_data* data)
 }
 
 // ------------------------------------------------------------------ apply
+}  // extern "C"
+
+namespace hmg {
+
+hmg_status apply_enqueue(h_matrix_data* data, Engine* e, const KernelParams& kp, const double* X, double* Y,
+                         int nrhs)
+{
+    if (!(data->U != nullptr && e->aca_k >= 0)) {
+        set_error("product of precomputed factors needs precompute_aca first");
+        return HMG_ERR_NOT_READY;
+    }
+    if (e->fact_prec == HMG_PREC_MIXED)
+        return hmvp_mixed(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[0], data->order[1], X, Y,
+                          nrhs);
+    const bool pre_dense = data->dA != nullptr && e->dense_stored;
+    if (nrhs >= e->fp64_block_min && !pre_dense)
+        return hmvp_multi(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[0], data->order[1], X, Y,
+                          nrhs, e->U.p, e->V.p);
+    return hmvp(*e, coords_of(data, 0), coords_of(data, 1), kp, data->order[0], data->order[1], X, Y, nrhs,
+                e->U.p, e->V.p, e->keff.p, pre_dense);
+}
+
+std::vector<const void*> product_state(const h_matrix_data* d, const Engine& e)
+{
+    return {d->U, d->V, d->dA, e.U.p, e.V.p, e.Up.p, e.Vp.p, e.A.p, e.keff.p, e.blk.p, e.lr_items.p,
+            e.lv_items.p, e.lv_slices.p, e.tmap.p, e.near_items.p, e.near_blk.p, e.fly_items.p, e.fly_refs.p,
+            e.mx_items.p, e.mx_refs.p, e.mx_wg.p, e.vw_items.p, e.uw_items.p, e.comb_stored.idx.p,
+            e.comb_fly.idx.p, e.comb_mx.idx.p, e.xs.p, e.Tp.p, e.T.p, e.P.p, e.X64.p, e.Tp64.p, e.T64.p,
+            e.P64.p, e.Ys64.p, e.inv1.p, e.Xp.p, e.Xt32.p, e.Tpan.p, e.Tp32.p, e.P32.p, e.Ys32.p,
+            e.cpan[0].p, e.cpan[1].p, e.mxp.p, d->assem, (const void*)(uintptr_t)e.epoch};
+}
+
+}  // namespace hmg
+
+extern "C" {
+
 hmg_status hmg_apply_h_matrix_mvp_multi(const double* X, double* Y, int nrhs, struct h_matrix_data* data)
 {
     Engine* e;
@@ -1215,6 +1254,7 @@ hmg_status hmg_set_option(struct h_matrix_data* data, int option, int64_t value)
         set_error("h_matrix_data not initialised");
         return HMG_ERR_NOT_READY;
     }
+    e->epoch++;
     switch (option) {
     case HMG_OPT_NEAR_SYMMETRIC:
         e->near_sym_enabled = value != 0;

@@ -24,6 +24,7 @@
 #include <atomic>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <thread>
 #include <tuple>
 
@@ -685,6 +686,136 @@ extern "C" hmg_status hmg_precompute_aca_batch(struct h_matrix_data* const* data
             data[i]->V = e->V.p;
         }
         e->fact_prec = e->precision;
+        e->epoch++;
     }
     return HMG_OK;
 }
+
+// ---- the many-cloud product as one captured hipGraph -----------------------------
+// The products of the listed H-matrices (precomputed factors) are captured from
+// one stream into one graph: the capture stream forks to every H-matrix's own
+// streams through an event, each H-matrix's product is enqueued there (its near
+// field on its second stream as usual), and every H-matrix joins back.  A launch
+// replays every kernel of every product with one call.  The graph holds the
+// device pointers of that moment (operands, factors, schedules, product
+// buffers); a launch checks that none of them has changed since (a new setup,
+// precompute, precision or a wider product on one of the H-matrices) and refuses
+// otherwise.
+struct hmg_batch_graph {
+    hipStream_t s = nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    std::vector<hipEvent_t> ev;
+    std::vector<h_matrix_data*> data;
+    std::vector<std::vector<const void*>> state;
+    ~hmg_batch_graph()
+    {
+        if (s) (void)hipStreamSynchronize(s);
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+extern "C" hmg_status hmg_apply_batch_graph_create(struct h_matrix_data* const* data, const double* const* X,
+                                                   double* const* Y, const int* nrhs, int count,
+                                                   hmg_batch_graph** out)
+{
+    using namespace hmg;
+    if (!out || !data || !X || !Y || !nrhs || count < 1) {
+        set_error("invalid batch graph arguments");
+        return HMG_ERR_INVALID;
+    }
+    *out = nullptr;
+    std::vector<Engine*> eng((size_t)count);
+    std::vector<KernelParams> kp((size_t)count);
+    for (int q = 0; q < count; q++) {
+        HMG_TRY(check_ready(data[q], &eng[q]));
+        for (int j = 0; j < q; j++)
+            if (data[j] == data[q]) {
+                set_error("batch graph: an H-matrix is listed twice");
+                return HMG_ERR_INVALID;
+            }
+        if (!X[q] || !Y[q] || nrhs[q] < 1) {
+            set_error("batch graph: invalid operands of H-matrix " + std::to_string(q));
+            return HMG_ERR_INVALID;
+        }
+        if (!(data[q]->U != nullptr && eng[q]->aca_k >= 0)) {
+            set_error("batch graph: H-matrix " + std::to_string(q) + " has no precomputed factors");
+            return HMG_ERR_NOT_READY;
+        }
+        HMG_TRY(read_kernel(data[q], &kp[q]));
+    }
+    // one eager product each: the product buffers of these widths exist afterwards
+    // (and the first-use clears are done), so the captured products allocate nothing
+    for (int q = 0; q < count; q++) {
+        AllocScope scope(eng[q]->stream);
+        HMG_TRY(apply_enqueue(data[q], eng[q], kp[q], X[q], Y[q], nrhs[q]));
+    }
+    for (int q = 0; q < count; q++) HMG_HIP(hipStreamSynchronize(eng[q]->stream));
+    auto* gr = new hmg_batch_graph();
+    std::unique_ptr<hmg_batch_graph> own(gr);
+    gr->data.assign(data, data + count);
+    for (int q = 0; q < count; q++) gr->state.push_back(product_state(data[q], *eng[q]));
+    HMG_HIP(hipStreamCreateWithFlags(&gr->s, hipStreamNonBlocking));
+    gr->ev.resize((size_t)2 * count, nullptr);
+    for (hipEvent_t& e : gr->ev) HMG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    std::vector<bool> prof((size_t)count);
+    for (int q = 0; q < count; q++) {
+        prof[q] = eng[q]->prof.on;
+        eng[q]->prof.on = false;  // no timing events inside the graph
+    }
+    HMG_HIP(hipStreamBeginCapture(gr->s, hipStreamCaptureModeRelaxed));
+    hmg_status st = HMG_OK;
+    for (int q = 0; q < count && st == HMG_OK; q++) {
+        Engine* e = eng[q];
+        AllocScope scope(e->stream);
+        if (hipEventRecord(gr->ev[2 * q], gr->s) != hipSuccess ||
+            hipStreamWaitEvent(e->stream, gr->ev[2 * q], 0) != hipSuccess) {
+            set_error("batch graph: fork into the capture failed");
+            st = HMG_ERR_HIP;
+            break;
+        }
+        st = apply_enqueue(data[q], e, kp[q], X[q], Y[q], nrhs[q]);
+        if (st == HMG_OK && (hipEventRecord(gr->ev[2 * q + 1], e->stream) != hipSuccess ||
+                             hipStreamWaitEvent(gr->s, gr->ev[2 * q + 1], 0) != hipSuccess)) {
+            set_error("batch graph: join into the capture failed");
+            st = HMG_ERR_HIP;
+        }
+    }
+    const hipError_t ec = hipStreamEndCapture(gr->s, &gr->g);
+    for (int q = 0; q < count; q++) eng[q]->prof.on = prof[q];
+    HMG_TRY(st);
+    HMG_HIP(ec);
+    for (int q = 0; q < count; q++)
+        if (product_state(data[q], *eng[q]) != gr->state[q]) {
+            set_error("batch graph: a product allocated while being captured");
+            return HMG_ERR_INVALID;
+        }
+    HMG_HIP(hipGraphInstantiate(&gr->x, gr->g, nullptr, nullptr, 0));
+    *out = own.release();
+    return HMG_OK;
+}
+
+extern "C" hmg_status hmg_apply_batch_graph_launch(hmg_batch_graph* g)
+{
+    using namespace hmg;
+    if (!g || !g->x) {
+        set_error("invalid batch graph");
+        return HMG_ERR_INVALID;
+    }
+    for (size_t q = 0; q < g->data.size(); q++) {
+        Engine* e = engine_of(g->data[q]);
+        if (!e || product_state(g->data[q], *e) != g->state[q]) {
+            set_error("batch graph: H-matrix " + std::to_string(q) +
+                      " changed since the capture (capture the graph again)");
+            return HMG_ERR_NOT_READY;
+        }
+    }
+    HMG_HIP(hipGraphLaunch(g->x, g->s));
+    HMG_HIP(hipStreamSynchronize(g->s));
+    return HMG_OK;
+}
+
+extern "C" void hmg_apply_batch_graph_destroy(hmg_batch_graph* g) { delete g; }

@@ -349,6 +349,7 @@ struct Engine {
     int dim = 0;
     int bits = 0;
     bool built = false;
+    uint64_t epoch = 0;  // bumped by every change of set-up, factors or options
 
     DevBuf<uint32_t> perm32[2];                 // sort values
     DevBuf<work_item> blocks;                   // final block list (DENSE first)
@@ -561,6 +562,14 @@ hmg_status build_trees_batched(hipStream_t s, int dim, const Coords& c, const ui
 hmg_status finish_setup(h_matrix_data* data, Engine* e);
 // returns what a previous setup built (the state a new setup starts from; api.hip)
 void release_build_state(Engine& e, h_matrix_data* d);
+// The product of precomputed factors enqueued on the engine's streams: no host
+// synchronisation, and no allocation once the engine's product buffers exist for
+// this width (what a hipGraph capture needs; api.hip).
+hmg_status apply_enqueue(h_matrix_data* data, Engine* e, const KernelParams& kp, const double* X, double* Y,
+                         int nrhs);
+// the device pointers an enqueued product reads or writes (schedule, factors,
+// product buffers): a captured product is valid while they stay the same
+std::vector<const void*> product_state(const h_matrix_data* d, const Engine& e);
 
 // Host <-> device copies of the set-up path (api.hip).  The bytes pass through a
 // pinned buffer of the calling thread (at most 4 MiB, allocated once per thread),

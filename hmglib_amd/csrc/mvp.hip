@@ -463,6 +463,7 @@ hmg_status dense_fill(Engine& eng, const Coords& c1, const Coords& c2, const Ker
     }
     HMG_HIP(hipStreamSynchronize(s));
     eng.dense_stored = true;
+    eng.epoch++;
     return HMG_OK;
 }
 

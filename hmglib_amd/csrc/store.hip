@@ -403,6 +403,7 @@ hmg_status hmg_load(struct h_matrix_data* data, const char* path)
     data->U = (double*)U;
     data->V = (double*)V;
     e->fact_prec = h.precision;
+    e->epoch++;
     return HMG_OK;
 }
 

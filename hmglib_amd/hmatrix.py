@@ -283,6 +283,36 @@ def precompute_aca_batch(hms, threads=8):
                "hmg_precompute_aca_batch")
 
 
+class BatchGraph:
+    """hmg_apply_batch_graph_*: the products of `hms` on the device arrays `xds` ->
+    `yds` (nrhs[i] columns each) captured once into one hipGraph; launch() replays
+    all of them.  The arrays stay bound to the graph; their contents may change."""
+
+    def __init__(self, hms, xds, yds, nrhs):
+        n = len(hms)
+        self._keep = (hms, xds, yds)
+        X = (ctypes.c_void_p * n)(*[x.ptr for x in xds])
+        Y = (ctypes.c_void_p * n)(*[y.ptr for y in yds])
+        R = (ctypes.c_int * n)(*[int(r) for r in nrhs])
+        self._g = ctypes.c_void_p()
+        _abi.check(_abi.lib().hmg_apply_batch_graph_create(_data_array(hms), X, Y, R, n, ctypes.byref(self._g)),
+                   "hmg_apply_batch_graph_create")
+
+    def launch(self):
+        _abi.check(_abi.lib().hmg_apply_batch_graph_launch(self._g), "hmg_apply_batch_graph_launch")
+
+    def destroy(self):
+        if self._g:
+            _abi.lib().hmg_apply_batch_graph_destroy(self._g)
+            self._g = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 def apply_batch(hms, xds, yds, nrhs, threads=8):
     """hmg_apply_batch: yds[i] = H_i xds[i] (device arrays, nrhs[i] columns each),
     the products running concurrently."""

@@ -151,6 +151,21 @@ hmg_status hmg_setup_batch(struct h_matrix_data* const* data, int count, int thr
 hmg_status hmg_precompute_aca_batch(struct h_matrix_data* const* data, int count, int threads);
 hmg_status hmg_apply_batch(struct h_matrix_data* const* data, const double* const* X, double* const* Y,
                            const int* nrhs, int count, int threads);
+/* The products of hmg_apply_batch captured once into one hipGraph and replayed
+ * (no counterpart in the reference).  _create runs each product once (so that its
+ * buffers exist), then captures all of them for the given operands (device
+ * pointers, fixed for the graph's life; their contents may change between
+ * launches); _launch replays every product and waits.  A launch refuses
+ * (HMG_ERR_NOT_READY) when an H-matrix changed since the capture (setup,
+ * precompute, hmg_set_option, another kernel object, a wider product that grew
+ * its buffers); capture again then.  The kernel object's parameters are read at
+ * capture.  Each H-matrix needs precomputed factors.  No other work may run on
+ * the listed H-matrices during a launch. */
+typedef struct hmg_batch_graph hmg_batch_graph;
+hmg_status hmg_apply_batch_graph_create(struct h_matrix_data* const* data, const double* const* X,
+                                        double* const* Y, const int* nrhs, int count, hmg_batch_graph** out);
+hmg_status hmg_apply_batch_graph_launch(hmg_batch_graph* graph);
+void hmg_apply_batch_graph_destroy(hmg_batch_graph* graph);
 /* Y = H X for an H-matrix with precomputed FP64 factors, allocating nothing (no
  * counterpart in the reference; for callers that cannot allocate device memory:
  * code under stream capture, serving loops that forbid allocation).  Same kernels

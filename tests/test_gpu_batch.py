@@ -169,3 +169,51 @@ def test_fused_setup_same_as_alone(gpu):
             kern.free()
     for h in ind + lut:
         h.destroy()
+
+
+def test_batch_graph_replays_equal_eager_products(gpu):
+    """The products of many H-matrices captured into one hipGraph (FP64 one- and
+    several-right-hand-side paths and mixed precision in the same graph) replay
+    bitwise the eager products, launch after launch, with new right-hand sides
+    and other eager work on the same H-matrices between the launches; a launch
+    after one H-matrix's factors were rebuilt is refused."""
+    from hmglib_amd import hip
+    from hmglib_amd.hmatrix import BatchGraph, precompute_aca_batch, setup_batch
+    clouds = _clouds(9, seed0=4701)
+    hms, kerns = [], []
+    for i, (X, dim, n, nrhs, Xh) in enumerate(clouds):
+        hm, kern = _hm(gpu, X, dim, gpu.PREC_MIXED if i % 3 == 2 else gpu.PREC_FP64)
+        hm.set_option(gpu.OPT_EXACT_BOXES, 1)
+        hms.append(hm)
+        kerns.append(kern)
+    setup_batch(hms, 4)
+    for hm, kern in zip(hms, kerns):
+        hm.set_kernel(kern)
+    precompute_aca_batch(hms, 4)
+    nrhs = [1 if i % 3 == 0 else c[3] for i, c in enumerate(clouds)]
+    xds = [hip.DeviceArray(np.asfortranarray(c[4][:, :r])) for c, r in zip(clouds, nrhs)]
+    yds = [hip.DeviceArray.empty((c[2], r)) for c, r in zip(clouds, nrhs)]
+    g = BatchGraph(hms, xds, yds, nrhs)
+    rng = np.random.default_rng(4799)
+    try:
+        for rep in range(3):
+            Xs = [np.asfortranarray(rng.standard_normal((c[2], r))) for c, r in zip(clouds, nrhs)]
+            for xd, Xh in zip(xds, Xs):
+                xd.upload(Xh)
+            # eager products on the same H-matrices between the launches
+            for hm, Xh in zip(hms[::2], Xs[::2]):
+                hm.apply(Xh)
+            g.launch()
+            for hm, yd, Xh, r in zip(hms, yds, Xs, nrhs):
+                Y = yd.download().reshape(-1, r, order="F")
+                np.testing.assert_array_equal(Y, hm.apply(Xh).reshape(-1, r, order="F"))
+        hms[1].precompute_aca()
+        with pytest.raises(gpu.HMGError):
+            g.launch()
+    finally:
+        g.destroy()
+        for a in xds + yds:
+            a.free()
+        for hm, kern in zip(hms, kerns):
+            hm.destroy()
+            kern.free()
