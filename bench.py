@@ -521,7 +521,7 @@ def batch_api_session(clouds, precision, log_tag, threads=8, exact_boxes=True):
     in ONE ACA launch set), then hmg_apply_batch.  Times each stage."""
     import hmglib_amd
     from hmglib_amd import hip
-    from hmglib_amd.hmatrix import HMatrix, Kernel, apply_batch, precompute_aca_batch, setup_batch
+    from hmglib_amd.hmatrix import BatchGraph, HMatrix, Kernel, apply_batch, precompute_aca_batch, setup_batch
     nmax = max(c["n"] for c in clouds)
     Xpool_h = np.random.default_rng(9).standard_normal((nmax * 8,))
     t0 = time.perf_counter()
@@ -550,6 +550,19 @@ def batch_api_session(clouds, precision, log_tag, threads=8, exact_boxes=True):
     apply_batch(hms, xds, yds, [c["nrhs"] for c in clouds], threads)
     hip.synchronize()
     t5 = time.perf_counter()
+    # the same products captured into one hipGraph: capture, then one timed replay,
+    # whose results must equal the eager ones bitwise
+    eager = [yds[i].download() for i in range(0, len(clouds), 32)]
+    t6 = time.perf_counter()
+    graph = BatchGraph(hms, xds, yds, [c["nrhs"] for c in clouds])
+    t7 = time.perf_counter()
+    replays = []
+    for _ in range(3):
+        tr = time.perf_counter()
+        graph.launch()
+        replays.append(time.perf_counter() - tr)
+    graph_equal = all(np.array_equal(yds[i].download(), eager[j]) for j, i in enumerate(range(0, len(clouds), 32)))
+    graph.destroy()
     errs = []
     for i in range(0, len(clouds), 32):
         c = clouds[i]
@@ -566,6 +579,8 @@ def batch_api_session(clouds, precision, log_tag, threads=8, exact_boxes=True):
     wall = (t3 - t0) + (t5 - t4)
     res = {"clouds": len(clouds), "rhs": rhs, "wall_s": wall, "clouds_per_s": len(clouds) / wall,
            "init_s": t1 - t0, "setup_s": t2 - t1, "aca_s": t3 - t2, "apply_s": t5 - t4,
+           "apply_graph_capture_s": t7 - t6, "apply_graph_first_replay_s": replays[0],
+           "apply_graph_replay_s": min(replays[1:]), "graph_equal_eager": graph_equal,
            "rel_err_exact_rows_max": max(e["exact_rows"] for e in errs) if errs else None,
            "rel_err_vs_fp64_hmvp_max": max(e["vs_fp64_hmvp"] for e in errs) if errs else None,
            "rel_err_sampled_clouds": len(errs)}
@@ -799,10 +814,11 @@ def main():
         for name, prec in (("fp64", hmglib_amd.PREC_FP64), ("mixed", hmglib_amd.PREC_MIXED)):
             batched[name] = batch_session(clouds, prec, args.batch_slots, name)
             log_free(f"what follows the {name} batching leg")
-        batched["batch_api"] = {"what": "hmg_setup_batch (8 host threads, a stream per cloud) + "
-                                        "hmg_precompute_aca_batch (one ACA launch set per dimension across "
+        batched["batch_api"] = {"what": "hmg_setup_batch (clouds of equal tree parameters set up together, "
+                                        "one launch set per step) + hmg_precompute_aca_batch (one ACA launch set per dimension across "
                                         "all clouds) + hmg_apply_batch; host-to-device copies of the "
-                                        "right-hand sides excluded"}
+                                        "right-hand sides excluded; then the products captured into one "
+                                        "hipGraph (hmg_apply_batch_graph_create) and replayed once"}
         for name, prec in (("fp64", hmglib_amd.PREC_FP64), ("mixed", hmglib_amd.PREC_MIXED)):
             batched["batch_api"][name] = batch_api_session(clouds, prec, name)
             log_free(f"what follows the {name} batch-API leg")

@@ -174,6 +174,11 @@ struct AcaTol {
 // largest rank step that scans (its U terms are staged per wavefront in LDS;
 // later steps keep the one-row search).
 constexpr int kScanRows = 4;
+// Factor columns per pass over the rows of V in the tolerance stop's cross terms
+// (each lane reads that many consecutive values of its row; the U side goes
+// kScanRows columns at a time, each a coalesced column read).
+constexpr int kCrossCols = 8;
+static_assert(kCrossCols >= 2 * kScanRows, "team sums share one LDS array");
 // Smallest block width whose rejected-row search first bounds the residual rows
 // (below it a batch of rows costs about as much as the bounds).
 constexpr int kBoundMinCols = 512;
@@ -544,28 +549,44 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
             nu = team.sum(nu);
             nv = team.sum(nv);
             double cross = 0.0;
-            for (int l0 = 0; l0 < r; l0 += kScanRows) {
-                double pu[kScanRows], pv[kScanRows];
+            for (int l0 = 0; l0 < r; l0 += kCrossCols) {
+                // u_l . u_r of the columns l0 .. l0 + kCrossCols - 1, kScanRows columns
+                // per pass over the rows of U (each a coalesced column read), kept in
+                // the wavefront's LDS slice
+                for (int c0 = l0; c0 < r && c0 < l0 + kCrossCols; c0 += kScanRows) {
+                    double pu[kScanRows];
 #pragma unroll
-                for (int e = 0; e < kScanRows; e++) pu[e] = pv[e] = 0.0;
-                for (int i = tid; i < m1; i += TS) {
-                    const double x = (double)ur[i];
+                    for (int e = 0; e < kScanRows; e++) pu[e] = 0.0;
+                    for (int i = tid; i < m1; i += TS) {
+                        const double x = (double)ur[i];
 #pragma unroll
-                    for (int e = 0; e < kScanRows; e++)
-                        if (l0 + e < r) pu[e] = __fma_rn((double)Ub[(int64_t)(l0 + e) * lu + i], x, pu[e]);
+                        for (int e = 0; e < kScanRows; e++)
+                            if (c0 + e < r) pu[e] = __fma_rn((double)Ub[(int64_t)(c0 + e) * lu + i], x, pu[e]);
+                    }
+                    team.template sum_n<kScanRows>(pu);
+                    if (lane == 0) {
+#pragma unroll
+                        for (int e = 0; e < kScanRows; e++) sU[c0 - l0 + e] = (T)pu[e];
+                    }
                 }
+                // v_l . v_r of the same columns in one pass over the rows of V (each
+                // lane reads kCrossCols consecutive values of its row)
+                double pv[kCrossCols];
+#pragma unroll
+                for (int e = 0; e < kCrossCols; e++) pv[e] = 0.0;
                 for (int j = tid; j < m2; j += TS) {
                     const T* __restrict__ vj = Vb + (int64_t)j * ld;
                     const double x = (double)vj[r];
 #pragma unroll
-                    for (int e = 0; e < kScanRows; e++)
+                    for (int e = 0; e < kCrossCols; e++)
                         if (l0 + e < r) pv[e] = __fma_rn((double)vj[l0 + e], x, pv[e]);
                 }
-                team.template sum_n<kScanRows>(pu);
-                team.template sum_n<kScanRows>(pv);
+                team.template sum_n<kCrossCols>(pv);
+                __threadfence_block();
 #pragma unroll
-                for (int e = 0; e < kScanRows; e++)
-                    if (l0 + e < r) cross = __fma_rn(pu[e], pv[e], cross);
+                for (int e = 0; e < kCrossCols; e++)
+                    if (l0 + e < r) cross = __fma_rn((double)sU[e], pv[e], cross);
+                __threadfence_block();
             }
             fro2 = fro2 + 2.0 * cross + nu * nv;
             if ((tol.mode == 2 || r % 5 == 0) && sqrt(nu) * sqrt(nv) <= tol.c * sqrt(fmax(fro2, 0.0))) break;
@@ -578,10 +599,12 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
 // One launch for every admissible block: nbig workgroups each run the ACA of one
 // large block (list[0 .. nbig)) with all kAcaTeam wavefronts; every other
 // workgroup runs kAcaTeam small blocks (list[nbig ..)), one per wavefront.  Four
-// wavefronts per SIMD: the rank loop is latency-bound, so occupancy pays.
+// wavefronts per SIMD: the rank loop is latency-bound, so occupancy pays.  The
+// tolerance stop's cross terms need more registers: three wavefronts per SIMD
+// there, which keeps them out of scratch memory.
 constexpr int kAcaTeam = 4;
 template <class T, int KIND, bool TOL>
-__global__ void __launch_bounds__(kAcaTeam * kWave) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kAcaTeam * kWave) __attribute__((amdgpu_waves_per_eu(TOL ? 3 : 4)))
 aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk,
            const work_item* __restrict__ items, const int* __restrict__ list,
            int nbig, int nsmall, const int* __restrict__ kb, const int64_t* __restrict__ uoff,
@@ -589,7 +612,7 @@ aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk,
            T* __restrict__ U, T* __restrict__ V, T* const* __restrict__ ublk, T* const* __restrict__ vblk,
            int* __restrict__ keff, int tie_last, int scan, AcaTol tol)
 {
-    __shared__ double s_v[kAcaTeam * kScanRows];
+    __shared__ double s_v[kAcaTeam * kCrossCols];
     __shared__ int s_i[kAcaTeam];
     __shared__ double s_x[kAcaTeam];
     __shared__ T sU[kAcaTeam][kScanRows * kScanL];
@@ -629,7 +652,7 @@ aca_huge_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ 
                 T* __restrict__ V, T* const* __restrict__ ublk, T* const* __restrict__ vblk,
                 int* __restrict__ keff, int tie_last, int scan, AcaTol tol)
 {
-    __shared__ double s_v[kAcaHuge * 2 * kScanRows];
+    __shared__ double s_v[kAcaHuge * kCrossCols];
     __shared__ int s_i[kAcaHuge];
     __shared__ double s_x[kAcaHuge];
     __shared__ T sU[kAcaHuge][2 * kScanRows * kScanL];

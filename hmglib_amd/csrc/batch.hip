@@ -794,6 +794,8 @@ extern "C" hmg_status hmg_apply_batch_graph_create(struct h_matrix_data* const* 
             return HMG_ERR_INVALID;
         }
     HMG_HIP(hipGraphInstantiate(&gr->x, gr->g, nullptr, nullptr, 0));
+    HMG_HIP(hipGraphUpload(gr->x, gr->s));  // the first launch then costs what later ones do
+    HMG_HIP(hipStreamSynchronize(gr->s));
     *out = own.release();
     return HMG_OK;
 }

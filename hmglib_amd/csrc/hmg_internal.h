@@ -457,7 +457,7 @@ struct Engine {
     // ACA tolerance stop: mode (option) and its constant for the current build
     int aca_tol_mode = 0;
     double aca_tol_c = 0.0;
-    int aca_big_entries = 4096;
+    int aca_big_entries = 512;
     int aca_huge_entries = 16384;               // above: an 8-wavefront workgroup, launched beside
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 

@@ -231,7 +231,7 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
  *   block whose entries are provably all too small for any row to pass the
  *   search (decreasing kernels, bound from the points' bounding boxes).  The
  *   factors are bitwise the same.
- * HMG_OPT_ACA_BIG_BLOCK [4096]: admissible blocks with more rows plus columns
+ * HMG_OPT_ACA_BIG_BLOCK [512]: admissible blocks with more rows plus columns
  *   than this run their ACA on a workgroup of 4 wavefronts instead of one
  *   wavefront (row norms summed in another order; same factors up to the
  *   rounding of those norms).
