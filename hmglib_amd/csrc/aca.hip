@@ -448,6 +448,10 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
     const int lane = threadIdx.x % kWave;
     for (int r = 0; r < k; r++) {
         bool stop = false;
+        // this lane's column-pivot candidate (|v|, index, v) of the accepted row,
+        // taken while the row is formed, in the order of the lane's columns
+        double best = -1.0, bval = 0.0;
+        int bj = -1;
         // residual-row search (compute_current_batched_v_r, :2068-2132)
         for (;;) {
             if (i_r >= m1 - 1) {
@@ -456,12 +460,23 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
             }
             i_r++;
             double nrm = 0.0;
+            best = -1.0;
+            bval = 0.0;
+            bj = -1;
             for (int j = tid; j < m2; j += TS) {
                 double val = kernel_entry_k<KIND>(kp, c1, l1 + i_r, c2, l2 + j);
                 const T* __restrict__ vrow = Vb + (int64_t)j * ld;
                 val = residual_update<T>(val, r, vrow, 1, Ub + i_r, lu);
-                Vb[(int64_t)j * ld + r] = (T)val;
+                const T stored = (T)val;
+                Vb[(int64_t)j * ld + r] = stored;
                 nrm = __fma_rn(val, val, nrm);
+                const double x = (double)stored;
+                const double ax = fabs(x);
+                if (tie_last ? (ax >= best) : (ax > best)) {
+                    best = ax;
+                    bj = j;
+                    bval = x;
+                }
             }
             nrm = team.sum(nrm);
             if (sqrt(nrm) >= 1.0e-13) break;
@@ -501,40 +516,32 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
             for (int j = tid; j < m2; j += TS) Vb[(int64_t)j * ld + r] = (T)0;
             break;
         }
-        // column pivot (tuple_absolute_maximum, :991-1018)
-        double best = -1.0, bval = 0.0;
-        int bj = -1;
-        for (int j = tid; j < m2; j += TS) {
-            const double x = (double)Vb[(int64_t)j * ld + r];
-            const double ax = fabs(x);
-            if (tie_last ? (ax >= best) : (ax > best)) {
-                best = ax;
-                bj = j;
-                bval = x;
-            }
-        }
+        // column pivot (tuple_absolute_maximum, :991-1018) over the lanes' candidates
         team.argmax(best, bj, bval, tie_last);
         const int jr = bj;
         const double p = bval;
+        // v_r /= p and the residual column (batched_fill_kernel_vector_and_scaled_
+        // substraction_for_u_r) in one pass: the column reads columns l < r of V only
         double vv = 0.0;
-        for (int j = tid; j < m2; j += TS) {
-            const T v = (T)((double)Vb[(int64_t)j * ld + r] / p);
-            Vb[(int64_t)j * ld + r] = v;
-            vv = __fma_rn((double)v, (double)v, vv);
+        const T* __restrict__ vpiv = Vb + (int64_t)jr * ld;
+        T* __restrict__ u = Ub + r * lu;
+        const int mm = m1 > m2 ? m1 : m2;
+        for (int t = tid; t < mm; t += TS) {
+            if (t < m2) {
+                const T v = (T)((double)Vb[(int64_t)t * ld + r] / p);
+                Vb[(int64_t)t * ld + r] = v;
+                vv = __fma_rn((double)v, (double)v, vv);
+            }
+            if (t < m1) {
+                double val = kernel_entry_k<KIND>(kp, c1, l1 + t, c2, l2 + jr);
+                val = residual_update<T>(val, r, Ub + t, lu, vpiv, 1);
+                u[t] = (T)val;
+            }
         }
         if (bound_rows && r < kScanL) {
             // ||V(:,r)|| for the residual bounds, rounded up
             vv = team.sum(vv);
             if (lane == 0) vn[r] = sqrt(vv) * (1.0 + 1e-9);
-        }
-        team.sync();
-        // residual column (batched_fill_kernel_vector_and_scaled_substraction_for_u_r)
-        const T* __restrict__ vpiv = Vb + (int64_t)jr * ld;
-        T* __restrict__ u = Ub + r * lu;
-        for (int i = tid; i < m1; i += TS) {
-            double val = kernel_entry_k<KIND>(kp, c1, l1 + i, c2, l2 + jr);
-            val = residual_update<T>(val, r, Ub + i, lu, vpiv, 1);
-            u[i] = (T)val;
         }
         team.sync();
         ke = r + 1;
