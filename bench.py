@@ -355,6 +355,7 @@ def serve_session(reqs, precision, budget, slots, log_tag, trace_path=None, exac
     lat = {"build": [], "apply": []}
     svc = {"build": [], "apply": []}
     fails = 0
+    fail_msgs = []
     trace = []
     srv_status = {}
     for rid, kind, r in ids:
@@ -365,6 +366,8 @@ def serve_session(reqs, precision, budget, slots, log_tag, trace_path=None, exac
                       "merged": info["merged"]})
         if info["status"] != 0:
             fails += 1
+            if len(fail_msgs) < 4:
+                fail_msgs.append(f"{kind} n={r['n']} dim={r['dim']}: status {info['status']}: {info['message']}")
             continue
         lat[kind].append((info["t_end"] - info["t_submit"]) * 1e3)
         svc[kind].append((info["t_end"] - info["t_start"]) * 1e3)
@@ -388,7 +391,8 @@ def serve_session(reqs, precision, budget, slots, log_tag, trace_path=None, exac
         b.free()
     Xpool.free()
     allv = np.array(lat["build"] + lat["apply"])
-    res = {"requests": len(ids), "failed": fails, "wall_s": wall, "throughput_rps": len(allv) / wall,
+    res = {"requests": len(ids), "failed": fails, "failed_messages": fail_msgs, "wall_s": wall,
+           "throughput_rps": len(allv) / wall,
            "p50_ms": float(np.percentile(allv, 50)), "p99_ms": float(np.percentile(allv, 99)),
            "apply_p50_ms": float(np.percentile(lat["apply"], 50)) if lat["apply"] else None,
            "apply_p99_ms": float(np.percentile(lat["apply"], 99)) if lat["apply"] else None,
@@ -789,7 +793,7 @@ def main():
                                    "dense product by up to 0.6 (oracle and engine alike)",
                   "sweep": []}
         keys = ("throughput_rps", "p50_ms", "p99_ms", "apply_p50_ms", "apply_p99_ms", "build_p99_ms",
-                "failed", "rel_err_exact_rows_max", "rel_err_vs_fp64_hmvp_max")
+                "failed", "failed_messages", "rel_err_exact_rows_max", "rel_err_vs_fp64_hmvp_max")
         for qps in loads:
             reqs = serving_requests(5001, 5002, qps, args.serve_seconds)
             point = {"qps": qps, "requests": len(reqs)}

@@ -133,7 +133,8 @@ class BuildRequest(ctypes.Structure):
 class RequestInfo(ctypes.Structure):
     """hmg_request_info (include/hmglib_server.h)."""
     _fields_ = [("status", ctypes.c_int), ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
-                ("t_end", ctypes.c_double), ("merged", ctypes.c_int), ("matrix", ctypes.c_int64)]
+                ("t_end", ctypes.c_double), ("merged", ctypes.c_int), ("matrix", ctypes.c_int64),
+                ("message", ctypes.c_char * 160)]
 
 
 # name -> (restype, argtypes) for every symbol declared in include/*.h

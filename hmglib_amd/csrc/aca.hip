@@ -11,6 +11,8 @@
 //   Residuals: v_j = K(i_r, j) - sum_{l<r} U[i_r,l] V[j,l], u_i likewise, each
 //   accumulated as fma(-s, w, val) in l order (nvcc's contraction of :955/:1143).
 // The 2-norm is a wavefront FP64 reduction; the argmax a (|v|, index, v) butterfly.
+#include <algorithm>
+
 #include "hmg_internal.h"
 #include "kernel_eval.h"
 
@@ -857,33 +859,37 @@ hmg_status aca_build_mixed(Engine& eng, const Coords& c1, const Coords& c2, cons
     if (na == 0) return HMG_OK;
     hipStream_t s = eng.stream;
     const int64_t cap = std::max<int64_t>(aca_mixed_scratch(eng), 1);
-    // batches: blocks dealt round-robin (each batch mixes large and small blocks);
-    // more batches until every batch's factors fit the scratch
-    int64_t total = 0, largest = 0;
-    for (int a = 0; a < na; a++) {
-        total += block_scratch(eng, a);
-        largest = std::max(largest, block_scratch(eng, a));
-    }
+    // batches filled in order of decreasing block size: a batch lasts as long as its
+    // longest block (the rank loop and rejected-row search of the largest blocks),
+    // so the largest blocks run together in the first batches instead of one of
+    // them ending every batch
+    int64_t largest = 0;
+    for (int a = 0; a < na; a++) largest = std::max(largest, block_scratch(eng, a));
     const int64_t scratch = std::max(cap, largest);
-    int nb = (int)std::max<int64_t>(1, (total + scratch - 1) / scratch);
-    std::vector<int64_t> suoff(na), svoff(na), used;
+    std::vector<int> by_size(na);
+    for (int a = 0; a < na; a++) by_size[a] = a;
+    std::stable_sort(by_size.begin(), by_size.end(),
+                     [&](int x, int y) { return block_scratch(eng, x) > block_scratch(eng, y); });
+    std::vector<int64_t> suoff(na), svoff(na);
     std::vector<int> ldu(na), ldv(na);
-    for (;; nb++) {
-        used.assign(nb, 0);
-        bool fits = true;
-        for (int a = 0; a < na && fits; a++) {
-            const int4 b = eng.h_blk[eng.ndense + a];
-            const int m1 = b.y - b.x + 1, m2 = b.w - b.z + 1, k = eng.h_kb[a];
-            int64_t& u = used[a % nb];
-            suoff[a] = u;
-            svoff[a] = u + (int64_t)m1 * k;
-            ldu[a] = m1;
-            ldv[a] = std::max(k, 1);
-            u += (int64_t)(m1 + m2) * k;
-            fits = u <= scratch;
+    std::vector<std::vector<int>> members(1);
+    int64_t used = 0;
+    for (const int a : by_size) {
+        const int4 b = eng.h_blk[eng.ndense + a];
+        const int m1 = b.y - b.x + 1, m2 = b.w - b.z + 1, k = eng.h_kb[a];
+        const int64_t need = (int64_t)(m1 + m2) * k;
+        if (used + need > scratch && !members.back().empty()) {
+            members.emplace_back();
+            used = 0;
         }
-        if (fits) break;
+        members.back().push_back(a);
+        suoff[a] = used;
+        svoff[a] = used + (int64_t)m1 * k;
+        ldu[a] = m1;
+        ldv[a] = std::max(k, 1);
+        used += need;
     }
+    const int nb = (int)members.size();
     DevBuf<double> S;
     HMG_TRY(S.alloc((size_t)scratch));
     DevBuf<int64_t> d_suoff, d_svoff;
@@ -900,7 +906,7 @@ hmg_status aca_build_mixed(Engine& eng, const Coords& c1, const Coords& c2, cons
         bptr[q] = (int)list.size();
         for (int cls = 0; cls < 3; cls++) {
             const int before = (int)list.size();
-            for (int a = q; a < na; a += nb)
+            for (const int a : members[q])
                 if (aca_class(eng, a) == cls) list.push_back(a);
             if (cls == 0) bhuge[q] = (int)list.size() - before;
             if (cls == 1) bbig[q] = (int)list.size() - before;

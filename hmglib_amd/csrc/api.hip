@@ -1563,7 +1563,8 @@ int64_t hmg_struct_layout(int which, int field)
     case 8: {
         const int64_t offs[] = {OFF(hmg_request_info, status), OFF(hmg_request_info, t_submit),
                                 OFF(hmg_request_info, t_start), OFF(hmg_request_info, t_end),
-                                OFF(hmg_request_info, merged), OFF(hmg_request_info, matrix)};
+                                OFF(hmg_request_info, merged), OFF(hmg_request_info, matrix),
+                                OFF(hmg_request_info, message)};
         if (field < 0) return (int64_t)sizeof(hmg_request_info);
         if (field < (int)(sizeof(offs) / sizeof(offs[0]))) return offs[field];
         return -1;

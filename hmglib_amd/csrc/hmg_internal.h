@@ -488,7 +488,7 @@ struct Engine {
     // bit 0 V^T X, bit 1 U T
     int lr_prio = 0;
     // mixed-precision ACA: its FP64 scratch holds 1/aca_mixed_parts of the factors
-    int aca_mixed_parts = 4;
+    int aca_mixed_parts = 16;
     // unused LDS reserved by every workgroup of the main ACA launch (caps the blocks
     // in flight per compute unit)
     int aca_lds_pad = 0;

@@ -43,7 +43,7 @@ struct Request {
     const double* X = nullptr;
     double* Y = nullptr;
     int nrhs = 0;
-    hmg_request_info info{-1, 0.0, -1.0, -1.0, 0, 0};
+    hmg_request_info info{-1, 0.0, -1.0, -1.0, 0, 0, {0}};
     bool done = false;
 };
 
@@ -103,6 +103,10 @@ void finish(hmg_server* s, Request* r, int status, double t_end)
 {
     r->info.status = status;
     r->info.t_end = t_end;
+    if (status != HMG_OK) {
+        // the failing call's message (set on this worker thread)
+        std::snprintf(r->info.message, sizeof(r->info.message), "%s", hmg_last_error());
+    }
     r->done = true;
     if (status == HMG_OK)
         s->n_done++;

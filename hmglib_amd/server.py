@@ -66,7 +66,7 @@ class Server:
     @staticmethod
     def _info(i):
         return {"status": i.status, "t_submit": i.t_submit, "t_start": i.t_start, "t_end": i.t_end,
-                "merged": i.merged, "matrix": i.matrix}
+                "merged": i.merged, "matrix": i.matrix, "message": i.message.decode(errors="replace")}
 
     def wait(self, request):
         i = _abi.RequestInfo()

@@ -271,7 +271,7 @@ hmg_status hmg_get_schedule(const struct h_matrix_data* data, int64_t out[11]);
  * HMG_OPT_LR_PRIORITY [0]: mixed precision: raises the issue priority of the
  *   low-rank wavefronts over the near field's that run beside them on the same
  *   compute units, bit 0 for V^T X, bit 1 for U T (results unchanged).
- * HMG_OPT_MIXED_ACA_PARTS [4]: mixed precision: the FP64 ACA runs in batches whose
+ * HMG_OPT_MIXED_ACA_PARTS [16]: mixed precision: the FP64 ACA runs in batches whose
  *   FP64 factors (before the split into BF16 panels) fill at most 1/parts of the
  *   factors' size (at least 1 GiB); results unchanged.
  * HMG_OPT_ACA_LDS_BYTES [0]: unused LDS each workgroup of the main ACA launch

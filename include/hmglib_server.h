@@ -69,6 +69,7 @@ typedef struct {
 	double t_end;               /* results complete on the device (-1 before) */
 	int merged;                 /* requests that shared this request's pass */
 	int64_t matrix;             /* the H-matrix the request built or applied */
+	char message[160];          /* why a failed request failed ("" otherwise) */
 } hmg_request_info;
 
 /* cfg may be NULL (defaults) */
