@@ -646,8 +646,18 @@ extern "C" hmg_status hmg_precompute_aca_batch(struct h_matrix_data* const* data
     // every set ends with its largest blocks, and the sets' tails overlap
     std::vector<std::vector<int>> jobs;
     for (const GroupKey& k : order) {
-        const std::vector<int>& g = groups[k];
+        std::vector<int>& g = groups[k];
         const bool mixed = engine_of(data[g[0]])->precision == HMG_PREC_MIXED;
+        if (mixed) {
+            // largest H-matrices first: a part lasts as long as its longest block,
+            // so the clouds with the longest blocks share the first parts
+            std::vector<int64_t> need_of(count, 0);
+            for (const int i : g) {
+                const Engine* e = engine_of(data[i]);
+                for (int a = 0; a < e->naca; a++) need_of[i] += scratch_of(*e, a);
+            }
+            std::stable_sort(g.begin(), g.end(), [&](int x, int y) { return need_of[x] > need_of[y]; });
+        }
         std::vector<int> part;
         int64_t used = 0;
         for (size_t q = 0; q <= g.size(); q++) {
