@@ -30,13 +30,23 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // val - sum_{l < r} p[l*sp] * q[l*sq], accumulated as fma(-p, q, val) in l order
 // (the reference's contraction order, linear_algebra.cu:955/:1143).  The loads of
-// eight consecutive l are issued before their FMAs, so a step waits for one
-// memory latency per eight terms instead of one per term.
+// sixteen (then eight) consecutive l are issued before their FMAs, so a step waits
+// for one memory latency per sixteen terms instead of one per term.
 template <class T>
 __device__ __forceinline__ double residual_update(double val, int r, const T* __restrict__ p, int64_t sp,
                                                   const T* __restrict__ q, int64_t sq)
 {
     int l = 0;
+    for (; l + 16 <= r; l += 16) {
+        T pv[16], qv[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            pv[e] = p[(int64_t)(l + e) * sp];
+            qv[e] = q[(int64_t)(l + e) * sq];
+        }
+#pragma unroll
+        for (int e = 0; e < 16; e++) val = __fma_rn(-(double)pv[e], (double)qv[e], val);
+    }
     for (; l + 8 <= r; l += 8) {
         T pv[8], qv[8];
 #pragma unroll
@@ -607,13 +617,13 @@ __device__ void aca_block(const Team<NW>& team, Coords c1, Coords c2, const Kern
 // one wavefront per block of list[] (indices into the ACA blocks)
 // One launch for every admissible block: nbig workgroups each run the ACA of one
 // large block (list[0 .. nbig)) with all kAcaTeam wavefronts; every other
-// workgroup runs kAcaTeam small blocks (list[nbig ..)), one per wavefront.  Four
-// wavefronts per SIMD: the rank loop is latency-bound, so occupancy pays.  The
-// tolerance stop's cross terms need more registers: three wavefronts per SIMD
-// there, which keeps them out of scratch memory.
+// workgroup runs kAcaTeam small blocks (list[nbig ..)), one per wavefront.  Three
+// wavefronts per SIMD: the rank loop is latency-bound, and 168 registers let each
+// wavefront keep sixteen rank terms' loads in flight (and the tolerance stop's
+// cross terms) without scratch memory.
 constexpr int kAcaTeam = 4;
 template <class T, int KIND, bool TOL>
-__global__ void __launch_bounds__(kAcaTeam * kWave) __attribute__((amdgpu_waves_per_eu(TOL ? 3 : 4)))
+__global__ void __launch_bounds__(kAcaTeam * kWave) __attribute__((amdgpu_waves_per_eu(3)))
 aca_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict__ blk,
            const work_item* __restrict__ items, const int* __restrict__ list,
            int nbig, int nsmall, const int* __restrict__ kb, const int64_t* __restrict__ uoff,
