@@ -298,8 +298,9 @@ def test_lr_workgroup_form_bitwise_equal(gpu, case):
 def test_tuning_options_leave_results_unchanged(gpu):
     """The low-rank issue priority, the batch count of the mixed ACA and the ACA's
     LDS reservation change launch shapes only: ranks, factors and products stay
-    bitwise the same (the mixed ACA's batches deal blocks round-robin into a
-    bounded FP64 scratch; each block's ACA is the same arithmetic in any batch)."""
+    bitwise the same (the mixed ACA's batches fill a bounded FP64 scratch in order
+    of decreasing block size; each block's ACA is the same arithmetic in any
+    batch)."""
     from hmglib_amd import hip
     n = 30000
     X = _points(1301, 3, n)
@@ -335,6 +336,35 @@ def test_tuning_options_leave_results_unchanged(gpu):
     hip.trim_pool()
     f = hip.mem_info()[0]
     assert f > 0
+
+
+def test_mixed_aca_in_many_batches_same_as_one(gpu):
+    """At N = 2^17 (3D, leaf 256, eta 1.0, rank 24: 1.2e9 FP64 factor elements) the
+    mixed build's FP64 ACA runs in 10 batches of at least 2^27 elements by default
+    and in one batch with HMG_OPT_MIXED_ACA_PARTS = 1: ranks and the 32-RHS
+    products are bitwise the same, and both agree with the FP64 H-matrix to the
+    BF16x3 precision."""
+    n = 1 << 17
+    X = O.uniform(2001, 3 * n).reshape(3, n)
+    R = 32
+    Xr = O.normal(1403, n * R).reshape(n, R, order="F")
+    out = {}
+    for parts in (16, 1, 0):
+        hm = gpu.HMatrix(X, X, 20, 1.0, 50, 256, 24)
+        hm.setup()
+        kern = gpu.Kernel(O.K_EXPONENTIAL, 0.0, 0.1)
+        hm.set_kernel(kern)
+        if parts:
+            hm.set_precision(gpu.PREC_MIXED)
+            hm.set_option(gpu.OPT_MIXED_ACA_PARTS, parts)
+        hm.precompute_aca()
+        out[parts] = (hm.work()["rank_sum"], hm.apply(Xr))
+        hm.destroy()
+        kern.free()
+    assert out[16][0] == out[1][0] == out[0][0]
+    np.testing.assert_array_equal(out[16][1], out[1][1])
+    for c in range(R):
+        assert _rel(out[16][1][:, c], out[0][1][:, c]) < 4e-5
 
 
 def test_released_large_buffers_are_reused_and_returned(gpu):
