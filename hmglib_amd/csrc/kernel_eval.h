@@ -41,6 +41,62 @@ __device__ __forceinline__ double kernel_of_r2_k(const KernelParams& kp, double 
     return __builtin_nan("");
 }
 
+// e^x for x <= 0 (the argument of the product kernels' entries), within 1 ulp:
+// Cody-Waite reduction x = n ln2 + r with ln2 split into two doubles (its double
+// and the double of the remainder), |r| <= ln2/2, the degree-13 Taylor polynomial
+// of e^r (truncation below 5e-18 relative) by Horner's rule, scaled by 2^n.
+// Arguments below -800 give 0 (e^-800 is below the smallest double).
+__device__ __forceinline__ double exp_nonpos(double x)
+{
+    x = fmax(x, -800.0);
+    const double n = rint(x * 1.4426950408889634);
+    double r = __fma_rn(n, -0.6931471805599453, x);
+    r = __fma_rn(n, -2.3190468138462996e-17, r);
+    double p = 1.6059043836821613e-10;  // 1/13!
+    p = __fma_rn(p, r, 2.08767569878681e-09);
+    p = __fma_rn(p, r, 2.505210838544172e-08);
+    p = __fma_rn(p, r, 2.755731922398589e-07);
+    p = __fma_rn(p, r, 2.7557319223985893e-06);
+    p = __fma_rn(p, r, 2.48015873015873e-05);
+    p = __fma_rn(p, r, 0.0001984126984126984);
+    p = __fma_rn(p, r, 0.001388888888888889);
+    p = __fma_rn(p, r, 0.008333333333333333);
+    p = __fma_rn(p, r, 0.041666666666666664);
+    p = __fma_rn(p, r, 0.16666666666666666);
+    p = __fma_rn(p, r, 0.5);
+    p = __fma_rn(p, r, 1.0);
+    p = __fma_rn(p, r, 1.0);
+    return ldexp(p, (int)n);
+}
+
+// sqrt(x) for finite x >= 0: the hardware reciprocal square root refined by a
+// Goldschmidt step and two Newton corrections of the root (g += (x - g^2) h)
+__device__ __forceinline__ double sqrt_nonneg(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __fma_rn(-g, h, 0.5);
+    g = __fma_rn(g, r, g);
+    h = __fma_rn(h, r, h);
+    double d = __fma_rn(-g, g, x);
+    g = __fma_rn(d, h, g);
+    d = __fma_rn(-g, g, x);
+    g = __fma_rn(d, h, g);
+    return x == 0.0 ? 0.0 : g;
+}
+
+// The entries of the FP64 products (near field, dense rows, stored near field):
+// the Gaussian and exponential kernels through exp_nonpos / sqrt_nonneg (fewer
+// instructions than the general functions: no overflow, scaling or class
+// handling), the reference kernels as above.  The ACA keeps kernel_of_r2_k.
+template <int KIND>
+__device__ __forceinline__ double kernel_of_r2_prod(const KernelParams& kp, double r2)
+{
+    if (KIND == HMG_KERNEL_GAUSSIAN) return exp_nonpos(-(r2 * kp.c0));
+    if (KIND == HMG_KERNEL_EXPONENTIAL) return exp_nonpos(-(sqrt_nonneg(r2) * kp.c0));
+    return kernel_of_r2_k<KIND>(kp, r2);
+}
+
 __device__ __forceinline__ double kernel_of_r2(const KernelParams& kp, double r2)
 {
     switch (kp.kind) {

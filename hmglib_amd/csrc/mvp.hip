@@ -135,14 +135,7 @@ lr_u_kernel(const LrItem* __restrict__ items, int nitems, const double* __restri
 template <int KIND>
 __device__ __forceinline__ double kernel_value(const KernelParams& kp, double r2)
 {
-    if (KIND == HMG_KERNEL_GAUSSIAN) return exp(-(r2 * kp.c0));
-    if (KIND == HMG_KERNEL_EXPONENTIAL) return exp(-(sqrt(r2) * kp.c0));
-    if (KIND == HMG_KERNEL_GAUSSIAN_REF) {
-        const double r = sqrt(r2);
-        return exp(-(r * r));
-    }
-    const double r = sqrt(r2);
-    return (yn(1, r + 1.0e-15) * r) / kp.c1;
+    return kernel_of_r2_prod<KIND>(kp, r2);
 }
 
 // The near field of one work item (up to 128 rows of a row cluster, one or two

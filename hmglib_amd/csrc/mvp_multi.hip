@@ -308,7 +308,7 @@ near_multi_kernel(Coords c1, Coords c2, KernelParams kp, const int4* __restrict_
                             r2 = __fma_rn(dd, dd, r2);
                         }
                     }
-                    double kv = kernel_of_r2_k<KIND>(kp, r2);
+                    double kv = kernel_of_r2_prod<KIND>(kp, r2);
                     if (row[t] == cc) kv += kp.reg;
                     a[t] = ok ? kv : 0.0;
                 }
