@@ -603,3 +603,26 @@ def test_aca_tolerance_reference_mode(gpu):
     assert abs(w["rank_sum"] - int(keff.sum())) <= 5e-3 * keff.sum()
     assert keff.sum() < 0.9 * om.aca_factors()["kb"].sum()
     assert _rel(hm.apply(x), om.hmvp(ok, x)) < 1e-3
+
+
+@pytest.mark.parametrize("kind,ls,dim", [(O.K_EXPONENTIAL, 0.1, 3), (O.K_GAUSSIAN, 0.1, 2),
+                                         (O.K_EXPONENTIAL, 0.01, 2), (O.K_GAUSSIAN, 0.015, 3)])
+def test_product_entries_within_two_ulps_of_glibc(gpu, kind, ls, dim):
+    """The FP64 products evaluate Gaussian and exponential entries with their own exp
+    (non-positive arguments) and sqrt (non-negative): the dense product of a one-hot
+    vector is one column of K with no summation, so each entry must be within 2 ulps
+    of the oracle's (glibc exp and sqrt), underflow and the diagonal's exact 1
+    included."""
+    n = 3000
+    X = _points(4901 + dim, dim, n)
+    hm, kern, om, ok = _pair(gpu, X, X, 20 if dim == 3 else 32, 1.5, 64, 8, kind, ls=ls, precompute=False)
+    for j in (0, 1234, n - 1):
+        x = np.zeros(n)
+        x[j] = 1.0
+        y = hm.apply_full(x)
+        yo = om.dense_mvp(ok, x)
+        assert yo[j] == 1.0 and y[j] == 1.0
+        ulps = np.abs(y - yo) / np.spacing(np.abs(yo))
+        assert float(np.max(ulps)) <= 2.0, float(np.max(ulps))
+        if kind == O.K_GAUSSIAN and ls < 0.1:
+            assert (yo == 0.0).any()  # arguments below -745: both give exactly 0
